@@ -1,0 +1,946 @@
+// Host-side drivers of the device algorithms (see algos.hpp).
+//
+// The estimator's bookkeeping (sign matrices, the parallel-column tests, the
+// sort of h) is O(n*t) and stays on the host, in the reference's order and
+// with the reference's random draws, so the probes it chooses are the same;
+// the operator products it drives run on the device. The exponential's and
+// the action's branch decisions read back a handful of scalars; their
+// products, norms and element-wise passes are device kernels.
+#include "algos.hpp"
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "blas1.hpp"
+#include "tns_scalar.hpp"
+
+namespace tns {
+
+double read_double(Context* ctx, const double* d)
+{
+    double h = 0.0;
+    TNS_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+}
+
+void read_doubles(Context* ctx, const double* d, double* h, int count)
+{
+    TNS_HIP(hipMemcpyAsync(h, d, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// System.Random(seed): Knuth's subtractive generator (TAOCP vol. 2, 3.6), in
+// the form .NET keeps for seeded instances; Next(k) = (int)(Sample() * k).
+// ---------------------------------------------------------------------------
+class SeededRandom {
+public:
+    explicit SeededRandom(int seed)
+    {
+        const int mbig = INT_MAX;
+        int subtraction = (seed == INT_MIN) ? INT_MAX : std::abs(seed);
+        int mj = 161803398 - subtraction;
+        s_[55] = mj;
+        int mk = 1, ii = 0;
+        for (int i = 1; i < 55; i++) {
+            if ((ii += 21) >= 55) ii -= 55;
+            s_[ii] = mk;
+            mk = mj - mk;
+            if (mk < 0) mk += mbig;
+            mj = s_[ii];
+        }
+        for (int k = 1; k < 5; k++)
+            for (int i = 1; i < 56; i++) {
+                int n = i + 30;
+                if (n >= 55) n -= 55;
+                s_[i] -= s_[1 + n];
+                if (s_[i] < 0) s_[i] += mbig;
+            }
+        a_ = 0;
+        b_ = 21;
+    }
+    double sample()
+    {
+        if (++a_ >= 56) a_ = 1;
+        if (++b_ >= 56) b_ = 1;
+        int v = s_[a_] - s_[b_];
+        if (v == INT_MAX) v--;
+        if (v < 0) v += INT_MAX;
+        s_[a_] = v;
+        return v * (1.0 / INT_MAX);
+    }
+    int next(int max_value) { return (int)(sample() * max_value); }
+
+private:
+    int s_[56] = {0};
+    int a_ = 0, b_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Array.Sort(double[] keys, int[] items): NaNs first, then introspective sort
+// (median-of-three quicksort, insertion sort at <= 16, heapsort past depth
+// 2*(floor(log2 n)+1)). Equal keys keep the order this procedure gives them,
+// which decides which of several equal h values the estimator probes.
+// ---------------------------------------------------------------------------
+struct KeySort {
+    double* k;
+    int* v;
+    void swap(int i, int j)
+    {
+        std::swap(k[i], k[j]);
+        std::swap(v[i], v[j]);
+    }
+    void swap_if_greater(int i, int j)
+    {
+        if (k[i] > k[j]) swap(i, j);
+    }
+    void insertion(int lo, int n)
+    {
+        for (int i = 0; i < n - 1; i++) {
+            double t = k[lo + i + 1];
+            int tv = v[lo + i + 1];
+            int j = i;
+            while (j >= 0 && t < k[lo + j]) {
+                k[lo + j + 1] = k[lo + j];
+                v[lo + j + 1] = v[lo + j];
+                j--;
+            }
+            k[lo + j + 1] = t;
+            v[lo + j + 1] = tv;
+        }
+    }
+    void down_heap(int lo, int i, int n)
+    {
+        double d = k[lo + i - 1];
+        int dv = v[lo + i - 1];
+        while (i <= n >> 1) {
+            int child = 2 * i;
+            if (child < n && k[lo + child - 1] < k[lo + child]) child++;
+            if (!(d < k[lo + child - 1])) break;
+            k[lo + i - 1] = k[lo + child - 1];
+            v[lo + i - 1] = v[lo + child - 1];
+            i = child;
+        }
+        k[lo + i - 1] = d;
+        v[lo + i - 1] = dv;
+    }
+    void heap(int lo, int n)
+    {
+        for (int i = n >> 1; i >= 1; i--) down_heap(lo, i, n);
+        for (int i = n; i > 1; i--) {
+            swap(lo, lo + i - 1);
+            down_heap(lo, 1, i - 1);
+        }
+    }
+    int partition(int lo, int n)
+    {
+        int hi = n - 1, mid = hi >> 1;
+        swap_if_greater(lo, lo + mid);
+        swap_if_greater(lo, lo + hi);
+        swap_if_greater(lo + mid, lo + hi);
+        double pivot = k[lo + mid];
+        swap(lo + mid, lo + hi - 1);
+        int left = 0, right = hi - 1;
+        while (left < right) {
+            while (k[lo + ++left] < pivot) {
+            }
+            while (pivot < k[lo + --right]) {
+            }
+            if (left >= right) break;
+            swap(lo + left, lo + right);
+        }
+        if (left != hi - 1) swap(lo + left, lo + hi - 1);
+        return left;
+    }
+    void intro(int lo, int n, int depth)
+    {
+        int size = n;
+        while (size > 1) {
+            if (size <= 16) {
+                if (size == 2) {
+                    swap_if_greater(lo, lo + 1);
+                    return;
+                }
+                if (size == 3) {
+                    swap_if_greater(lo, lo + 1);
+                    swap_if_greater(lo, lo + 2);
+                    swap_if_greater(lo + 1, lo + 2);
+                    return;
+                }
+                insertion(lo, size);
+                return;
+            }
+            if (depth == 0) {
+                heap(lo, size);
+                return;
+            }
+            depth--;
+            int p = partition(lo, size);
+            intro(lo + p + 1, size - (p + 1), depth);
+            size = p;
+        }
+    }
+};
+
+void sort_keys_items(int n, double* keys, int* items)
+{
+    if (n <= 1) return;
+    int left = 0;
+    for (int i = 0; i < n; i++)
+        if (std::isnan(keys[i])) {
+            std::swap(keys[left], keys[i]);
+            std::swap(items[left], items[i]);
+            left++;
+        }
+    int m = n - left;
+    if (m <= 1) return;
+    int log2n = 31 - __builtin_clz((unsigned)m);
+    KeySort s{keys + left, items + left};
+    s.intro(0, m, 2 * (log2n + 1));
+}
+
+// Host element rules of the estimator (ElementKernels.cs:125-170,
+// ComplexKernels.cs:110-130).
+inline double mag_h(double x) { return std::fabs(x); }
+inline double mag_h(double2 x) { return s_abs(x); }
+inline double sign_h(double x) { return x >= 0.0 ? 1.0 : -1.0; }
+inline double2 sign_h(double2 x)
+{
+    if (x.x == 0.0 && x.y == 0.0) return make_double2(1.0, 0.0);
+    double m = s_abs(x);
+    return make_double2(x.x / m, x.y / m);
+}
+inline bool eq_h(double a, double b) { return a == b; }
+inline bool eq_h(double2 a, double2 b) { return a.x == b.x && a.y == b.y; }
+inline double neg_h(double a) { return -a; }
+inline double2 neg_h(double2 a) { return make_double2(-a.x, -a.y); }
+inline double from_h(double x, double) { return x; }
+inline double2 from_h(double x, double2) { return make_double2(x, 0.0); }
+template <typename T>
+constexpr bool discrete_signs()
+{
+    return sizeof(T) == 8;
+}
+
+template <typename T>
+bool is_parallel(const T* u, const T* v, int n)
+{
+    bool same = true, opposite = true;
+    for (int i = 0; i < n; i++) {
+        if (!eq_h(u[i], v[i])) same = false;
+        if (!eq_h(u[i], neg_h(v[i]))) opposite = false;
+        if (!same && !opposite) return false;
+    }
+    return true;
+}
+
+template <typename T>
+bool parallel_to_any(int n, const T* col, const T* panel, int count)
+{
+    for (int j = 0; j < count; j++)
+        if (is_parallel(col, panel + (size_t)j * n, n)) return true;
+    return false;
+}
+
+template <typename T>
+void fill_signs(SeededRandom& rng, T* x, int n)
+{
+    for (int i = 0; i < n; i++) x[i] = from_h(rng.next(2) == 0 ? -1.0 : 1.0, T{});
+}
+
+template <typename T>
+double column_one_norm(const T* x, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += mag_h(x[i]);
+    return s;
+}
+
+template <typename T>
+void upload(Context* ctx, T* d, const std::vector<T>& h)
+{
+    TNS_HIP(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+}
+
+template <typename T>
+void download(Context* ctx, std::vector<T>& h, const T* d)
+{
+    TNS_HIP(hipMemcpyAsync(h.data(), d, sizeof(T) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+template <typename T>
+void gemm_t(Context* ctx, int m, int n, int k, double alpha, const T* a, int lda, const T* b, int ldb, double beta,
+            T* c, int ldc)
+{
+    if constexpr (sizeof(T) == 8)
+        dgemm(ctx, m, n, k, alpha, (const double*)a, lda, (const double*)b, ldb, beta, (double*)c, ldc);
+    else
+        zgemm(ctx, m, n, k, make_double2(alpha, 0.0), (const double2*)a, lda, (const double2*)b, ldb,
+              make_double2(beta, 0.0), (double2*)c, ldc);
+}
+
+template <typename T>
+void norm_t(Context* ctx, int kind, int m, int n, const T* a, int lda, double* d_out)
+{
+    if constexpr (sizeof(T) == 8) dnorm(ctx, kind, m, n, (const double*)a, lda, d_out);
+    else znorm(ctx, kind, m, n, (const double2*)a, lda, d_out);
+}
+
+template <typename T>
+void panel_t(Context* ctx, int trans, int n, int t, const T* a, int lda, const T* x, int ldx, T* y, int ldy)
+{
+    if constexpr (sizeof(T) == 8)
+        dpanel(ctx, trans, n, t, (const double*)a, lda, (const double*)x, ldx, (double*)y, ldy);
+    else
+        zpanel(ctx, trans, n, t, (const double2*)a, lda, (const double2*)x, ldx, (double2*)y, ldy);
+}
+
+template <typename T>
+void copy_t(Context* ctx, int m, int n, const T* src, int lds, T* dst, int ldd)
+{
+    if constexpr (sizeof(T) == 8) dcopy2d(ctx, m, n, (const double*)src, lds, (double*)dst, ldd);
+    else zcopy2d(ctx, m, n, (const double2*)src, lds, (double2*)dst, ldd);
+}
+
+template <typename T>
+void getrf_t(Context* ctx, int n, T* a, int lda, int* piv, int* perm, int* info)
+{
+    if constexpr (sizeof(T) == 8) dgetrf(ctx, n, n, (double*)a, lda, 0, piv, perm, info);
+    else zgetrf(ctx, n, n, (double2*)a, lda, 0, piv, perm, info);
+}
+
+template <typename T>
+void getrs_t(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu, const int* perm, T* b, int ldb)
+{
+    if constexpr (sizeof(T) == 8) dgetrs(ctx, trans, n, nrhs, (const double*)lu, ldlu, perm, (double*)b, ldb);
+    else zgetrs(ctx, trans, n, nrhs, (const double2*)lu, ldlu, perm, (double2*)b, ldb);
+}
+
+int ceil_int(double x)
+{
+    // (int)Math.Ceiling(x) on .NET: NaN and out-of-range saturate.
+    if (std::isnan(x)) return INT_MIN;
+    double c = std::ceil(x);
+    if (c >= 2147483647.0) return INT_MAX;
+    if (c <= -2147483648.0) return INT_MIN;
+    return (int)c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// NormEstimate.Estimate (NormEstimate.cs:104-274).
+// ---------------------------------------------------------------------------
+template <typename T>
+NormEst normest(Context* ctx, const DevOp<T>& op, int columns, int max_iterations, int seed)
+{
+    NormEst res;
+    const int n = op.n;
+    if (n <= 0) return res;
+    const int t = columns < 1 ? 1 : (columns > n ? n : columns);
+    const int itmax = max_iterations > 2 ? max_iterations : 2;
+    if ((long long)n * t > INT_MAX) fail(TNS_ERANGE, "probe panel exceeds a buffer's extent");
+    const size_t extent = (size_t)n * t;
+
+    std::vector<T> x(extent), y(extent), s(extent), s_old(extent), z(extent);
+    std::vector<double> h(n), keys(n);
+    std::vector<int> order(n), scratch(n), current(t);
+    std::vector<char> used(n, 0);
+    DevBuf<T> dx(ctx, extent), dy(ctx, extent);
+
+    SeededRandom rng(seed);
+    const T one = from_h(1.0, T{});
+    for (int i = 0; i < n; i++) x[i] = one;
+    for (int j = 1; j < t; j++)
+        for (int attempt = 0; attempt < 8; attempt++) {
+            fill_signs(rng, x.data() + (size_t)j * n, n);
+            if (!parallel_to_any(n, x.data() + (size_t)j * n, x.data(), j)) break;
+        }
+    const T inv = from_h(1.0 / n, T{});
+    for (size_t i = 0; i < extent; i++) x[i] = s_mul(x[i], inv);
+
+    double est = 0.0, est_old = 0.0;
+    int best_index = -1, iterations = 0, products = 0;
+    for (int k = 1;; k++) {
+        iterations = k;
+        upload(ctx, dx.get(), x);
+        op.apply(t, dx.get(), dy.get());
+        download(ctx, y, dy.get());
+        products += t;
+
+        int argmax = 0;
+        est = column_one_norm(y.data(), n);
+        for (int j = 1; j < t; j++) {
+            double v = column_one_norm(y.data() + (size_t)j * n, n);
+            if (v > est) {
+                est = v;
+                argmax = j;
+            }
+        }
+        if (k >= 2 && (est > est_old || k == 2)) best_index = current[argmax];
+        if (k >= 2 && est <= est_old) {
+            est = est_old;
+            break;
+        }
+        est_old = est;
+        if (k > itmax) break;
+
+        std::swap(s, s_old);
+        for (size_t i = 0; i < extent; i++) s[i] = sign_h(y[i]);
+
+        if (discrete_signs<T>() && t > 1) {
+            bool all = true;
+            for (int j = 0; j < t && all; j++)
+                if (!parallel_to_any(n, s.data() + (size_t)j * n, s_old.data(), t)) all = false;
+            if (all) break;
+            for (int j = 0; j < t; j++)
+                for (int attempt = 0; attempt < 8; attempt++) {
+                    if (!parallel_to_any(n, s.data() + (size_t)j * n, s.data(), j) &&
+                        !parallel_to_any(n, s.data() + (size_t)j * n, s_old.data(), t))
+                        break;
+                    fill_signs(rng, s.data() + (size_t)j * n, n);
+                }
+        }
+
+        upload(ctx, dx.get(), s);
+        op.adjoint(t, dx.get(), dy.get());
+        download(ctx, z, dy.get());
+        products += t;
+
+        for (int i = 0; i < n; i++) {
+            double best = 0.0;
+            for (int j = 0; j < t; j++) best = std::fmax(best, mag_h(z[(size_t)j * n + i]));
+            h[i] = best;
+        }
+        if (k >= 2 && best_index >= 0) {
+            double largest = 0.0;
+            for (int i = 0; i < n; i++) largest = std::fmax(largest, h[i]);
+            if (largest == h[best_index]) break;
+        }
+        for (int i = 0; i < n; i++) {
+            keys[i] = -h[i];
+            order[i] = i;
+        }
+        sort_keys_items(n, keys.data(), order.data());
+        if (t > 1) {
+            bool all_used = true;
+            for (int j = 0; j < t; j++)
+                if (!used[order[j]]) {
+                    all_used = false;
+                    break;
+                }
+            if (all_used) break;
+            int at = 0;
+            for (int i = 0; i < n; i++)
+                if (!used[order[i]]) scratch[at++] = order[i];
+            for (int i = 0; i < n; i++)
+                if (used[order[i]]) scratch[at++] = order[i];
+            order.swap(scratch);
+        }
+        for (size_t i = 0; i < extent; i++) x[i] = from_h(0.0, T{});
+        for (int j = 0; j < t; j++) {
+            int idx = order[j];
+            current[j] = idx;
+            used[idx] = 1;
+            x[(size_t)j * n + idx] = one;
+        }
+    }
+    res.value = est;
+    res.iterations = iterations;
+    res.products = products;
+    return res;
+}
+
+template <typename T>
+DevOp<T> power_operator(Context* ctx, int n, const T* a, int lda, int power)
+{
+    DevOp<T> op;
+    op.n = n;
+    auto run = [ctx, n, a, lda, power](int trans, int t, const T* x, T* y) {
+        panel_t<T>(ctx, trans, n, t, a, lda, x, n, y, n);
+        if (power == 1) return;
+        DevBuf<T> staging(ctx, (size_t)n * t);
+        for (int step = 1; step < power; step++) {
+            copy_t<T>(ctx, n, t, y, n, staging.get(), n);
+            panel_t<T>(ctx, trans, n, t, a, lda, staging.get(), n, y, n);
+        }
+    };
+    op.apply = [run](int t, const T* x, T* y) { run(0, t, x, y); };
+    op.adjoint = [run](int t, const T* x, T* y) { run(1, t, x, y); };
+    return op;
+}
+
+template <typename T>
+DevOp<T> inverse_operator(Context* ctx, int n, const T* lu, int ldlu, const int* d_perm)
+{
+    DevOp<T> op;
+    op.n = n;
+    op.apply = [=](int t, const T* x, T* y) {
+        copy_t<T>(ctx, n, t, x, n, y, n);
+        getrs_t<T>(ctx, 0, n, t, lu, ldlu, d_perm, y, n);
+    };
+    op.adjoint = [=](int t, const T* x, T* y) {
+        copy_t<T>(ctx, n, t, x, n, y, n);
+        getrs_t<T>(ctx, 1, n, t, lu, ldlu, d_perm, y, n);
+    };
+    return op;
+}
+
+template <typename T>
+double gecon(Context* ctx, int n, const T* lu, int ldlu, const int* d_perm, int singular_column, double anorm,
+             int columns)
+{
+    if (singular_column >= 0 || anorm == 0.0 || n == 0) return 0.0;
+    NormEst r = normest<T>(ctx, inverse_operator<T>(ctx, n, lu, ldlu, d_perm), columns, 5, 20260915);
+    return r.value == 0.0 ? 0.0 : 1.0 / (anorm * r.value);
+}
+
+// ---------------------------------------------------------------------------
+// The matrix exponential (MatrixExponential.cs:181-505).
+// ---------------------------------------------------------------------------
+namespace {
+
+const double kPade3[] = {120, 60, 12, 1};
+const double kPade5[] = {30240, 15120, 3360, 420, 30, 1};
+const double kPade7[] = {17297280, 8648640, 1995840, 277200, 25200, 1512, 56, 1};
+const double kPade9[] = {17643225600.0, 8821612800.0, 2075673600.0, 302702400.0, 30270240.0,
+                         2162160.0,     110880.0,     3960.0,       90.0,         1.0};
+const double kPade13[] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
+                          129060195264000.0,   10559470521600.0,    670442572800.0,     33522128640.0,
+                          1323241920.0,        40840800.0,          960960.0,           16380.0,
+                          182.0,               1.0};
+constexpr double kUnitRoundoff = 1.1102230246251565e-16;
+
+double abs_coef_recip(int m)
+{
+    switch (m) {
+    case 3: return 100800.0;
+    case 5: return 10059033600.0;
+    case 7: return 4487938430976000.0;
+    case 9: return 5914384781877411840000.0;
+    default: return 113250775606021113483283660800000000.0;
+    }
+}
+
+template <typename T>
+struct Mat {
+    DevBuf<T> buf;
+    Mat(Context* ctx, int n) : buf(ctx, (size_t)n * n) {}
+    T* p() const { return buf.get(); }
+};
+
+template <typename T>
+double one_norm(Context* ctx, int n, const T* a)
+{
+    norm_t<T>(ctx, '1', n, n, a, n, ctx->d_scal);
+    return read_double(ctx, ctx->d_scal);
+}
+
+// ||(|A|)^p||_1 by the e^T |A|^p recurrence (MatrixExponential.cs:468-505).
+template <typename T>
+double abs_power_norm(Context* ctx, int n, const T* a, int p)
+{
+    DevBuf<double> mags(ctx, (size_t)n * n), row(ctx, (size_t)n), next(ctx, (size_t)n);
+    magnitudes<T>(ctx, (size_t)n * n, a, mags.get());
+    fill(ctx, n, row.get(), 1.0);
+    double* r = row.get();
+    double* q = next.get();
+    for (int step = 0; step < p; step++) {
+        rowvec(ctx, n, mags.get(), r, q);
+        std::swap(r, q);
+    }
+    vec_max(ctx, n, r, ctx->d_scal);
+    return read_double(ctx, ctx->d_scal);
+}
+
+// The ell correction (MatrixExponential.cs:416-449).
+template <typename T>
+int ell(Context* ctx, int n, const T* a, int m)
+{
+    double anorm = one_norm<T>(ctx, n, a);
+    if (anorm == 0.0) return 0;
+    double apn = abs_power_norm<T>(ctx, n, a, 2 * m + 1);
+    if (apn == 0.0) return 0;
+    double alpha = apn / (anorm * abs_coef_recip(m));
+    int v = ceil_int(std::log2(alpha / kUnitRoundoff) / (2 * m));
+    return v > 0 ? v : 0;
+}
+
+template <typename T>
+double power_norm_estimate(Context* ctx, int n, const T* a, int p)
+{
+    return normest<T>(ctx, power_operator<T>(ctx, n, a, n, p), 2, 5, 20260915).value;
+}
+
+template <typename T>
+void product(Context* ctx, int n, const T* x, const T* y, T* out)
+{
+    gemm_t<T>(ctx, n, n, n, 1.0, x, n, y, n, 0.0, out, n);
+}
+
+// V - U \ V + U into out (MatrixExponential.cs:372-405).
+template <typename T>
+void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out)
+{
+    Mat<T> den(ctx, n);
+    sub_add<T>(ctx, (size_t)n * n, u, v, den.p(), out);
+    DevBuf<int> piv(ctx, n), perm(ctx, n);
+    int* d_info = ctx->d_int;
+    getrf_t<T>(ctx, n, den.p(), n, piv.get(), perm.get(), d_info);
+    int info = 0;
+    TNS_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    if (info >= 0)
+        fail(TNS_ESTATE, "The Pade denominator is exactly singular at column " + std::to_string(info) +
+                             ", which indicates a non-finite input rather than a property of the algorithm.");
+    getrs_t<T>(ctx, 0, n, n, den.p(), n, perm.get(), out, n);
+}
+
+template <typename T>
+void low_degree(Context* ctx, const double* b, int m, int n, const T* a, const T* a2, const T* a4, const T* a6,
+                const T* a8, T* out)
+{
+    LinTerms<T> lw{}, lv{};
+    const T* pw[4] = {a2, a4, a6, a8};
+    int terms = (m - 1) / 2;  // 3:1, 5:2, 7:3, 9:4
+    for (int q = 0; q < terms; q++) {
+        lw.src[q] = pw[q];
+        lw.coef[q] = b[2 * q + 3];
+        lv.src[q] = pw[q];
+        lv.coef[q] = b[2 * q + 2];
+    }
+    Mat<T> w(ctx, n), u(ctx, n), v(ctx, n);
+    lincomb<T>(ctx, n, b[1], terms, lw, w.p());
+    product<T>(ctx, n, a, w.p(), u.p());
+    lincomb<T>(ctx, n, b[0], terms, lv, v.p());
+    solve_pade<T>(ctx, n, u.p(), v.p(), out);
+}
+
+// Degree 13 by Paterson-Stockmeyer (MatrixExponential.cs:323-360).
+template <typename T>
+void degree13(Context* ctx, int n, const T* a, const T* a2, const T* a4, const T* a6, T* out)
+{
+    const double* b = kPade13;
+    Mat<T> inner(ctx, n), w(ctx, n), u(ctx, n), v(ctx, n);
+    LinTerms<T> l{};
+    l.src[0] = a6; l.coef[0] = b[13];
+    l.src[1] = a4; l.coef[1] = b[11];
+    l.src[2] = a2; l.coef[2] = b[9];
+    lincomb<T>(ctx, n, 0.0, 3, l, inner.p());
+    l.src[0] = a2; l.coef[0] = b[3];
+    l.src[1] = a4; l.coef[1] = b[5];
+    l.src[2] = a6; l.coef[2] = b[7];
+    lincomb<T>(ctx, n, b[1], 3, l, w.p());
+    gemm_t<T>(ctx, n, n, n, 1.0, a6, n, inner.p(), n, 1.0, w.p(), n);
+    product<T>(ctx, n, a, w.p(), u.p());
+    l.src[0] = a6; l.coef[0] = b[12];
+    l.src[1] = a4; l.coef[1] = b[10];
+    l.src[2] = a2; l.coef[2] = b[8];
+    lincomb<T>(ctx, n, 0.0, 3, l, inner.p());
+    l.src[0] = a2; l.coef[0] = b[2];
+    l.src[1] = a4; l.coef[1] = b[4];
+    l.src[2] = a6; l.coef[2] = b[6];
+    lincomb<T>(ctx, n, b[0], 3, l, v.p());
+    gemm_t<T>(ctx, n, n, n, 1.0, a6, n, inner.p(), n, 1.0, v.p(), n);
+    solve_pade<T>(ctx, n, u.p(), v.p(), out);
+}
+
+}  // namespace
+
+template <typename T>
+void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* diag)
+{
+    if (diag) *diag = ExpmDiag{};
+    if (n <= 0) return;
+    Mat<T> a(ctx, n), a2(ctx, n), a4(ctx, n), a6(ctx, n), x(ctx, n);
+    copy_t<T>(ctx, n, n, ain, lda, a.p(), n);
+    if (n == 1) {
+        // Scalar exponential (MatrixExponential.cs:160-168).
+        T h{};
+        TNS_HIP(hipMemcpyAsync(&h, a.p(), sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        T r;
+        if constexpr (sizeof(T) == 8) r = std::exp(h);
+        else {
+            double ex = std::exp(h.x);
+            r = make_double2(ex * std::cos(h.y), ex * std::sin(h.y));
+        }
+        TNS_HIP(hipMemcpyAsync(e, &r, sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        return;
+    }
+    product<T>(ctx, n, a.p(), a.p(), a2.p());
+    product<T>(ctx, n, a2.p(), a2.p(), a4.p());
+    product<T>(ctx, n, a2.p(), a4.p(), a6.p());
+    norm_t<T>(ctx, '1', n, n, a4.p(), n, ctx->d_scal);
+    norm_t<T>(ctx, '1', n, n, a6.p(), n, ctx->d_scal + 1);
+    double nn[2];
+    read_doubles(ctx, ctx->d_scal, nn, 2);
+    const double d4 = std::pow(nn[0], 1.0 / 4.0), d6 = std::pow(nn[1], 1.0 / 6.0);
+    const double eta1 = std::fmax(d4, d6);
+    int degree = 13, s = 0;
+
+    if (eta1 < 1.495585217958292e-2 && ell<T>(ctx, n, a.p(), 3) == 0) {
+        degree = 3;
+        low_degree<T>(ctx, kPade3, 3, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
+    } else if (eta1 < 2.539398330063230e-1 && ell<T>(ctx, n, a.p(), 5) == 0) {
+        degree = 5;
+        low_degree<T>(ctx, kPade5, 5, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
+    } else {
+        const double d8 = std::pow(power_norm_estimate<T>(ctx, n, a.p(), 8), 1.0 / 8.0);
+        const double eta3 = std::fmax(d6, d8);
+        if (eta3 < 9.504178996162932e-1 && ell<T>(ctx, n, a.p(), 7) == 0) {
+            degree = 7;
+            low_degree<T>(ctx, kPade7, 7, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
+        } else if (eta3 < 2.097847961257068e0 && ell<T>(ctx, n, a.p(), 9) == 0) {
+            degree = 9;
+            Mat<T> a8(ctx, n);
+            product<T>(ctx, n, a4.p(), a4.p(), a8.p());
+            low_degree<T>(ctx, kPade9, 9, n, a.p(), a2.p(), a4.p(), a6.p(), a8.p(), x.p());
+        } else {
+            const double d10 = std::pow(power_norm_estimate<T>(ctx, n, a.p(), 10), 1.0 / 10.0);
+            const double eta4 = std::fmax(d8, d10);
+            const double eta5 = std::fmin(eta3, eta4);
+            s = eta5 == 0.0 ? 0 : ceil_int(std::log2(eta5 / 4.25));
+            if (s < 0) s = 0;
+            {
+                Mat<T> sc(ctx, n);
+                scale_to<T>(ctx, (size_t)n * n, a.p(), std::ldexp(1.0, -s), sc.p());
+                s += ell<T>(ctx, n, sc.p(), 13);
+            }
+            const double f = std::ldexp(1.0, -s);
+            // Rescaled from the originals, in place: a, a2, a4, a6 are not
+            // needed unscaled again.
+            scale_to<T>(ctx, (size_t)n * n, a.p(), f, a.p());
+            scale_to<T>(ctx, (size_t)n * n, a2.p(), f * f, a2.p());
+            scale_to<T>(ctx, (size_t)n * n, a4.p(), std::pow(f, 4), a4.p());
+            scale_to<T>(ctx, (size_t)n * n, a6.p(), std::pow(f, 6), a6.p());
+            degree13<T>(ctx, n, a.p(), a2.p(), a4.p(), a6.p(), x.p());
+            // exp(A) = exp(A/2^s)^(2^s): s squarings, ping-ponging buffers.
+            T* cur = x.p();
+            T* other = a2.p();
+            for (int i = 0; i < s; i++) {
+                product<T>(ctx, n, cur, cur, other);
+                std::swap(cur, other);
+            }
+            if (cur != x.p()) copy_t<T>(ctx, n, n, cur, n, x.p(), n);
+        }
+    }
+    if (diag) {
+        diag->degree = degree;
+        diag->squarings = s;
+    }
+    copy_t<T>(ctx, n, n, x.p(), n, e, lde);
+}
+
+// ---------------------------------------------------------------------------
+// The exponential action (MatrixExponentialAction.cs:149-416).
+// ---------------------------------------------------------------------------
+namespace {
+
+const int kDegrees[] = {1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18,
+                        19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 35, 40, 45, 50, 55};
+const double kThetas[] = {2.29e-16, 2.58e-8, 1.39e-5, 3.40e-4, 2.40e-3, 9.07e-3, 2.38e-2, 5.00e-2, 8.96e-2,
+                          1.44e-1,  2.14e-1, 3.00e-1, 4.00e-1, 5.14e-1, 6.41e-1, 7.81e-1, 9.31e-1, 1.09,
+                          1.26,     1.44,    1.62,    1.82,    2.01,    2.22,    2.43,    2.64,    2.86,
+                          3.08,     3.31,    3.54,    4.7,     6.0,     7.2,     8.5,     9.9};
+constexpr int kMaxDegree = 55, kMaxPower = 8, kEll = 2, kCount = 35;
+
+// SelectParameters (MatrixExponentialAction.cs:347-398); alpha(p) returns
+// max(d(p), d(p+1)) or is empty for the norm-only selection.
+void select_parameters(double scaled_norm, const std::function<double(int)>* alpha, double scale, int columns,
+                       int& m_out, int& s_out)
+{
+    int best_degree = 0;
+    double best_scaling = 0.0, best_cost = INFINITY;
+    auto consider = [&](int index, double norm) {
+        double scaling = std::ceil(norm / kThetas[index]);
+        double cost = kDegrees[index] * scaling;
+        if (cost >= best_cost) return;
+        best_cost = cost;
+        best_degree = kDegrees[index];
+        best_scaling = scaling;
+    };
+    bool norm_only = alpha == nullptr;
+    if (!norm_only) {
+        double a = 2.0 * kEll * kMaxPower * (kMaxPower + 3);
+        double b = kThetas[kCount - 1] / (columns * (double)kMaxDegree);
+        norm_only = scaled_norm <= a * b;
+    }
+    if (norm_only) {
+        for (int i = 0; i < kCount; i++) consider(i, scaled_norm);
+    } else {
+        for (int p = 2; p <= kMaxPower; p++) {
+            double ap = scale * (*alpha)(p);
+            for (int i = 0; i < kCount; i++) {
+                if (kDegrees[i] < p * (p - 1) - 1) continue;
+                consider(i, ap);
+            }
+        }
+    }
+    double steps = std::fmax(best_scaling, 1.0);
+    if (!std::isfinite(best_cost) || steps > 2147483647.0)
+        fail(TNS_ERANGE, "||tA||_1 would need a number of operator applications that is not a computation this "
+                         "method can finish. Reduce t, or shift the operator.");
+    m_out = best_degree;
+    s_out = (int)steps;
+}
+
+template <typename T>
+T exp_scalar(T z)
+{
+    if constexpr (sizeof(T) == 8) return std::exp(z);
+    else {
+        double e = std::exp(z.x);
+        return make_double2(e * std::cos(z.y), e * std::sin(z.y));
+    }
+}
+
+// Run (MatrixExponentialAction.cs:239-320) with a device operator. When the
+// operator is a dense device matrix (dense != null) the product and the step
+// are fused (k-split partials, then one pass that finishes the product,
+// scales, accumulates and takes both infinity norms).
+template <typename T>
+void run(Context* ctx, const DevOp<T>& op, const T* dense, int lda, int nb, const T* b, int ldb, double t, T mu,
+         double one_norm, const std::function<double(int)>* alpha, T* fout, int ldf, ExpmvDiag* diag)
+{
+    const int n = op.n;
+    const double scale = std::fabs(t);
+    const double scaled_norm = scale * one_norm;
+    int m, s;
+    if (scaled_norm == 0.0) {
+        m = 0;
+        s = 1;
+    } else {
+        select_parameters(scaled_norm, alpha, scale, nb, m, s);
+    }
+    if (diag) {
+        diag->degree = m;
+        diag->scaling = s;
+        diag->applications = 0;
+        diag->early_exits = 0;
+    }
+    const size_t extent = (size_t)n * nb;
+    DevBuf<T> f(ctx, extent), work(ctx, extent), next(ctx, extent);
+    copy_t<T>(ctx, n, nb, b, ldb, f.get(), n);
+    copy_t<T>(ctx, n, nb, b, ldb, work.get(), n);
+
+    T eta;
+    if constexpr (sizeof(T) == 8) eta = exp_scalar<T>((mu * t) / (double)s);
+    else eta = exp_scalar<T>(s_div(s_scale(mu, t), make_double2((double)s, 0.0)));
+    const bool unit_eta = eq_h(eta, from_h(1.0, T{}));
+
+    const bool fused = dense != nullptr && (sizeof(T) == 8 ? nb <= 16 : nb <= 8);
+    const int ks = fused ? panel_ksplit<T>(n) : 1;
+    DevBuf<T> part(ctx, fused ? (size_t)ks * extent : 1);
+    double* d_norms = ctx->d_scal + 8;
+    double hn[2];
+
+    for (int i = 0; i < s; i++) {
+        zero_scalars(ctx, d_norms + 2, 1);
+        panel_infnorm<T>(ctx, n, nb, work.get(), d_norms + 2);
+        double c1 = read_double(ctx, d_norms + 2);
+        for (int j = 0; j < m; j++) {
+            const double coef = t / (s * (double)(j + 1));
+            zero_scalars(ctx, d_norms, 2);
+            if (fused) {
+                panel_n_partials<T>(ctx, n, nb, dense, lda, work.get(), n, part.get(), ks);
+                taylor_step<T>(ctx, n, nb, ks, part.get(), coef, work.get(), f.get(), d_norms);
+            } else {
+                op.apply(nb, work.get(), next.get());
+                taylor_step<T>(ctx, n, nb, 1, next.get(), coef, work.get(), f.get(), d_norms);
+            }
+            read_doubles(ctx, d_norms, hn, 2);
+            const double c2 = hn[0];
+            if (diag) diag->applications++;
+            if (c1 + c2 <= kUnitRoundoff * hn[1]) {
+                if (diag) diag->early_exits++;
+                break;
+            }
+            c1 = c2;
+        }
+        if (!unit_eta) mul_scalar<T>(ctx, extent, f.get(), eta);
+        copy_t<T>(ctx, n, nb, f.get(), n, work.get(), n);
+    }
+    copy_t<T>(ctx, n, nb, f.get(), n, fout, ldf);
+}
+
+}  // namespace
+
+template <typename T>
+void expmv(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb, double t, T* f, int ldf,
+           ExpmvDiag* diag)
+{
+    if (diag) *diag = ExpmvDiag{};
+    if (n == 0 || nb == 0) {
+        copy_t<T>(ctx, n, nb, b, ldb, f, ldf);
+        return;
+    }
+    Mat<T> shifted(ctx, n);
+    copy_t<T>(ctx, n, n, a, lda, shifted.p(), n);
+    // The trace shift (MatrixExponentialAction.cs:124-128, Complex.cs:53-60).
+    trace<T>(ctx, n, a, lda, (T*)ctx->d_scal);
+    T tr{};
+    TNS_HIP(hipMemcpyAsync(&tr, ctx->d_scal, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    T mu;
+    if constexpr (sizeof(T) == 8) mu = tr / n;
+    else mu = make_double2(tr.x / n, tr.y / n);
+    shift_diag<T>(ctx, n, shifted.p(), n, neg_h(mu));
+    const double anorm1 = one_norm<T>(ctx, n, shifted.p());
+
+    // d(p) = ||A^p||^(1/p), estimated once each (MatrixExponentialAction.cs:
+    // 155-177, 413-416), t = Ell = 2 probe columns.
+    double dcache[kMaxPower + 2];
+    for (double& v : dcache) v = NAN;
+    const T* sp = shifted.p();
+    auto dpow = [&](int p) {
+        if (std::isnan(dcache[p]))
+            dcache[p] = std::pow(normest<T>(ctx, power_operator<T>(ctx, n, sp, n, p), kEll, 5, 20260915).value,
+                                 1.0 / p);
+        return dcache[p];
+    };
+    std::function<double(int)> alpha = [&](int p) { return std::fmax(dpow(p), dpow(p + 1)); };
+    DevOp<T> op = power_operator<T>(ctx, n, sp, n, 1);
+    run<T>(ctx, op, sp, n, nb, b, ldb, t, mu, anorm1, &alpha, f, ldf, diag);
+}
+
+template <typename T>
+void expmv_op(Context* ctx, const DevOp<T>& op, int nb, const T* b, int ldb, double t, double one_norm_bound, T* f,
+              int ldf, ExpmvDiag* diag)
+{
+    if (diag) *diag = ExpmvDiag{};
+    if (!std::isfinite(one_norm_bound) || one_norm_bound < 0.0)
+        fail(TNS_ERANGE, "The norm bound must be finite and not negative.");
+    if (op.n == 0 || nb == 0) {
+        copy_t<T>(ctx, op.n, nb, b, ldb, f, ldf);
+        return;
+    }
+    run<T>(ctx, op, nullptr, 0, nb, b, ldb, t, from_h(0.0, T{}), one_norm_bound, nullptr, f, ldf, diag);
+}
+
+template NormEst normest<double>(Context*, const DevOp<double>&, int, int, int);
+template NormEst normest<double2>(Context*, const DevOp<double2>&, int, int, int);
+template DevOp<double> power_operator<double>(Context*, int, const double*, int, int);
+template DevOp<double2> power_operator<double2>(Context*, int, const double2*, int, int);
+template DevOp<double> inverse_operator<double>(Context*, int, const double*, int, const int*);
+template DevOp<double2> inverse_operator<double2>(Context*, int, const double2*, int, const int*);
+template double gecon<double>(Context*, int, const double*, int, const int*, int, double, int);
+template double gecon<double2>(Context*, int, const double2*, int, const int*, int, double, int);
+template void expm<double>(Context*, int, const double*, int, double*, int, ExpmDiag*);
+template void expm<double2>(Context*, int, const double2*, int, double2*, int, ExpmDiag*);
+template void expmv<double>(Context*, int, int, const double*, int, const double*, int, double, double*, int,
+                            ExpmvDiag*);
+template void expmv<double2>(Context*, int, int, const double2*, int, const double2*, int, double, double2*, int,
+                             ExpmvDiag*);
+template void expmv_op<double>(Context*, const DevOp<double>&, int, const double*, int, double, double, double*, int,
+                               ExpmvDiag*);
+template void expmv_op<double2>(Context*, const DevOp<double2>&, int, const double2*, int, double, double, double2*,
+                                int, ExpmvDiag*);
+
+}  // namespace tns

@@ -1,0 +1,571 @@
+// Memory-bound device kernels: exact norms (Norms.cs:11-66,
+// ComplexKernels.cs:144-180), narrow panel products (PanelProduct.cs:25-55,
+// LinearOperators.cs:232-283), and the element-wise steps of the
+// exponentials (MatrixExponential.cs:518-558, MatrixExponentialAction.cs:
+// 286-317, 446-496), fused where the reference makes several passes.
+//
+// Every reduction is deterministic: partial results go to a buffer in a
+// fixed layout and are combined in a fixed order, so repeated calls give the
+// same bits (the norm estimator and the (m, s) selection depend on that).
+// Maxima are combined with NaN propagating, as Math.Max does.
+#include "blas1.hpp"
+
+#include "tns_scalar.hpp"
+
+namespace tns {
+namespace {
+
+__device__ __forceinline__ double nan_max(double a, double b)
+{
+    if (isnan(a) || isnan(b)) return NAN;
+    return a > b ? a : b;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = nan_max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double2 wave_sum(double2 v)
+{
+    return make_double2(wave_sum(v.x), wave_sum(v.y));
+}
+
+template <typename T>
+__device__ __forceinline__ double sq(T x);
+template <>
+__device__ __forceinline__ double sq<double>(double x) { return x * x; }
+template <>
+__device__ __forceinline__ double sq<double2>(double2 x) { return x.x * x.x + x.y * x.y; }
+
+// One wavefront per column: sum of |a_ij| (kind 0) or a_ij^2 (kind 2).
+template <typename T>
+__global__ void __launch_bounds__(256) col_reduce_kernel(int m, int n, const T* __restrict__ a, int lda, int kind,
+                                                         double* __restrict__ out)
+{
+    int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const T* col = a + (size_t)j * lda;
+    double s = 0.0;
+    if (kind == 0)
+        for (int i = lane; i < m; i += 64) s += s_abs(col[i]);
+    else
+        for (int i = lane; i < m; i += 64) s += sq(col[i]);
+    s = wave_sum(s);
+    if (lane == 0) out[j] = s;
+}
+
+// Partial row sums of |a_ij| over column chunks of 64: part[chunk][i].
+template <typename T>
+__global__ void __launch_bounds__(256) row_partial_kernel(int m, int n, const T* __restrict__ a, int lda,
+                                                          double* __restrict__ part)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int chunk = blockIdx.y;
+    if (i >= m) return;
+    int j0 = chunk * 64, j1 = j0 + 64 < n ? j0 + 64 : n;
+    double s = 0.0;
+    for (int j = j0; j < j1; j++) s += s_abs(a[i + (size_t)j * lda]);
+    part[(size_t)chunk * m + i] = s;
+}
+
+__global__ void __launch_bounds__(256) row_finish_kernel(int m, int chunks, const double* __restrict__ part,
+                                                         double* __restrict__ out)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int c = 0; c < chunks; c++) s += part[(size_t)c * m + i];
+    out[i] = s;
+}
+
+// One workgroup: out[0] = max (mode 0) or sqrt(sum) (mode 1) of v[0..n).
+__global__ void __launch_bounds__(256) vec_finish_kernel(int n, const double* __restrict__ v, int mode,
+                                                         double* __restrict__ out)
+{
+    __shared__ double red[4];
+    double acc = 0.0;
+    if (mode == 0)
+        for (int i = threadIdx.x; i < n; i += 256) acc = nan_max(acc, v[i]);
+    else
+        for (int i = threadIdx.x; i < n; i += 256) acc += v[i];
+    acc = mode == 0 ? wave_max(acc) : wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int q = 1; q < 4; q++) r = mode == 0 ? nan_max(r, red[q]) : r + red[q];
+        out[0] = mode == 0 ? r : sqrt(r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Narrow panel products. Y = op(A) X with A n x n and X n x t, t small.
+// ---------------------------------------------------------------------------
+constexpr int PN_ROWS = 64;   // rows per workgroup
+constexpr int PN_KL = 4;      // k-lanes per workgroup (one wavefront each)
+
+// Y = A X, split over k: part[ks][c][i] = sum over the ks-th column chunk.
+// A wavefront covers 64 consecutive rows of one column of A at a time
+// (coalesced); X's row j is uniform across the wavefront.
+template <typename T, int TMAX>
+__global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __restrict__ a, int lda,
+                                                      const T* __restrict__ x, int ldx, int kchunk,
+                                                      T* __restrict__ part)
+{
+    __shared__ T red[PN_KL][TMAX][PN_ROWS];
+    const int lane = threadIdx.x & 63, kl = threadIdx.x >> 6;
+    const int i = blockIdx.x * PN_ROWS + lane;
+    const int ks = blockIdx.y;
+    const int j0 = ks * kchunk, j1 = j0 + kchunk < n ? j0 + kchunk : n;
+    T acc[TMAX];
+#pragma unroll
+    for (int c = 0; c < TMAX; c++) acc[c] = s_zero(T{});
+    if (i < n) {
+        for (int j = j0 + kl; j < j1; j += PN_KL) {
+            const T av = a[i + (size_t)j * lda];
+#pragma unroll
+            for (int c = 0; c < TMAX; c++)
+                if (c < t) acc[c] = s_add(acc[c], s_mul(av, x[j + (size_t)c * ldx]));
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < TMAX; c++) red[kl][c][lane] = acc[c];
+    __syncthreads();
+    if (kl == 0 && i < n) {
+#pragma unroll
+        for (int c = 0; c < TMAX; c++) {
+            if (c >= t) break;
+            T s = red[0][c][lane];
+#pragma unroll
+            for (int q = 1; q < PN_KL; q++) s = s_add(s, red[q][c][lane]);
+            part[((size_t)ks * t + c) * n + i] = s;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) panel_finish_kernel(int n, int t, int ksplit, const T* __restrict__ part,
+                                                           T* __restrict__ y, int ldy)
+{
+    size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (e >= (size_t)n * t) return;
+    int c = (int)(e / n), i = (int)(e % n);
+    T s = part[(size_t)c * n + i];
+    for (int q = 1; q < ksplit; q++) s = s_add(s, part[((size_t)q * t + c) * n + i]);
+    y[i + (size_t)c * ldy] = s;
+}
+
+// Y = op(A)^T-form: y_j = sum_i conj?(a_ij) x_i, one wavefront per column j.
+template <typename T, int TMAX, bool CONJ>
+__global__ void __launch_bounds__(256) panel_t_kernel(int n, int t, const T* __restrict__ a, int lda,
+                                                      const T* __restrict__ x, int ldx, T* __restrict__ y, int ldy)
+{
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const T* col = a + (size_t)j * lda;
+    T acc[TMAX];
+#pragma unroll
+    for (int c = 0; c < TMAX; c++) acc[c] = s_zero(T{});
+    for (int i = lane; i < n; i += 64) {
+        T av = col[i];
+        if (CONJ) av = s_conj(av);
+#pragma unroll
+        for (int c = 0; c < TMAX; c++)
+            if (c < t) acc[c] = s_add(acc[c], s_mul(av, x[i + (size_t)c * ldx]));
+    }
+#pragma unroll
+    for (int c = 0; c < TMAX; c++) {
+        if (c >= t) break;
+        T s = wave_sum(acc[c]);
+        if (lane == 0) y[j + (size_t)c * ldy] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise.
+// ---------------------------------------------------------------------------
+
+// out := c0*I + sum_q coef[q] * src[q], accumulated in the reference's
+// order (Identity, then AddScaled one operand at a time), n x n, ld n.
+template <typename T>
+__global__ void lincomb_kernel(int n, double c0, int terms, LinTerms<T> lt, T* __restrict__ out)
+{
+    size_t total = (size_t)n * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t j = e / n, i = e % n;
+        T v = s_zero(T{});
+        if (i == j) v = s_add(v, s_scale(s_one(T{}), c0));
+        for (int q = 0; q < terms; q++) v = s_add(v, s_scale(lt.src[q][e], lt.coef[q]));
+        out[e] = v;
+    }
+}
+
+// den := v - u, num := v + u.
+template <typename T>
+__global__ void sub_add_kernel(size_t count, const T* __restrict__ u, const T* __restrict__ v, T* __restrict__ den,
+                               T* __restrict__ num)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+        den[e] = s_sub(v[e], u[e]);
+        num[e] = s_add(v[e], u[e]);
+    }
+}
+
+template <typename T>
+__global__ void scale_kernel(size_t count, const T* __restrict__ src, double f, T* __restrict__ dst)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = s_scale(src[e], f);
+}
+
+template <typename T>
+__global__ void mul_scalar_kernel(size_t count, T* __restrict__ x, T s)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        x[e] = s_mul(x[e], s);
+}
+
+template <typename T>
+__global__ void shift_diag_kernel(int n, T* __restrict__ a, int lda, T delta)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i + (size_t)i * lda] = s_add(a[i + (size_t)i * lda], delta);
+}
+
+// Sequential diagonal sum, one thread: the reference's Trace order.
+template <typename T>
+__global__ void trace_kernel(int n, const T* __restrict__ a, int lda, T* __restrict__ out)
+{
+    T s = s_zero(T{});
+    for (int i = 0; i < n; i++) s = s_add(s, a[i + (size_t)i * lda]);
+    out[0] = s;
+}
+
+template <typename T>
+__global__ void abs_kernel(size_t count, const T* __restrict__ a, double* __restrict__ out)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        out[e] = s_abs(a[e]);
+}
+
+// next_j = sum_i row_i * m_ij (the e^T |A|^p recurrence of Ell), one
+// wavefront per column.
+__global__ void __launch_bounds__(256) rowvec_kernel(int n, const double* __restrict__ mags,
+                                                     const double* __restrict__ row, double* __restrict__ next)
+{
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const double* col = mags + (size_t)j * n;
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s += row[i] * col[i];
+    s = wave_sum(s);
+    if (lane == 0) next[j] = s;
+}
+
+__global__ void fill_kernel(int n, double* __restrict__ v, double value)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = value;
+}
+
+// Bits of a non-negative double (or +NaN) order as the integers do, so the
+// maximum over rows is an integer atomic max: order-independent.
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v)
+{
+    if (isnan(v)) v = NAN;
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// The Taylor step of MatrixExponentialAction.Run (MatrixExponentialAction.cs:
+// 292-303), fused with the second pass of the panel product:
+//   next = sum of the k-split partials, work = coef*next, f += work,
+//   norms[0] = ||work||_inf, norms[1] = ||f||_inf.
+// One thread per row, all nb columns, so the row sums are local.
+template <typename T>
+__global__ void __launch_bounds__(256) taylor_step_kernel(int n, int nb, int ksplit, const T* __restrict__ part,
+                                                          double coef, T* __restrict__ work, T* __restrict__ f,
+                                                          double* __restrict__ norms)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double sw = 0.0, sf = 0.0;
+    for (int c = 0; c < nb; c++) {
+        T s = part[(size_t)c * n + i];
+        for (int q = 1; q < ksplit; q++) s = s_add(s, part[((size_t)q * nb + c) * n + i]);
+        T w = s_scale(s, coef);
+        size_t e = i + (size_t)c * n;
+        work[e] = w;
+        T fv = s_add(f[e], w);
+        f[e] = fv;
+        sw += s_abs(w);
+        sf += s_abs(fv);
+    }
+    atomic_max_nonneg(norms + 0, sw);
+    atomic_max_nonneg(norms + 1, sf);
+}
+
+// ||x||_inf of an n x nb panel (ld n) into out[0] (which must start at 0).
+template <typename T>
+__global__ void __launch_bounds__(256) panel_infnorm_kernel(int n, int nb, const T* __restrict__ x,
+                                                            double* __restrict__ out)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nb; c++) s += s_abs(x[i + (size_t)c * n]);
+    atomic_max_nonneg(out, s);
+}
+
+__global__ void zero_doubles_kernel(double* p, int count)
+{
+    int i = threadIdx.x;
+    if (i < count) p[i] = 0.0;
+}
+
+inline int grid_for(size_t total, int block = 256, int cap = 16384)
+{
+    size_t g = (total + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)(g < (size_t)cap ? g : cap);
+}
+
+template <typename T>
+void norm_impl(Context* ctx, int kind, int m, int n, const T* a, int lda, double* d_out)
+{
+    if (m == 0 || n == 0) {
+        hipLaunchKernelGGL(zero_doubles_kernel, dim3(1), dim3(64), 0, ctx->stream, d_out, 1);
+        return;
+    }
+    if (kind == '1' || kind == 'F') {
+        DevBuf<double> col(ctx, (size_t)n);
+        hipLaunchKernelGGL((col_reduce_kernel<T>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, m, n, a, lda,
+                           kind == '1' ? 0 : 2, col.get());
+        hipLaunchKernelGGL(vec_finish_kernel, dim3(1), dim3(256), 0, ctx->stream, n, col.get(), kind == '1' ? 0 : 1,
+                           d_out);
+    } else {
+        int chunks = (n + 63) / 64;
+        DevBuf<double> part(ctx, (size_t)chunks * m), rows(ctx, (size_t)m);
+        hipLaunchKernelGGL((row_partial_kernel<T>), dim3((m + 255) / 256, chunks), dim3(256), 0, ctx->stream, m, n, a,
+                           lda, part.get());
+        hipLaunchKernelGGL(row_finish_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, chunks, part.get(),
+                           rows.get());
+        hipLaunchKernelGGL(vec_finish_kernel, dim3(1), dim3(256), 0, ctx->stream, m, rows.get(), 0, d_out);
+    }
+    TNS_HIP(hipGetLastError());
+}
+
+// k-split for the n-form panel product: enough workgroups to cover the chip.
+inline int ksplit_for(int n)
+{
+    int rowblocks = (n + PN_ROWS - 1) / PN_ROWS;
+    int ks = (512 + rowblocks - 1) / rowblocks;
+    int maxks = (n + 63) / 64;  // at least 64 columns per chunk
+    if (ks > maxks) ks = maxks;
+    if (ks > 64) ks = 64;
+    if (ks < 1) ks = 1;
+    return ks;
+}
+
+template <typename T>
+void panel_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks)
+{
+    int kchunk = (n + ks - 1) / ks;
+    dim3 grid((n + PN_ROWS - 1) / PN_ROWS, ks);
+    if (t <= 2)
+        hipLaunchKernelGGL((panel_n_kernel<T, 2>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part);
+    else if (t <= 4)
+        hipLaunchKernelGGL((panel_n_kernel<T, 4>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part);
+    else if (t <= 8 || sizeof(T) != 8)
+        hipLaunchKernelGGL((panel_n_kernel<T, 8>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part);
+    else
+        hipLaunchKernelGGL((panel_n_kernel<T, (sizeof(T) == 8 ? 16 : 8)>), grid, dim3(256), 0, ctx->stream, n, t, a,
+                           lda, x, ldx, kchunk, part);
+}
+
+template <typename T>
+void panel_impl(Context* ctx, int trans, int n, int t, const T* a, int lda, const T* x, int ldx, T* y, int ldy)
+{
+    if (n == 0 || t == 0) return;
+    // Wide panels are GEMMs (complex beyond 8 columns: the k-split kernel's
+    // LDS holds 8 complex columns).
+    if (t > 16 || (sizeof(T) != 8 && t > 8)) {
+        if constexpr (sizeof(T) == 8)
+            dgemm_ex(ctx, trans ? 1 : 0, n, t, n, 1.0, (const double*)a, lda, (const double*)x, ldx, 0.0, (double*)y,
+                     ldy);
+        else
+            zgemm_ex(ctx, trans ? 2 : 0, n, t, n, make_double2(1.0, 0.0), (const double2*)a, lda, (const double2*)x,
+                     ldx, make_double2(0.0, 0.0), (double2*)y, ldy);
+        return;
+    }
+    if (!trans) {
+        int ks = ksplit_for(n);
+        DevBuf<T> part(ctx, (size_t)ks * t * n);
+        panel_partials<T>(ctx, n, t, a, lda, x, ldx, part, ks);
+        hipLaunchKernelGGL((panel_finish_kernel<T>), dim3(grid_for((size_t)n * t, 256, 1 << 30)), dim3(256), 0,
+                           ctx->stream, n, t, ks, part.get(), y, ldy);
+    } else {
+        dim3 grid((n + 3) / 4);
+        constexpr bool C = sizeof(T) != 8;
+        if (t <= 2)
+            hipLaunchKernelGGL((panel_t_kernel<T, 2, C>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, y, ldy);
+        else if (t <= 4)
+            hipLaunchKernelGGL((panel_t_kernel<T, 4, C>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, y, ldy);
+        else if (t <= 8)
+            hipLaunchKernelGGL((panel_t_kernel<T, 8, C>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, y, ldy);
+        else
+            hipLaunchKernelGGL((panel_t_kernel<T, 16, C>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, y,
+                               ldy);
+    }
+    TNS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void dnorm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* d_out)
+{
+    norm_impl<double>(ctx, kind, m, n, a, lda, d_out);
+}
+
+void znorm(Context* ctx, int kind, int m, int n, const cplx* a, int lda, double* d_out)
+{
+    norm_impl<double2>(ctx, kind, m, n, a, lda, d_out);
+}
+
+void dpanel(Context* ctx, int trans, int n, int t, const double* a, int lda, const double* x, int ldx, double* y,
+            int ldy)
+{
+    panel_impl<double>(ctx, trans, n, t, a, lda, x, ldx, y, ldy);
+}
+
+void zpanel(Context* ctx, int adjoint, int n, int t, const cplx* a, int lda, const cplx* x, int ldx, cplx* y, int ldy)
+{
+    panel_impl<double2>(ctx, adjoint, n, t, a, lda, x, ldx, y, ldy);
+}
+
+template <typename T>
+void lincomb(Context* ctx, int n, double c0, int terms, const LinTerms<T>& lt, T* out)
+{
+    hipLaunchKernelGGL((lincomb_kernel<T>), dim3(grid_for((size_t)n * n)), dim3(256), 0, ctx->stream, n, c0, terms, lt,
+                       out);
+    TNS_HIP(hipGetLastError());
+}
+template void lincomb<double>(Context*, int, double, int, const LinTerms<double>&, double*);
+template void lincomb<double2>(Context*, int, double, int, const LinTerms<double2>&, double2*);
+
+template <typename T>
+void sub_add(Context* ctx, size_t count, const T* u, const T* v, T* den, T* num)
+{
+    hipLaunchKernelGGL((sub_add_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, u, v, den, num);
+}
+template void sub_add<double>(Context*, size_t, const double*, const double*, double*, double*);
+template void sub_add<double2>(Context*, size_t, const double2*, const double2*, double2*, double2*);
+
+template <typename T>
+void scale_to(Context* ctx, size_t count, const T* src, double f, T* dst)
+{
+    if (!count) return;
+    hipLaunchKernelGGL((scale_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, src, f, dst);
+}
+template void scale_to<double>(Context*, size_t, const double*, double, double*);
+template void scale_to<double2>(Context*, size_t, const double2*, double, double2*);
+
+template <typename T>
+void mul_scalar(Context* ctx, size_t count, T* x, T s)
+{
+    if (!count) return;
+    hipLaunchKernelGGL((mul_scalar_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, x, s);
+}
+template void mul_scalar<double>(Context*, size_t, double*, double);
+template void mul_scalar<double2>(Context*, size_t, double2*, double2);
+
+template <typename T>
+void shift_diag(Context* ctx, int n, T* a, int lda, T delta)
+{
+    hipLaunchKernelGGL((shift_diag_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, a, lda, delta);
+}
+template void shift_diag<double>(Context*, int, double*, int, double);
+template void shift_diag<double2>(Context*, int, double2*, int, double2);
+
+template <typename T>
+void trace(Context* ctx, int n, const T* a, int lda, T* d_out)
+{
+    hipLaunchKernelGGL((trace_kernel<T>), dim3(1), dim3(1), 0, ctx->stream, n, a, lda, d_out);
+}
+template void trace<double>(Context*, int, const double*, int, double*);
+template void trace<double2>(Context*, int, const double2*, int, double2*);
+
+template <typename T>
+void magnitudes(Context* ctx, size_t count, const T* a, double* out)
+{
+    hipLaunchKernelGGL((abs_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, a, out);
+}
+template void magnitudes<double>(Context*, size_t, const double*, double*);
+template void magnitudes<double2>(Context*, size_t, const double2*, double*);
+
+void rowvec(Context* ctx, int n, const double* mags, const double* row, double* next)
+{
+    hipLaunchKernelGGL(rowvec_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, mags, row, next);
+}
+
+void fill(Context* ctx, int n, double* v, double value)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, v, value);
+}
+
+void vec_max(Context* ctx, int n, const double* v, double* d_out)
+{
+    hipLaunchKernelGGL(vec_finish_kernel, dim3(1), dim3(256), 0, ctx->stream, n, v, 0, d_out);
+}
+
+void zero_scalars(Context* ctx, double* d, int count)
+{
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(1), dim3(64), 0, ctx->stream, d, count);
+}
+
+template <typename T>
+int panel_ksplit(int n)
+{
+    return ksplit_for(n);
+}
+template int panel_ksplit<double>(int);
+template int panel_ksplit<double2>(int);
+
+template <typename T>
+void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks)
+{
+    panel_partials<T>(ctx, n, t, a, lda, x, ldx, part, ks);
+}
+template void panel_n_partials<double>(Context*, int, int, const double*, int, const double*, int, double*, int);
+template void panel_n_partials<double2>(Context*, int, int, const double2*, int, const double2*, int, double2*, int);
+
+template <typename T>
+void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms)
+{
+    hipLaunchKernelGGL((taylor_step_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nb, ks, part,
+                       coef, work, f, d_norms);
+}
+template void taylor_step<double>(Context*, int, int, int, const double*, double, double*, double*, double*);
+template void taylor_step<double2>(Context*, int, int, int, const double2*, double, double2*, double2*, double*);
+
+template <typename T>
+void panel_infnorm(Context* ctx, int n, int nb, const T* x, double* d_out)
+{
+    hipLaunchKernelGGL((panel_infnorm_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nb, x, d_out);
+}
+template void panel_infnorm<double>(Context*, int, int, const double*, double*);
+template void panel_infnorm<double2>(Context*, int, int, const double2*, double*);
+
+}  // namespace tns

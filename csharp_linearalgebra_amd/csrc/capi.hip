@@ -1,0 +1,728 @@
+// The C ABI (include/tensile_hip.h): argument checks in the form of
+// Tensile's seam (Operand.cs:80-96, KernelEntry.cs:246-269), staging of
+// host operands through the device, the context and its device pool.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <set>
+
+#include "algos.hpp"
+#include "blas1.hpp"
+
+struct tns_context {
+    tns::Context impl;
+    bool disposed = false;
+};
+
+namespace tns {
+
+[[noreturn]] void fail(int status, const std::string& message) { throw Error{status, message}; }
+
+void check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    (void)hipGetLastError();
+    fail(e == hipErrorOutOfMemory ? TNS_ENOMEM : TNS_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t Pool::round(size_t bytes)
+{
+    const size_t small = size_t(4) << 10, big = size_t(1) << 20;
+    if (bytes <= big) return (bytes + small - 1) / small * small;
+    return (bytes + big - 1) / big * big;
+}
+
+void* Pool::take(size_t bytes)
+{
+    size_t want = round(bytes);
+    auto it = free_.lower_bound(want);
+    if (it != free_.end() && it->first <= 2 * want) {
+        void* p = it->second;
+        size_t have = it->first;
+        free_.erase(it);
+        // Remember the block's real size by handing the caller's count back
+        // through give(): blocks are re-keyed by the size they were made with.
+        (void)have;
+        return p;
+    }
+    if (held_ + want > limit_) trim();
+    if (held_ + want > limit_)
+        fail(TNS_ENOMEM, "device working memory would exceed the context's limit (" + std::to_string(limit_) +
+                             " bytes)");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(TNS_ENOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+        }
+    }
+    held_ += want;
+    sizes_[p] = want;
+    return p;
+}
+
+void Pool::give(void* p, size_t)
+{
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;
+    free_.emplace(it->second, p);
+}
+
+void Pool::trim()
+{
+    if (free_.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto& kv : free_) {
+        (void)hipFree(kv.second);
+        held_ -= kv.first;
+        sizes_.erase(kv.second);
+    }
+    free_.clear();
+}
+
+Pool::~Pool()
+{
+    trim();
+    for (auto& kv : sizes_) (void)hipFree(kv.first);
+}
+
+bool is_device_pointer(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged || attr.type == hipMemoryTypeUnified;
+}
+
+void sync(Context* ctx) { TNS_HIP(hipStreamSynchronize(ctx->stream)); }
+
+}  // namespace tns
+
+namespace {
+
+using tns::Context;
+using tns::DevBuf;
+using tns::fail;
+
+// A caller's matrix as the device sees it: the caller's own pointer when it
+// is device memory, else a packed device copy (ld = rows).
+template <typename T>
+struct View {
+    T* d = nullptr;
+    int ld = 1;
+    bool staged = false;
+    DevBuf<T> buf;
+};
+
+template <typename T>
+View<T> bring(Context* ctx, const T* p, int rows, int cols, int ld, bool copy_in)
+{
+    View<T> v;
+    if (rows <= 0 || cols <= 0) return v;
+    if (tns::is_device_pointer(p)) {
+        v.d = const_cast<T*>(p);
+        v.ld = ld;
+        return v;
+    }
+    v.buf = DevBuf<T>(ctx, (size_t)rows * cols);
+    v.d = v.buf.get();
+    v.ld = rows;
+    v.staged = true;
+    if (copy_in)
+        TNS_HIP(hipMemcpy2DAsync(v.d, sizeof(T) * (size_t)v.ld, p, sizeof(T) * (size_t)ld, sizeof(T) * (size_t)rows,
+                                 cols, hipMemcpyHostToDevice, ctx->stream));
+    return v;
+}
+
+template <typename T>
+void send_back(Context* ctx, const View<T>& v, T* p, int rows, int cols, int ld)
+{
+    if (!v.staged || rows <= 0 || cols <= 0) return;
+    TNS_HIP(hipMemcpy2DAsync(p, sizeof(T) * (size_t)ld, v.d, sizeof(T) * (size_t)v.ld, sizeof(T) * (size_t)rows, cols,
+                             hipMemcpyDeviceToHost, ctx->stream));
+}
+
+// Operand.cs:80-96: non-negative shape, stride >= rows, extent in 64 bits.
+void check_shape(int rows, int cols, int ld, const char* what)
+{
+    if (rows < 0 || cols < 0) fail(TNS_EINVAL, std::string(what) + ": negative dimension");
+    if (ld < (rows > 1 ? rows : 1)) fail(TNS_EINVAL, std::string(what) + ": leading dimension below the row count");
+}
+
+void check_operand(const void* p, int rows, int cols, int ld, const char* what)
+{
+    check_shape(rows, cols, ld, what);
+    if ((long long)rows * cols > 0 && !p) fail(TNS_EINVAL, std::string(what) + ": null data for a non-empty operand");
+}
+
+void require(bool cond, const char* what)
+{
+    if (!cond) fail(TNS_EINVAL, std::string("Kernel operands disagree on ") + what + ".");
+}
+
+template <typename F>
+int guarded(tns_context* ctx, F&& body)
+{
+    if (!ctx) return TNS_EINVAL;
+    if (ctx->disposed) return TNS_EDISPOSED;
+    std::lock_guard<std::mutex> lock(ctx->impl.gate);
+    try {
+        TNS_HIP(hipSetDevice(ctx->impl.device));
+        body(&ctx->impl);
+        ctx->impl.last_error.clear();
+        return TNS_OK;
+    } catch (const tns::Error& e) {
+        ctx->impl.last_error = e.message;
+        (void)hipStreamSynchronize(ctx->impl.stream);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        ctx->impl.last_error = "host allocation failed";
+        return TNS_ENOMEM;
+    } catch (const std::exception& e) {
+        ctx->impl.last_error = e.what();
+        return TNS_EDEVICE;
+    }
+}
+
+// P from LAPACK-order interchanges (0-based): row i of P*A is row perm[i] of
+// A. Checks each interchange names a row at or below its step.
+std::vector<int> permutation_of(Context* ctx, const int* pivots, int count, int rows)
+{
+    std::vector<int> piv(count), perm(rows);
+    if (count > 0) {
+        if (tns::is_device_pointer(pivots)) {
+            TNS_HIP(hipMemcpyAsync(piv.data(), pivots, sizeof(int) * count, hipMemcpyDeviceToHost, ctx->stream));
+            tns::sync(ctx);
+        } else {
+            if (!pivots) fail(TNS_EINVAL, "pivots: null");
+            std::memcpy(piv.data(), pivots, sizeof(int) * count);
+        }
+    }
+    std::iota(perm.begin(), perm.end(), 0);
+    for (int k = 0; k < count; k++) {
+        if (piv[k] < k || piv[k] >= rows) fail(TNS_EINVAL, "pivots: interchange out of range");
+        std::swap(perm[k], perm[piv[k]]);
+    }
+    return perm;
+}
+
+DevBuf<int> upload_ints(Context* ctx, const std::vector<int>& h)
+{
+    DevBuf<int> d(ctx, h.size());
+    if (!h.empty())
+        TNS_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    return d;
+}
+
+inline double2 c2(tns_complex z) { return make_double2(z.re, z.im); }
+
+// ---- generic bodies -----------------------------------------------------
+
+template <typename T>
+void getrf_body(Context* ctx, int m, int n, T* a, int lda, int nb, int* pivots, tns_lu_info* info)
+{
+    check_operand(a, m, n, lda, "A");
+    const int limit = std::min(m, n);
+    if (limit > 0 && !pivots) fail(TNS_EINVAL, "pivots: null");
+    View<T> va = bring(ctx, a, m, n, lda, true);
+    DevBuf<int> piv(ctx, (size_t)std::max(limit, 1)), perm(ctx, (size_t)std::max(m, 1));
+    if constexpr (sizeof(T) == 8) tns::dgetrf(ctx, m, n, (double*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
+    else tns::zgetrf(ctx, m, n, (double2*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
+    if (limit > 0) {
+        if constexpr (sizeof(T) == 8) tns::dpivot_range(ctx, limit, (const double*)va.d, va.ld, ctx->d_scal);
+        else tns::zpivot_range(ctx, limit, (const double2*)va.d, va.ld, ctx->d_scal);
+        if (tns::is_device_pointer(pivots))
+            TNS_HIP(hipMemcpyAsync(pivots, piv.get(), sizeof(int) * limit, hipMemcpyDeviceToDevice, ctx->stream));
+        else
+            TNS_HIP(hipMemcpyAsync(pivots, piv.get(), sizeof(int) * limit, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    send_back(ctx, va, a, m, n, lda);
+    int sing = -1;
+    double range[2] = {0.0, 0.0};
+    TNS_HIP(hipMemcpyAsync(&sing, ctx->d_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (limit > 0) TNS_HIP(hipMemcpyAsync(range, ctx->d_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    tns::sync(ctx);
+    if (info) {
+        info->singular_column = sing;
+        info->smallest_pivot = range[0];
+        info->largest_pivot = range[1];
+    }
+}
+
+template <typename T>
+void getrs_body(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu, const int* pivots, int singular,
+                T* b, int ldb)
+{
+    check_operand(lu, n, n, ldlu, "factors");
+    check_operand(b, n, nrhs, ldb, "B");
+    if (trans != 0 && trans != 1) fail(TNS_EINVAL, "trans must be 0 or 1");
+    if (singular >= 0)
+        fail(TNS_ESTATE, "Matrix is singular: zero pivot at column " + std::to_string(singular) + ".");
+    if (n == 0 || nrhs == 0) return;
+    std::vector<int> hperm = permutation_of(ctx, pivots, n, n);
+    DevBuf<int> perm = upload_ints(ctx, hperm);
+    View<T> vlu = bring(ctx, lu, n, n, ldlu, true);
+    View<T> vb = bring(ctx, b, n, nrhs, ldb, true);
+    if constexpr (sizeof(T) == 8)
+        tns::dgetrs(ctx, trans, n, nrhs, (const double*)vlu.d, vlu.ld, perm.get(), (double*)vb.d, vb.ld);
+    else
+        tns::zgetrs(ctx, trans, n, nrhs, (const double2*)vlu.d, vlu.ld, perm.get(), (double2*)vb.d, vb.ld);
+    send_back(ctx, vb, b, n, nrhs, ldb);
+    tns::sync(ctx);
+}
+
+template <typename T>
+void gecon_body(Context* ctx, int n, const T* lu, int ldlu, const int* pivots, int singular, double anorm,
+                int columns, double* rcond)
+{
+    check_operand(lu, n, n, ldlu, "factors");
+    if (!rcond) fail(TNS_EINVAL, "rcond: null");
+    if (singular >= 0 || anorm == 0.0 || n == 0) {
+        *rcond = 0.0;
+        return;
+    }
+    std::vector<int> hperm = permutation_of(ctx, pivots, n, n);
+    DevBuf<int> perm = upload_ints(ctx, hperm);
+    View<T> vlu = bring(ctx, lu, n, n, ldlu, true);
+    if (vlu.ld != n) {
+        // The solves run on packed factors.
+        View<T> packed;
+        packed.buf = DevBuf<T>(ctx, (size_t)n * n);
+        packed.d = packed.buf.get();
+        packed.ld = n;
+        if constexpr (sizeof(T) == 8) tns::dcopy2d(ctx, n, n, (const double*)vlu.d, vlu.ld, (double*)packed.d, n);
+        else tns::zcopy2d(ctx, n, n, (const double2*)vlu.d, vlu.ld, (double2*)packed.d, n);
+        *rcond = tns::gecon<T>(ctx, n, packed.d, n, perm.get(), singular, anorm, columns);
+    } else {
+        *rcond = tns::gecon<T>(ctx, n, vlu.d, vlu.ld, perm.get(), singular, anorm, columns);
+    }
+}
+
+template <typename T>
+void expm_body(Context* ctx, int n, const T* a, int lda, T* e, int lde, tns_expm_diag* diag)
+{
+    if (n < 0) fail(TNS_EINVAL, "negative order");
+    check_operand(a, n, n, lda, "A");
+    check_operand(e, n, n, lde, "E");
+    if (n == 0) {
+        if (diag) *diag = tns_expm_diag{0, 0};
+        return;
+    }
+    View<T> va = bring(ctx, a, n, n, lda, true);
+    View<T> ve = bring(ctx, e, n, n, lde, false);
+    tns::ExpmDiag d;
+    tns::expm<T>(ctx, n, va.d, va.ld, ve.d, ve.ld, &d);
+    send_back(ctx, ve, e, n, n, lde);
+    tns::sync(ctx);
+    if (diag) *diag = tns_expm_diag{d.degree, d.squarings};
+}
+
+template <typename T>
+void expmv_body(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb, double t, T* f, int ldf,
+                tns_expmv_diag* diag)
+{
+    check_operand(a, n, n, lda, "A");
+    check_operand(b, n, nb, ldb, "B");
+    check_operand(f, n, nb, ldf, "F");
+    View<T> vb = bring(ctx, b, n, nb, ldb, true);
+    View<T> vf = bring(ctx, f, n, nb, ldf, false);
+    tns::ExpmvDiag d;
+    if (n > 0 && nb > 0) {
+        View<T> va = bring(ctx, a, n, n, lda, true);
+        tns::expmv<T>(ctx, n, nb, va.d, va.ld, vb.d, vb.ld, t, vf.d, vf.ld, &d);
+    } else if (n > 0) {
+        // No columns: F is B (MatrixExponentialAction.cs:116).
+    }
+    send_back(ctx, vf, f, n, nb, ldf);
+    tns::sync(ctx);
+    if (diag) *diag = tns_expmv_diag{d.degree, d.scaling, d.applications, d.early_exits};
+}
+
+// The caller's operator, host or device side, as a device operator.
+template <typename T, typename Fn>
+tns::DevOp<T> callback_operator(Context* ctx, int n, Fn apply, void* user, int device_operator)
+{
+    tns::DevOp<T> op;
+    op.n = n;
+    using Pub = typename std::conditional<sizeof(T) == 8, double, tns_complex>::type;
+    op.apply = [=](int t, const T* x, T* y) {
+        int rc;
+        if (device_operator) {
+            tns::sync(ctx);
+            rc = apply(user, n, t, (const Pub*)x, n, (Pub*)y, n);
+        } else {
+            std::vector<T> hx((size_t)n * t), hy((size_t)n * t);
+            TNS_HIP(hipMemcpyAsync(hx.data(), x, sizeof(T) * hx.size(), hipMemcpyDeviceToHost, ctx->stream));
+            tns::sync(ctx);
+            rc = apply(user, n, t, (const Pub*)hx.data(), n, (Pub*)hy.data(), n);
+            if (rc == 0)
+                TNS_HIP(hipMemcpyAsync(y, hy.data(), sizeof(T) * hy.size(), hipMemcpyHostToDevice, ctx->stream));
+            tns::sync(ctx);
+        }
+        if (rc != 0) fail(TNS_ECALLBACK, "the operator callback reported failure (" + std::to_string(rc) + ")");
+    };
+    op.adjoint = [](int, const T*, T*) { fail(TNS_EINVAL, "the matrix-free path never applies an adjoint"); };
+    return op;
+}
+
+template <typename T, typename Fn>
+void expmv_op_body(Context* ctx, int n, Fn apply, void* user, int device_operator, int nb, const T* b, int ldb,
+                   double t, double bound, T* f, int ldf, tns_expmv_diag* diag)
+{
+    if (n < 0) fail(TNS_EINVAL, "negative order");
+    if (!apply) fail(TNS_EINVAL, "operator: null");
+    check_operand(b, n, nb, ldb, "B");
+    check_operand(f, n, nb, ldf, "F");
+    if (!std::isfinite(bound) || bound < 0.0) fail(TNS_ERANGE, "The norm bound must be finite and not negative.");
+    View<T> vb = bring(ctx, b, n, nb, ldb, true);
+    View<T> vf = bring(ctx, f, n, nb, ldf, false);
+    tns::ExpmvDiag d;
+    if (n > 0 && nb > 0) {
+        tns::DevOp<T> op = callback_operator<T>(ctx, n, apply, user, device_operator);
+        tns::expmv_op<T>(ctx, op, nb, vb.d, vb.ld, t, bound, vf.d, vf.ld, &d);
+    }
+    send_back(ctx, vf, f, n, nb, ldf);
+    tns::sync(ctx);
+    if (diag) *diag = tns_expmv_diag{d.degree, d.scaling, d.applications, d.early_exits};
+}
+
+template <typename T>
+void normest_body(Context* ctx, int n, const T* a, int lda, int power, int columns, int itmax, int seed,
+                  tns_normest* out)
+{
+    check_operand(a, n, n, lda, "A");
+    if (!out) fail(TNS_EINVAL, "out: null");
+    if (power < 1) fail(TNS_ERANGE, "power must be at least 1");
+    if (n == 0) {
+        *out = tns_normest{0.0, 0, 0};
+        return;
+    }
+    View<T> va = bring(ctx, a, n, n, lda, true);
+    tns::NormEst r = tns::normest<T>(ctx, tns::power_operator<T>(ctx, n, va.d, va.ld, power), columns, itmax, seed);
+    *out = tns_normest{r.value, r.iterations, r.products};
+}
+
+template <typename T>
+void panel_body(Context* ctx, int trans, int n, int t, const T* a, int lda, const T* x, int ldx, T* y, int ldy)
+{
+    check_operand(a, n, n, lda, "A");
+    check_operand(x, n, t, ldx, "X");
+    check_operand(y, n, t, ldy, "Y");
+    if (n == 0 || t == 0) return;
+    View<T> va = bring(ctx, a, n, n, lda, true);
+    View<T> vx = bring(ctx, x, n, t, ldx, true);
+    View<T> vy = bring(ctx, y, n, t, ldy, false);
+    if constexpr (sizeof(T) == 8)
+        tns::dpanel(ctx, trans, n, t, (const double*)va.d, va.ld, (const double*)vx.d, vx.ld, (double*)vy.d, vy.ld);
+    else
+        tns::zpanel(ctx, trans, n, t, (const double2*)va.d, va.ld, (const double2*)vx.d, vx.ld, (double2*)vy.d,
+                    vy.ld);
+    send_back(ctx, vy, y, n, t, ldy);
+    if (va.staged || vx.staged || vy.staged) tns::sync(ctx);
+}
+
+template <typename T>
+void norm_body(Context* ctx, char kind, int m, int n, const T* a, int lda, double* out)
+{
+    check_operand(a, m, n, lda, "A");
+    if (!out) fail(TNS_EINVAL, "out: null");
+    const bool ok = kind == '1' || kind == 'I' || (kind == 'F' && sizeof(T) == 8);
+    if (!ok) fail(TNS_EINVAL, "unknown norm kind");
+    if (m == 0 || n == 0) {
+        *out = 0.0;
+        return;
+    }
+    View<T> va = bring(ctx, a, m, n, lda, true);
+    if constexpr (sizeof(T) == 8) tns::dnorm(ctx, kind, m, n, (const double*)va.d, va.ld, ctx->d_scal);
+    else tns::znorm(ctx, kind, m, n, (const double2*)va.d, va.ld, ctx->d_scal);
+    *out = tns::read_double(ctx, ctx->d_scal);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Exported entry points.
+// ===========================================================================
+extern "C" {
+
+TNS_API const char* tns_version(void) { return "tensile-hip 0.1 gfx950"; }
+
+TNS_API int tns_context_create(int device, tns_context** out)
+{
+    if (!out) return TNS_EINVAL;
+    *out = nullptr;
+    tns_context* ctx = new (std::nothrow) tns_context();
+    if (!ctx) return TNS_ENOMEM;
+    tns::Context& c = ctx->impl;
+    c.device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c.d_scal, 64 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c.d_int, 64 * sizeof(int));
+    if (e == hipSuccess) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0) c.pool.set_limit(total_b / 4 * 3);
+        else (void)hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (c.d_scal) (void)hipFree(c.d_scal);
+        if (c.d_int) (void)hipFree(c.d_int);
+        if (c.stream) (void)hipStreamDestroy(c.stream);
+        delete ctx;
+        return e == hipErrorOutOfMemory ? TNS_ENOMEM : TNS_EDEVICE;
+    }
+    *out = ctx;
+    return TNS_OK;
+}
+
+// Releases the device resources. The handle itself stays valid so a late
+// call on it is refused with TNS_EDISPOSED (the Workspace's
+// ObjectDisposedException) instead of touching freed memory.
+TNS_API int tns_context_destroy(tns_context* ctx)
+{
+    if (!ctx) return TNS_EINVAL;
+    if (ctx->disposed) return TNS_OK;
+    std::lock_guard<std::mutex> lock(ctx->impl.gate);
+    tns::Context& c = ctx->impl;
+    (void)hipSetDevice(c.device);
+    (void)hipStreamSynchronize(c.stream);
+    c.pool.trim();
+    (void)hipFree(c.d_scal);
+    (void)hipFree(c.d_int);
+    (void)hipStreamDestroy(c.stream);
+    c.d_scal = nullptr;
+    c.d_int = nullptr;
+    c.stream = nullptr;
+    ctx->disposed = true;
+    return TNS_OK;
+}
+
+TNS_API const char* tns_last_error(const tns_context* ctx)
+{
+    if (!ctx) return "null context";
+    if (ctx->disposed) return "the context has been destroyed";
+    return ctx->impl.last_error.c_str();
+}
+
+TNS_API int tns_context_set_limit(tns_context* ctx, size_t bytes)
+{
+    return guarded(ctx, [&](Context* c) {
+        c->pool.set_limit(bytes);
+        c->pool.trim();
+    });
+}
+
+TNS_API int tns_context_stream(tns_context* ctx, void** stream)
+{
+    return guarded(ctx, [&](Context* c) {
+        if (!stream) fail(TNS_EINVAL, "stream: null");
+        *stream = (void*)c->stream;
+    });
+}
+
+TNS_API int tns_synchronize(tns_context* ctx)
+{
+    return guarded(ctx, [&](Context* c) { tns::sync(c); });
+}
+
+TNS_API int tns_dgemm(tns_context* ctx, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+                      int ldb, double beta, double* c, int ldc)
+{
+    return guarded(ctx, [&](Context* cx) {
+        check_shape(m, k, lda, "A");
+        check_shape(k, n, ldb, "B");
+        check_shape(m, n, ldc, "C");
+        // An empty destination returns before anything is read
+        // (KernelEntry.cs:41-44), however many columns it nominally has.
+        if (m == 0 || n == 0) return;
+        check_operand(a, m, k, lda, "A");
+        check_operand(b, k, n, ldb, "B");
+        check_operand(c, m, n, ldc, "C");
+        View<double> va = bring(cx, a, m, k, lda, true);
+        View<double> vb = bring(cx, b, k, n, ldb, true);
+        View<double> vc = bring(cx, c, m, n, ldc, beta != 0.0);
+        tns::dgemm(cx, m, n, k, alpha, va.d, va.ld, vb.d, vb.ld, beta, vc.d, vc.ld);
+        send_back(cx, vc, c, m, n, ldc);
+        if (va.staged || vb.staged || vc.staged) tns::sync(cx);
+    });
+}
+
+TNS_API int tns_zgemm(tns_context* ctx, int m, int n, int k, tns_complex alpha, const tns_complex* a, int lda,
+                      const tns_complex* b, int ldb, tns_complex beta, tns_complex* c, int ldc)
+{
+    return guarded(ctx, [&](Context* cx) {
+        check_shape(m, k, lda, "A");
+        check_shape(k, n, ldb, "B");
+        check_shape(m, n, ldc, "C");
+        // An empty destination returns before anything is read
+        // (KernelEntry.cs:41-44), however many columns it nominally has.
+        if (m == 0 || n == 0) return;
+        check_operand(a, m, k, lda, "A");
+        check_operand(b, k, n, ldb, "B");
+        check_operand(c, m, n, ldc, "C");
+        const double2 *pa = (const double2*)a, *pb = (const double2*)b;
+        double2* pc = (double2*)c;
+        View<double2> va = bring(cx, pa, m, k, lda, true);
+        View<double2> vb = bring(cx, pb, k, n, ldb, true);
+        View<double2> vc = bring(cx, (const double2*)pc, m, n, ldc, !(beta.re == 0.0 && beta.im == 0.0));
+        tns::zgemm(cx, m, n, k, c2(alpha), va.d, va.ld, vb.d, vb.ld, c2(beta), vc.d, vc.ld);
+        send_back(cx, vc, pc, m, n, ldc);
+        if (va.staged || vb.staged || vc.staged) tns::sync(cx);
+    });
+}
+
+TNS_API int tns_dgetrf(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots, tns_lu_info* info)
+{
+    return guarded(ctx, [&](Context* c) { getrf_body<double>(c, m, n, a, lda, nb, pivots, info); });
+}
+
+TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
+                       tns_lu_info* info)
+{
+    return guarded(ctx, [&](Context* c) { getrf_body<double2>(c, m, n, (double2*)a, lda, nb, pivots, info); });
+}
+
+TNS_API int tns_dgetrs(tns_context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* pivots,
+                       int singular_column, double* b, int ldb)
+{
+    return guarded(ctx, [&](Context* c) {
+        getrs_body<double>(c, trans, n, nrhs, lu, ldlu, pivots, singular_column, b, ldb);
+    });
+}
+
+TNS_API int tns_zgetrs(tns_context* ctx, int trans, int n, int nrhs, const tns_complex* lu, int ldlu,
+                       const int* pivots, int singular_column, tns_complex* b, int ldb)
+{
+    return guarded(ctx, [&](Context* c) {
+        getrs_body<double2>(c, trans, n, nrhs, (const double2*)lu, ldlu, pivots, singular_column, (double2*)b, ldb);
+    });
+}
+
+TNS_API int tns_dtrsm(tns_context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb)
+{
+    return guarded(ctx, [&](Context* c) {
+        check_operand(a, m, m, lda, "A");
+        check_operand(b, m, n, ldb, "B");
+        if (kind < 0 || kind > 5) fail(TNS_EINVAL, "unknown triangular solve kind");
+        if (m == 0 || n == 0) return;
+        View<double> va = bring(c, a, m, m, lda, true);
+        View<double> vb = bring(c, b, m, n, ldb, true);
+        tns::dtrsm(c, kind, m, n, va.d, va.ld, vb.d, vb.ld);
+        send_back(c, vb, b, m, n, ldb);
+        if (va.staged || vb.staged) tns::sync(c);
+    });
+}
+
+TNS_API int tns_dpanel(tns_context* ctx, int trans, int n, int t, const double* a, int lda, const double* x, int ldx,
+                       double* y, int ldy)
+{
+    return guarded(ctx, [&](Context* c) { panel_body<double>(c, trans, n, t, a, lda, x, ldx, y, ldy); });
+}
+
+TNS_API int tns_zpanel(tns_context* ctx, int adjoint, int n, int t, const tns_complex* a, int lda,
+                       const tns_complex* x, int ldx, tns_complex* y, int ldy)
+{
+    return guarded(ctx, [&](Context* c) {
+        panel_body<double2>(c, adjoint, n, t, (const double2*)a, lda, (const double2*)x, ldx, (double2*)y, ldy);
+    });
+}
+
+TNS_API int tns_dnorm(tns_context* ctx, char kind, int m, int n, const double* a, int lda, double* out)
+{
+    return guarded(ctx, [&](Context* c) { norm_body<double>(c, kind, m, n, a, lda, out); });
+}
+
+TNS_API int tns_znorm(tns_context* ctx, char kind, int m, int n, const tns_complex* a, int lda, double* out)
+{
+    return guarded(ctx, [&](Context* c) { norm_body<double2>(c, kind, m, n, (const double2*)a, lda, out); });
+}
+
+TNS_API int tns_dnormest_power(tns_context* ctx, int n, const double* a, int lda, int power, int columns,
+                               int max_iterations, int seed, tns_normest* out)
+{
+    return guarded(ctx, [&](Context* c) {
+        normest_body<double>(c, n, a, lda, power, columns, max_iterations, seed, out);
+    });
+}
+
+TNS_API int tns_znormest_power(tns_context* ctx, int n, const tns_complex* a, int lda, int power, int columns,
+                               int max_iterations, int seed, tns_normest* out)
+{
+    return guarded(ctx, [&](Context* c) {
+        normest_body<double2>(c, n, (const double2*)a, lda, power, columns, max_iterations, seed, out);
+    });
+}
+
+TNS_API int tns_dgecon(tns_context* ctx, int n, const double* lu, int ldlu, const int* pivots, int singular_column,
+                       double anorm, int columns, double* rcond)
+{
+    return guarded(ctx, [&](Context* c) {
+        gecon_body<double>(c, n, lu, ldlu, pivots, singular_column, anorm, columns, rcond);
+    });
+}
+
+TNS_API int tns_zgecon(tns_context* ctx, int n, const tns_complex* lu, int ldlu, const int* pivots,
+                       int singular_column, double anorm, int columns, double* rcond)
+{
+    return guarded(ctx, [&](Context* c) {
+        gecon_body<double2>(c, n, (const double2*)lu, ldlu, pivots, singular_column, anorm, columns, rcond);
+    });
+}
+
+TNS_API int tns_dexpm(tns_context* ctx, int n, const double* a, int lda, double* e, int lde, tns_expm_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) { expm_body<double>(c, n, a, lda, e, lde, diag); });
+}
+
+TNS_API int tns_zexpm(tns_context* ctx, int n, const tns_complex* a, int lda, tns_complex* e, int lde,
+                      tns_expm_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) {
+        expm_body<double2>(c, n, (const double2*)a, lda, (double2*)e, lde, diag);
+    });
+}
+
+TNS_API int tns_dexpmv(tns_context* ctx, int n, int nb, const double* a, int lda, const double* b, int ldb, double t,
+                       double* f, int ldf, tns_expmv_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) { expmv_body<double>(c, n, nb, a, lda, b, ldb, t, f, ldf, diag); });
+}
+
+TNS_API int tns_zexpmv(tns_context* ctx, int n, int nb, const tns_complex* a, int lda, const tns_complex* b, int ldb,
+                       double t, tns_complex* f, int ldf, tns_expmv_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) {
+        expmv_body<double2>(c, n, nb, (const double2*)a, lda, (const double2*)b, ldb, t, (double2*)f, ldf, diag);
+    });
+}
+
+TNS_API int tns_dexpmv_op(tns_context* ctx, int n, tns_dapply_fn apply, void* user, int device_operator, int nb,
+                          const double* b, int ldb, double t, double one_norm_bound, double* f, int ldf,
+                          tns_expmv_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) {
+        expmv_op_body<double>(c, n, apply, user, device_operator, nb, b, ldb, t, one_norm_bound, f, ldf, diag);
+    });
+}
+
+TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* user, int device_operator, int nb,
+                          const tns_complex* b, int ldb, double t, double one_norm_bound, tns_complex* f, int ldf,
+                          tns_expmv_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) {
+        expmv_op_body<double2>(c, n, apply, user, device_operator, nb, (const double2*)b, ldb, t, one_norm_bound,
+                               (double2*)f, ldf, diag);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// Real and complex GEMM on gfx950 FP64 matrix cores.
+//
+// Replaces Tensile's BLIS-style path (Packing.cs, Gemm.cs, ParallelGemm.cs,
+// MicroKernels.cs) and the 4M complex product (ComplexKernels.cs:49-108).
+//
+// Design, per workgroup: a BM x BN tile of C, 4 waves each owning a WM x WN
+// sub-tile built from 16x16 v_mfma_f64_16x16x4_f64 blocks held in
+// registers. A and B stream through LDS in BK-deep slabs, double-buffered:
+// the next slab is fetched into registers while the current one feeds the
+// matrix cores, then written to the other LDS buffer, one barrier per slab.
+// Complex operands are split into real and imaginary LDS planes as they are
+// staged, and each complex 16x16 block takes four real MFMAs,
+//     Re += Ar*Br - Ai*Bi,   Im += Ar*Bi + Ai*Br,
+// the 4M method (never 3M): every part is a signed sum of exactly the
+// products the conventional method forms.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; the tile order is
+// remapped so each XCD receives a contiguous band of tiles, grouped 8 tile
+// rows at a time, and the A row-panels and B column-panels a band shares
+// stay in that XCD's L2.
+#include "tns_internal.hpp"
+
+namespace tns {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// blockIdx -> (tile_m, tile_n): XCD-contiguous bands, then grouped order.
+__device__ __forceinline__ void tile_of(int bid, int grid, int tiles_m, int tiles_n, int& tm, int& tn)
+{
+    int per = grid >> 3;
+    int id = bid;
+    if (bid < (per << 3)) id = (bid & 7) * per + (bid >> 3);
+    constexpr int GROUP = 8;
+    int group_size = GROUP * tiles_n;
+    int group = id / group_size;
+    int first = group * GROUP;
+    int gm = tiles_m - first < GROUP ? tiles_m - first : GROUP;
+    int in = id - group * group_size;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+struct Shape {
+    static constexpr int WAVES_M = BM / WM;
+    static constexpr int WAVES_N = BN / WN;
+    static constexpr int NT = WAVES_M * WAVES_N * 64;
+    static constexpr int TM = WM / 16;
+    static constexpr int TN = WN / 16;
+    static constexpr int LA = BM * BK / NT;
+    static constexpr int LB = BK * BN / NT;
+    static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "slab must split evenly over threads");
+    static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA granularity");
+};
+
+template <int BM, int BN, int BK, int WM, int WN, int TA>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, int lda,
+             const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
+             int tiles_m, int tiles_n)
+{
+    using S = Shape<BM, BN, BK, WM, WN>;
+    __shared__ double As[2][BK][BM + 1];
+    __shared__ double Bs[2][BK][BN + 1];
+
+    int tmi, tni;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
+    const int m0 = tmi * BM, n0 = tni * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % S::WAVES_M, wn = wave / S::WAVES_M;
+
+    v4d acc[S::TM][S::TN];
+#pragma unroll
+    for (int i = 0; i < S::TM; i++)
+#pragma unroll
+        for (int j = 0; j < S::TN; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double ra[S::LA], rb[S::LB];
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < S::LA; r++) {
+            int e = tid + r * S::NT;
+            if (TA == 0) {
+                int i = e % BM, kk = e / BM;
+                int gi = m0 + i, gk = k0 + kk;
+                ra[r] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : 0.0;
+            } else {
+                int kk = e % BK, i = e / BK;
+                int gi = m0 + i, gk = k0 + kk;
+                ra[r] = (gi < m && gk < k) ? A[gk + (size_t)gi * lda] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < S::LB; r++) {
+            int e = tid + r * S::NT;
+            int kk = e % BK, j = e / BK;
+            int gk = k0 + kk, gj = n0 + j;
+            rb[r] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : 0.0;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < S::LA; r++) {
+            int e = tid + r * S::NT;
+            if (TA == 0) As[buf][e / BM][e % BM] = ra[r];
+            else As[buf][e % BK][e / BK] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < S::LB; r++) {
+            int e = tid + r * S::NT;
+            Bs[buf][e % BK][e / BK] = rb[r];
+        }
+    };
+
+    const int ktiles = (k + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+
+    const int arow = wm * WM + (lane & 15);
+    const int bcol = wn * WN + (lane & 15);
+    const int kq = lane >> 4;
+
+    for (int t = 0; t < ktiles; ++t) {
+        const int cur = t & 1;
+        if (t + 1 < ktiles) gload((t + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double af[S::TM], bf[S::TN];
+#pragma unroll
+            for (int i = 0; i < S::TM; i++) af[i] = As[cur][kk + kq][arow + i * 16];
+#pragma unroll
+            for (int j = 0; j < S::TN; j++) bf[j] = Bs[cur][kk + kq][bcol + j * 16];
+#pragma unroll
+            for (int i = 0; i < S::TM; i++)
+#pragma unroll
+                for (int j = 0; j < S::TN; j++) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
+        }
+        if (t + 1 < ktiles) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    // D block layout of v_mfma_f64_16x16x4_f64: lane l, register r holds
+    // row 4*(l/16)+r, column l%16 of the 16x16 block.
+#pragma unroll
+    for (int i = 0; i < S::TM; i++)
+#pragma unroll
+        for (int j = 0; j < S::TN; j++) {
+            const int col = n0 + wn * WN + j * 16 + (lane & 15);
+            if (col >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm * WM + i * 16 + 4 * kq + r;
+                if (row >= m) continue;
+                double* p = C + row + (size_t)col * ldc;
+                double v = alpha * acc[i][j][r];
+                *p = beta == 0.0 ? v : beta * *p + v;
+            }
+        }
+}
+
+// Complex, with ComplexKernels.Combine's scalar special cases folded into
+// the write-back (ComplexKernels.cs:213-240). flags: 1 unit alpha, 2 real
+// alpha, 4 zero beta, 8 unit beta.
+template <int BM, int BN, int BK, int WM, int WN, int TA>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A, int lda,
+             const double2* __restrict__ B, int ldb, double br_, double bi_, double2* __restrict__ C,
+             int ldc, int flags, int tiles_m, int tiles_n)
+{
+    using S = Shape<BM, BN, BK, WM, WN>;
+    __shared__ double Ar[2][BK][BM + 1];
+    __shared__ double Ai[2][BK][BM + 1];
+    __shared__ double Br[2][BK][BN + 1];
+    __shared__ double Bi[2][BK][BN + 1];
+
+    int tmi, tni;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
+    const int m0 = tmi * BM, n0 = tni * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % S::WAVES_M, wn = wave / S::WAVES_M;
+
+    v4d re[S::TM][S::TN], im[S::TM][S::TN];
+#pragma unroll
+    for (int i = 0; i < S::TM; i++)
+#pragma unroll
+        for (int j = 0; j < S::TN; j++) {
+            re[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+            im[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+
+    double2 ra[S::LA], rb[S::LB];
+    const double2 zero2 = make_double2(0.0, 0.0);
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < S::LA; r++) {
+            int e = tid + r * S::NT;
+            if (TA == 0) {
+                int i = e % BM, kk = e / BM;
+                int gi = m0 + i, gk = k0 + kk;
+                ra[r] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : zero2;
+            } else {
+                int kk = e % BK, i = e / BK;
+                int gi = m0 + i, gk = k0 + kk;
+                ra[r] = (gi < m && gk < k) ? A[gk + (size_t)gi * lda] : zero2;
+                if (TA == 2) ra[r].y = -ra[r].y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < S::LB; r++) {
+            int e = tid + r * S::NT;
+            int kk = e % BK, j = e / BK;
+            int gk = k0 + kk, gj = n0 + j;
+            rb[r] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : zero2;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < S::LA; r++) {
+            int e = tid + r * S::NT;
+            if (TA == 0) {
+                Ar[buf][e / BM][e % BM] = ra[r].x;
+                Ai[buf][e / BM][e % BM] = ra[r].y;
+            } else {
+                Ar[buf][e % BK][e / BK] = ra[r].x;
+                Ai[buf][e % BK][e / BK] = ra[r].y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < S::LB; r++) {
+            int e = tid + r * S::NT;
+            Br[buf][e % BK][e / BK] = rb[r].x;
+            Bi[buf][e % BK][e / BK] = rb[r].y;
+        }
+    };
+
+    const int ktiles = (k + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+
+    const int arow = wm * WM + (lane & 15);
+    const int bcol = wn * WN + (lane & 15);
+    const int kq = lane >> 4;
+
+    for (int t = 0; t < ktiles; ++t) {
+        const int cur = t & 1;
+        if (t + 1 < ktiles) gload((t + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double a_r[S::TM], a_i[S::TM], a_n[S::TM], b_r[S::TN], b_i[S::TN];
+#pragma unroll
+            for (int i = 0; i < S::TM; i++) {
+                a_r[i] = Ar[cur][kk + kq][arow + i * 16];
+                a_i[i] = Ai[cur][kk + kq][arow + i * 16];
+                a_n[i] = -a_i[i];
+            }
+#pragma unroll
+            for (int j = 0; j < S::TN; j++) {
+                b_r[j] = Br[cur][kk + kq][bcol + j * 16];
+                b_i[j] = Bi[cur][kk + kq][bcol + j * 16];
+            }
+#pragma unroll
+            for (int i = 0; i < S::TM; i++)
+#pragma unroll
+                for (int j = 0; j < S::TN; j++) {
+                    re[i][j] = mfma_f64(a_r[i], b_r[j], re[i][j]);
+                    im[i][j] = mfma_f64(a_r[i], b_i[j], im[i][j]);
+                    re[i][j] = mfma_f64(a_n[i], b_i[j], re[i][j]);
+                    im[i][j] = mfma_f64(a_i[i], b_r[j], im[i][j]);
+                }
+        }
+        if (t + 1 < ktiles) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const bool unit_alpha = flags & 1, real_alpha = flags & 2, zero_beta = flags & 4, unit_beta = flags & 8;
+#pragma unroll
+    for (int i = 0; i < S::TM; i++)
+#pragma unroll
+        for (int j = 0; j < S::TN; j++) {
+            const int col = n0 + wn * WN + j * 16 + (lane & 15);
+            if (col >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm * WM + i * 16 + 4 * kq + r;
+                if (row >= m) continue;
+                const double p = re[i][j][r], q = im[i][j][r];
+                double pr, pi;
+                if (unit_alpha) { pr = p; pi = q; }
+                else if (real_alpha) { pr = ar_ * p; pi = ar_ * q; }
+                else { pr = ar_ * p - ai_ * q; pi = ai_ * p + ar_ * q; }
+                double2* ptr = C + row + (size_t)col * ldc;
+                if (zero_beta) { *ptr = make_double2(pr, pi); }
+                else {
+                    double2 c = *ptr;
+                    if (unit_beta) *ptr = make_double2(c.x + pr, c.y + pi);
+                    else *ptr = make_double2((br_ * c.x - bi_ * c.y) + pr, (bi_ * c.x + br_ * c.y) + pi);
+                }
+            }
+        }
+}
+
+__global__ void dscale_kernel(int m, int n, double beta, double* __restrict__ c, int ldc)
+{
+    size_t total = (size_t)m * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t j = e / m, i = e % m;
+        double* p = c + i + j * ldc;
+        *p = beta == 0.0 ? 0.0 : beta * *p;
+    }
+}
+
+__global__ void zscale_kernel(int m, int n, double br, double bi, double2* __restrict__ c, int ldc)
+{
+    size_t total = (size_t)m * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t j = e / m, i = e % m;
+        double2* p = c + i + j * ldc;
+        double2 v = *p;
+        *p = (br == 0.0 && bi == 0.0) ? make_double2(0.0, 0.0) : make_double2(br * v.x - bi * v.y, bi * v.x + br * v.y);
+    }
+}
+
+int g_real_cfg = -1, g_cplx_cfg = -1;
+
+template <int BM, int BN, int BK, int WM, int WN, int TA>
+void launch_d(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda, const double* b, int ldb,
+              double beta, double* c, int ldc)
+{
+    int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tm * tn), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
+                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn);
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int TA>
+void launch_z(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda, const cplx* b, int ldb,
+              cplx beta, cplx* c, int ldc, int flags)
+{
+    int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    hipLaunchKernelGGL((zgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tm * tn), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
+                       ctx->stream, m, n, k, alpha.x, alpha.y, a, lda, b, ldb, beta.x, beta.y, c, ldc, flags, tm, tn);
+}
+
+long tiles(int m, int n, int bm, int bn) { return (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn); }
+
+}  // namespace
+
+void gemm_force_config(int real_cfg, int cplx_cfg)
+{
+    g_real_cfg = real_cfg;
+    g_cplx_cfg = cplx_cfg;
+}
+
+int gemm_config_count(bool complex_) { return complex_ ? 3 : 3; }
+
+void dscale_c(Context* ctx, int m, int n, double beta, double* c, int ldc)
+{
+    if (beta == 1.0 || m == 0 || n == 0) return;
+    size_t total = (size_t)m * n;
+    int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+    hipLaunchKernelGGL(dscale_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, beta, c, ldc);
+}
+
+void zscale_c(Context* ctx, int m, int n, cplx beta, cplx* c, int ldc)
+{
+    if ((beta.x == 1.0 && beta.y == 0.0) || m == 0 || n == 0) return;
+    size_t total = (size_t)m * n;
+    int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+    hipLaunchKernelGGL(zscale_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, beta.x, beta.y, c, ldc);
+}
+
+// Real: 0 = 128x128 (4 waves of 64x64), 1 = 128x64 (4 of 64x32),
+// 2 = 64x64 (4 of 32x32). The largest tile that still gives every CU a
+// workgroup wins; small problems take the 64x64 tile.
+void dgemm_ex(Context* ctx, int ta, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+              int ldb, double beta, double* c, int ldc)
+{
+    if (m == 0 || n == 0) return;
+    if (k == 0 || alpha == 0.0) {
+        dscale_c(ctx, m, n, beta, c, ldc);
+        return;
+    }
+    int cfg = g_real_cfg;
+    if (cfg < 0) {
+        if (tiles(m, n, 128, 128) >= 256) cfg = 0;
+        else if (tiles(m, n, 128, 64) >= 256) cfg = 1;
+        else cfg = 2;
+    }
+#define TNS_D(TA_)                                                                                          \
+    switch (cfg) {                                                                                         \
+    case 0: launch_d<128, 128, 8, 64, 64, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break; \
+    case 1: launch_d<128, 64, 8, 64, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break;  \
+    default: launch_d<64, 64, 16, 32, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break; \
+    }
+    if (ta == 0) { TNS_D(0) } else { TNS_D(1) }
+#undef TNS_D
+    TNS_HIP(hipGetLastError());
+}
+
+// Complex: 0 = 128x64 (4 waves of 64x32), 1 = 64x64 (4 of 32x32),
+// 2 = 64x32 (4 of 32x16).
+void zgemm_ex(Context* ctx, int ta, int m, int n, int k, cplx alpha, const cplx* a, int lda, const cplx* b, int ldb,
+              cplx beta, cplx* c, int ldc)
+{
+    if (m == 0 || n == 0) return;
+    if (k == 0 || (alpha.x == 0.0 && alpha.y == 0.0)) {
+        zscale_c(ctx, m, n, beta, c, ldc);
+        return;
+    }
+    int flags = 0;
+    if (alpha.x == 1.0 && alpha.y == 0.0) flags |= 1;
+    if (alpha.y == 0.0) flags |= 2;
+    if (beta.x == 0.0 && beta.y == 0.0) flags |= 4;
+    if (beta.x == 1.0 && beta.y == 0.0) flags |= 8;
+    int cfg = g_cplx_cfg;
+    if (cfg < 0) {
+        if (tiles(m, n, 128, 64) >= 256) cfg = 0;
+        else if (tiles(m, n, 64, 64) >= 160) cfg = 1;
+        else cfg = 2;
+    }
+#define TNS_Z(TA_)                                                                                                  \
+    switch (cfg) {                                                                                                 \
+    case 0: launch_z<128, 64, 8, 64, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;  \
+    case 1: launch_z<64, 64, 8, 32, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;   \
+    default: launch_z<64, 32, 8, 32, 16, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;  \
+    }
+    if (ta == 0) { TNS_Z(0) } else if (ta == 1) { TNS_Z(1) } else { TNS_Z(2) }
+#undef TNS_Z
+    TNS_HIP(hipGetLastError());
+}
+
+void dgemm(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda, const double* b, int ldb,
+           double beta, double* c, int ldc)
+{
+    dgemm_ex(ctx, 0, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void zgemm(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda, const cplx* b, int ldb, cplx beta,
+           cplx* c, int ldc)
+{
+    zgemm_ex(ctx, 0, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+}  // namespace tns

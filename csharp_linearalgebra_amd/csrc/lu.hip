@@ -1,0 +1,919 @@
+// LU with partial pivoting and its solves on the device, real and complex.
+//
+// Replaces Lu.Factor/Solve/SolveTransposed (Lu.cs:230-378), the generic
+// BlockedLu (BlockedLu.cs:45-381) and the triangular solves
+// (Triangular.cs:26-206). The algorithm is the reference's right-looking
+// blocked getrf: factor an m x nb panel, apply its interchanges to the
+// columns either side, U12 := L11^-1 A12, A22 -= A21 U12 on the matrix
+// cores. What moves to the device:
+//
+//  * The panel (getf2) runs in one workgroup with the panel held in
+//    registers: thread t owns rows t, t+T, ... across every panel column.
+//    Each column is a wavefront argmax (shuffles, then LDS across waves), one
+//    barrier, the pivot row published through LDS, a second barrier, then
+//    every thread scales and updates its own rows. The pivot rule is the
+//    reference's exactly: largest |x| (|Re|+|Im| for complex), ties to the
+//    lowest row, a NaN never wins unless it sits on the diagonal row, where
+//    the sequential scan starts. A panel wider than the register file holds
+//    for its height is factored in column chunks, right-looking (factor a
+//    chunk, move its interchanges, solve and update the rest of the panel).
+//  * Row interchanges are carried with the rows as payloads: the row's
+//    original index (the permutation P, for the solves) and its index at the
+//    start of the block step (and of the chunk). After a block step the rows
+//    whose payload moved are compacted into a (destination, source) list, and
+//    one kernel applies that list to every column outside the panel: a
+//    gather of at most 2*nb rows per column instead of nb dependent swaps.
+//  * Triangular solves with few right-hand sides run one workgroup per
+//    column: 64-row diagonal blocks solved by one wavefront with shuffles,
+//    the rest updated by the whole workgroup. Many right-hand sides (the
+//    Pade denominator's n) use blocked substitution: the diagonal block per
+//    column in registers, the off-diagonal update as a GEMM.
+#include <climits>
+
+#include "tns_internal.hpp"
+#include "tns_scalar.hpp"
+
+namespace tns {
+namespace {
+
+constexpr double kSafeMin = 2.2250738585072014e-308;
+
+__device__ __forceinline__ void best_combine(double& bv, int& bi, double v, int i)
+{
+    if (v > bv || (v == bv && i < bi)) {
+        bv = v;
+        bi = i;
+    }
+}
+
+__device__ __forceinline__ void wave_best(double& bv, int& bi)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(bv, off, 64);
+        int oi = __shfl_xor(bi, off, 64);
+        best_combine(bv, bi, ov, oi);
+    }
+}
+
+// Pivot-search key of row `row` in column j: the pivot measure, with a NaN
+// never winning except on the diagonal row, where the reference's sequential
+// scan starts and so returns it.
+template <typename T>
+__device__ __forceinline__ double pivot_key(T x, int row, int j)
+{
+    double v = s_pivmag(x);
+    if (isnan(v)) v = (row == j) ? INFINITY : -1.0;
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_t(T v, int src);
+template <>
+__device__ __forceinline__ double shfl_t<double>(double v, int src)
+{
+    return __shfl(v, src, 64);
+}
+template <>
+__device__ __forceinline__ double2 shfl_t<double2>(double2 v, int src)
+{
+    return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+__device__ __forceinline__ double shfl_xor_t(double v, int off) { return __shfl_xor(v, off, 64); }
+__device__ __forceinline__ double2 shfl_xor_t(double2 v, int off)
+{
+    return make_double2(__shfl_xor(v.x, off, 64), __shfl_xor(v.y, off, 64));
+}
+
+// ---------------------------------------------------------------------------
+// Panel factorization, register resident. m <= blockDim.x * R, w <= NB.
+// piv[j] receives the global pivot row (off + local row). perm, rs and rc
+// are payloads that travel with the rows (indexed by local row, offset by
+// the caller): the original row index, the row's index at the start of the
+// block step, and (if rc is not null) at the start of this chunk. info[0] is
+// min-reduced with the first zero-pivot column.
+// ---------------------------------------------------------------------------
+template <typename T, int NB, int R>
+__global__ void __launch_bounds__(512)
+getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
+                 int* __restrict__ rs, int* __restrict__ rc, int* __restrict__ info)
+{
+    __shared__ double red_v[2][8];
+    __shared__ int red_i[2][8];
+    __shared__ T prow[2][NB];
+    __shared__ T jrow[2][NB];
+    __shared__ int pay[2][6];
+
+    const int nt = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const T zero = s_zero(T{});
+    const bool chunk = rc != nullptr;
+
+    T v[R][NB];
+    int p0[R], p1[R], p2[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int row = tid + r * nt;
+        const bool ok = row < m;
+#pragma unroll
+        for (int c = 0; c < NB; c++) v[r][c] = (ok && c < w) ? a[row + (size_t)c * lda] : zero;
+        p0[r] = ok ? perm[row] : 0;
+        p1[r] = ok ? rs[row] : 0;
+        p2[r] = (ok && chunk) ? rc[row] : 0;
+    }
+
+    const int jl = m < w ? m : w;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j >= jl) break;
+        const int buf = j & 1;
+
+        double bv = -2.0;
+        int bi = INT_MAX;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = tid + r * nt;
+            if (row >= j && row < m) best_combine(bv, bi, pivot_key(v[r][j], row, j), row);
+        }
+        wave_best(bv, bi);
+        if (lane == 0) {
+            red_v[buf][wave] = bv;
+            red_i[buf][wave] = bi;
+        }
+        __syncthreads();
+        bv = red_v[buf][0];
+        bi = red_i[buf][0];
+        for (int q = 1; q < nw; ++q) best_combine(bv, bi, red_v[buf][q], red_i[buf][q]);
+        const int p = bi;
+
+        // Publish the pivot row and the diagonal row with their payloads.
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = tid + r * nt;
+            if (row == p) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) prow[buf][c] = v[r][c];
+                pay[buf][0] = p0[r];
+                pay[buf][1] = p1[r];
+                pay[buf][2] = p2[r];
+            }
+            if (row == j) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) jrow[buf][c] = v[r][c];
+                pay[buf][3] = p0[r];
+                pay[buf][4] = p1[r];
+                pay[buf][5] = p2[r];
+            }
+        }
+        __syncthreads();
+
+        const T pv = prow[buf][j];
+        const bool zero_pivot = s_iszero(pv);
+        if (tid == 0) {
+            piv[j] = off + p;
+            if (zero_pivot) atomicMin(info, off + j);
+        }
+        const bool swap = !zero_pivot && p != j;
+        const double mag = s_abs(pv);
+        const T recip = s_div(s_one(T{}), pv);
+
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = tid + r * nt;
+            if (swap && row == j) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) v[r][c] = prow[buf][c];
+                p0[r] = pay[buf][0];
+                p1[r] = pay[buf][1];
+                p2[r] = pay[buf][2];
+            } else if (swap && row == p) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) v[r][c] = jrow[buf][c];
+                p0[r] = pay[buf][3];
+                p1[r] = pay[buf][4];
+                p2[r] = pay[buf][5];
+            }
+            if (row > j && row < m) {
+                T l = v[r][j];
+                if (!zero_pivot) l = (mag >= kSafeMin) ? s_mul(l, recip) : s_div(l, pv);
+                v[r][j] = l;
+#pragma unroll
+                for (int c = j + 1; c < NB; c++) {
+                    if (c < w) {
+                        const T u = prow[buf][c];
+                        if (!s_iszero(u)) v[r][c] = s_add(v[r][c], s_mul(s_neg(u), l));
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int row = tid + r * nt;
+        if (row < m) {
+#pragma unroll
+            for (int c = 0; c < NB; c++)
+                if (c < w) a[row + (size_t)c * lda] = v[r][c];
+            perm[row] = p0[r];
+            rs[row] = p1[r];
+            if (chunk) rc[row] = p2[r];
+        }
+    }
+}
+
+// Panel factorization straight from global memory, for panels taller than
+// the register-resident kernel holds. One workgroup, column by column.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+getf2_global_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
+                    int* __restrict__ rs, int* __restrict__ rc, int* __restrict__ info)
+{
+    __shared__ double red_v[16];
+    __shared__ int red_i[16];
+    __shared__ int s_p;
+    const int nt = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const int jl = m < w ? m : w;
+    for (int j = 0; j < jl; ++j) {
+        T* col = a + (size_t)j * lda;
+        double bv = -2.0;
+        int bi = INT_MAX;
+        for (int row = j + tid; row < m; row += nt) best_combine(bv, bi, pivot_key(col[row], row, j), row);
+        wave_best(bv, bi);
+        if (lane == 0) {
+            red_v[wave] = bv;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double b = red_v[0];
+            int i = red_i[0];
+            for (int q = 1; q < nw; ++q) best_combine(b, i, red_v[q], red_i[q]);
+            s_p = i;
+        }
+        __syncthreads();
+        const int p = s_p;
+        const T pv0 = col[p];
+        const bool zero_pivot = s_iszero(pv0);
+        if (tid == 0) {
+            piv[j] = off + p;
+            if (zero_pivot) atomicMin(info, off + j);
+        }
+        if (!zero_pivot && p != j) {
+            for (int c = tid; c < w; c += nt) {
+                T* cc = a + (size_t)c * lda;
+                T t = cc[j];
+                cc[j] = cc[p];
+                cc[p] = t;
+            }
+            if (tid == 0) {
+                int t = perm[j]; perm[j] = perm[p]; perm[p] = t;
+                t = rs[j]; rs[j] = rs[p]; rs[p] = t;
+                if (rc) { t = rc[j]; rc[j] = rc[p]; rc[p] = t; }
+            }
+        }
+        __syncthreads();
+        const T pv = col[j];
+        const double mag = s_abs(pv);
+        const T recip = s_div(s_one(T{}), pv);
+        if (!zero_pivot)
+            for (int row = j + 1 + tid; row < m; row += nt)
+                col[row] = (mag >= kSafeMin) ? s_mul(col[row], recip) : s_div(col[row], pv);
+        __syncthreads();
+        for (int c = j + 1; c < w; ++c) {
+            T* tc = a + (size_t)c * lda;
+            const T u = tc[j];
+            if (s_iszero(u)) continue;
+            const T nu = s_neg(u);
+            for (int row = j + 1 + tid; row < m; row += nt) tc[row] = s_add(tc[row], s_mul(nu, col[row]));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void iota_kernel(int n, int* __restrict__ a, int* __restrict__ b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = i;
+        if (b) b[i] = i;
+    }
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+__global__ void finish_info_kernel(int* info)
+{
+    if (*info == INT_MAX) *info = -1;
+}
+
+// Rows of [0, count) whose payload says they moved: append (row, source),
+// and reset the payload for the next block step. list[0] holds the count.
+__global__ void compact_moves_kernel(int count, int* __restrict__ rowid, int* __restrict__ list)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int src = rowid[i];
+    if (src != i) {
+        int at = atomicAdd(list, 1);
+        list[1 + 2 * at] = i;
+        list[2 + 2 * at] = src;
+        rowid[i] = i;
+    }
+}
+
+// Apply a move list (rows relative to `a`) to columns [0, c0) and [c1, ncols).
+// Every source value is read before any destination is written.
+template <typename T, int MAXMOV>
+__global__ void __launch_bounds__(64)
+apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const int* __restrict__ list)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int left = c0, right = ncols - c1;
+    if (idx >= left + right) return;
+    int j = idx < left ? idx : c1 + (idx - left);
+    const int count = list[0];
+    T* col = a + (size_t)j * lda;
+    T vals[MAXMOV];
+#pragma unroll
+    for (int q = 0; q < MAXMOV; q++)
+        if (q < count) vals[q] = col[list[2 + 2 * q]];
+#pragma unroll
+    for (int q = 0; q < MAXMOV; q++)
+        if (q < count) col[list[1 + 2 * q]] = vals[q];
+}
+
+// Larger move lists (never produced by nb <= 64, kept for generality).
+template <typename T>
+__global__ void apply_moves_slow_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols,
+                                        const int* __restrict__ list, T* __restrict__ scratch)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int left = c0, right = ncols - c1;
+    if (idx >= left + right) return;
+    int j = idx < left ? idx : c1 + (idx - left);
+    const int count = list[0];
+    T* col = a + (size_t)j * lda;
+    T* s = scratch + (size_t)idx * count;
+    for (int q = 0; q < count; q++) s[q] = col[list[2 + 2 * q]];
+    for (int q = 0; q < count; q++) col[list[1 + 2 * q]] = s[q];
+}
+
+// U12 := L11^-1 A12 for the panel step, one thread per column of A12, the
+// column in registers, L11 read at uniform addresses (scalar loads).
+// The update order is the reference's column-oriented substitution
+// (Triangular.cs:45-57): x_i += (-x_p) * l_ip for p ascending.
+template <typename T, int W>
+__global__ void __launch_bounds__(64)
+trsm_llnu_kernel(int w, int ncols, const T* __restrict__ L, int ldl, T* __restrict__ B, int ldb)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    T* col = B + (size_t)c * ldb;
+    T x[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) x[i] = i < w ? col[i] : s_zero(T{});
+#pragma unroll
+    for (int p = 0; p < W; p++) {
+        if (p >= w) break;
+        const T xp = x[p];
+        if (s_iszero(xp)) continue;
+        const T s = s_neg(xp);
+#pragma unroll
+        for (int i = p + 1; i < W; i++)
+            if (i < w) x[i] = s_add(x[i], s_mul(s, L[i + (size_t)p * ldl]));
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++)
+        if (i < w) col[i] = x[i];
+}
+
+// Diagonal-block solve with the block in registers, one thread per
+// right-hand side column; all eight triangle/transpose/diagonal cases.
+// TR: 0 none, 1 transpose, 2 conjugate transpose.
+template <typename T, int W, bool LOWER, int TR, bool UNIT>
+__global__ void __launch_bounds__(64)
+trsm_diag_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    T* col = B + (size_t)c * ldb;
+    T x[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) x[i] = i < w ? col[i] : s_zero(T{});
+    auto at = [&](int i, int k) -> T {  // element (i, k) of op(A)
+        T v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+        return TR == 2 ? s_conj(v) : v;
+    };
+    // op(A) lower <=> forward substitution.
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    if (FORWARD) {
+#pragma unroll
+        for (int p = 0; p < W; p++) {
+            if (p >= w) break;
+            if (!UNIT) x[p] = s_div(x[p], at(p, p));
+            const T xp = x[p];
+            if (s_iszero(xp)) continue;
+            const T s = s_neg(xp);
+#pragma unroll
+            for (int i = p + 1; i < W; i++)
+                if (i < w) x[i] = s_add(x[i], s_mul(s, at(i, p)));
+        }
+    } else {
+#pragma unroll
+        for (int p = W - 1; p >= 0; p--) {
+            if (p >= w) continue;
+            if (!UNIT) x[p] = s_div(x[p], at(p, p));
+            const T xp = x[p];
+            if (s_iszero(xp)) continue;
+            const T s = s_neg(xp);
+#pragma unroll
+            for (int i = 0; i < W; i++)
+                if (i < p) x[i] = s_add(x[i], s_mul(s, at(i, p)));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++)
+        if (i < w) col[i] = x[i];
+}
+
+// Whole triangular solve for few right-hand sides: one workgroup of 256
+// threads per column of B. op(A) lower runs forward, upper backward, in
+// 64-row blocks; each block's diagonal part is solved by wavefront 0 with
+// shuffles, and the coupling to the other blocks is done by all threads:
+// right-looking (axpy order, the reference's) for op = none, left-looking
+// dot products for the transposed forms.
+template <typename T, bool LOWER, int TR, bool UNIT>
+__global__ void __launch_bounds__(256)
+trsv_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
+{
+    __shared__ T xb[64];
+    __shared__ T part[4][64];
+    T* x = B + (size_t)blockIdx.x * ldb;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto at = [&](int i, int k) -> T {
+        T v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+        return TR == 2 ? s_conj(v) : v;
+    };
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    const int nblk = (n + 63) / 64;
+    for (int bq = 0; bq < nblk; bq++) {
+        const int blk = FORWARD ? bq : nblk - 1 - bq;
+        const int k0 = blk * 64;
+        const int kb = n - k0 < 64 ? n - k0 : 64;
+
+        if (TR != 0) {
+            // Left-looking: subtract the already-solved part, a dot product
+            // down column (k0 + i) of A per block row i.
+            const int lo = FORWARD ? 0 : k0 + kb;
+            const int hi = FORWARD ? k0 : n;
+            for (int i = wave; i < kb; i += 4) {
+                T s = s_zero(T{});
+                for (int k = lo + lane; k < hi; k += 64) s = s_add(s, s_mul(at(k0 + i, k), x[k]));
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) s = s_add(s, shfl_xor_t(s, off));
+                if (lane == 0) part[0][i] = s;
+            }
+            __syncthreads();
+        }
+
+        if (wave == 0) {
+            T xi = lane < kb ? x[k0 + lane] : s_zero(T{});
+            if (TR != 0 && lane < kb) xi = s_sub(xi, part[0][lane]);
+            if (FORWARD) {
+                for (int p = 0; p < kb; p++) {
+                    if (!UNIT && lane == p) xi = s_div(xi, at(k0 + p, k0 + p));
+                    const T xp = shfl_t(xi, p);
+                    if (lane > p && lane < kb && !s_iszero(xp)) xi = s_add(xi, s_mul(s_neg(xp), at(k0 + lane, k0 + p)));
+                }
+            } else {
+                for (int p = kb - 1; p >= 0; p--) {
+                    if (!UNIT && lane == p) xi = s_div(xi, at(k0 + p, k0 + p));
+                    const T xp = shfl_t(xi, p);
+                    if (lane < p && !s_iszero(xp)) xi = s_add(xi, s_mul(s_neg(xp), at(k0 + lane, k0 + p)));
+                }
+            }
+            if (lane < kb) {
+                x[k0 + lane] = xi;
+                xb[lane] = xi;
+            }
+        }
+        __syncthreads();
+
+        if (TR == 0) {
+            // Right-looking: push this block into the rows still to come.
+            if (FORWARD) {
+                for (int i = k0 + kb + tid; i < n; i += 256) {
+                    T acc = x[i];
+                    for (int p = 0; p < kb; p++)
+                        if (!s_iszero(xb[p])) acc = s_add(acc, s_mul(s_neg(xb[p]), at(i, k0 + p)));
+                    x[i] = acc;
+                }
+            } else {
+                for (int i = tid; i < k0; i += 256) {
+                    T acc = x[i];
+                    for (int p = kb - 1; p >= 0; p--)
+                        if (!s_iszero(xb[p])) acc = s_add(acc, s_mul(s_neg(xb[p]), at(i, k0 + p)));
+                    x[i] = acc;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// dst[i] = src[perm[i]] (gather) or dst[perm[i]] = src[i] (scatter).
+template <typename T>
+__global__ void permute_rows_kernel(int n, int ncols, const int* __restrict__ perm, const T* __restrict__ src, int lds,
+                                    T* __restrict__ dst, int ldd, int scatter)
+{
+    size_t total = (size_t)n * ncols;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        int j = (int)(e / n), i = (int)(e % n);
+        if (scatter) dst[perm[i] + (size_t)j * ldd] = src[i + (size_t)j * lds];
+        else dst[i + (size_t)j * ldd] = src[perm[i] + (size_t)j * lds];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pivot_range_kernel(int n, const T* __restrict__ a, int lda, double* __restrict__ out)
+{
+    __shared__ double smin[4], smax[4];
+    __shared__ int snan[4];
+    double lo = INFINITY, hi = 0.0;
+    int nan_seen = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double v = s_abs(a[i + (size_t)i * lda]);
+        if (isnan(v)) nan_seen = 1;
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+        nan_seen |= __shfl_xor(nan_seen, off, 64);
+    }
+    int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        smin[wave] = lo;
+        smax[wave] = hi;
+        snan[wave] = nan_seen;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            lo = fmin(lo, smin[q]);
+            hi = fmax(hi, smax[q]);
+            nan_seen |= snan[q];
+        }
+        // Math.Min/Math.Max propagate a NaN; fmin/fmax drop it.
+        if (nan_seen) lo = hi = NAN;
+        if (n == 0) lo = 0.0;
+        out[0] = lo;
+        out[1] = hi;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side drivers.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Ops;
+template <>
+struct Ops<double> {
+    static void gemm(Context* c, int ta, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+                     int ldb, double beta, double* cc, int ldc)
+    {
+        dgemm_ex(c, ta, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
+    }
+    static double one() { return 1.0; }
+    static double neg_one() { return -1.0; }
+    static constexpr int RMAX = 16;  // register panel: R rows x 64/R columns
+};
+template <>
+struct Ops<double2> {
+    static void gemm(Context* c, int ta, int m, int n, int k, double2 alpha, const double2* a, int lda,
+                     const double2* b, int ldb, double2 beta, double2* cc, int ldc)
+    {
+        zgemm_ex(c, ta, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
+    }
+    static double2 one() { return make_double2(1.0, 0.0); }
+    static double2 neg_one() { return make_double2(-1.0, 0.0); }
+    static constexpr int RMAX = 8;
+};
+
+template <typename T, int NB, int R>
+void launch_getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rs, int* rc,
+                      int* info)
+{
+    int need = (m + R - 1) / R;
+    int nt = ((need + 63) / 64) * 64;
+    if (nt < 64) nt = 64;
+    hipLaunchKernelGGL((getf2_reg_kernel<T, NB, R>), dim3(1), dim3(nt), 0, ctx->stream, m, w, a, lda, piv, off, perm,
+                       rs, rc, info);
+}
+
+// Register-resident panel shapes: R rows per thread (512 threads), NB
+// columns. Real: 64 doubles per thread at R=1, 96 above; complex half that.
+template <typename T>
+int reg_panel_width(int m)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (m <= 512) return 64;
+        if (m <= 1024) return 48;
+        if (m <= 2048) return 24;
+        if (m <= 4096) return 12;
+        if (m <= 8192) return 6;
+        return 0;
+    } else {
+        if (m <= 512) return 48;
+        if (m <= 1024) return 24;
+        if (m <= 2048) return 12;
+        if (m <= 4096) return 6;
+        return 0;
+    }
+}
+
+template <typename T>
+void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rs, int* rc, int* info)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 1024) launch_getf2_reg<T, 48, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 2048) launch_getf2_reg<T, 24, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 4096) launch_getf2_reg<T, 12, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else launch_getf2_reg<T, 6, 16>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+    } else {
+        if (m <= 512) launch_getf2_reg<T, 48, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 1024) launch_getf2_reg<T, 24, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 2048) launch_getf2_reg<T, 12, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else launch_getf2_reg<T, 6, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+    }
+}
+
+template <typename T>
+void apply_moves(Context* ctx, T* a, int lda, int c0, int c1, int ncols, const int* list, int max_moves)
+{
+    int count = c0 + (ncols - c1);
+    if (count <= 0) return;
+    int blocks = (count + 63) / 64;
+    const int reg_cap = sizeof(T) == 8 ? 128 : 64;
+    if (max_moves <= 32)
+        hipLaunchKernelGGL((apply_moves_kernel<T, 32>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
+                           list);
+    else if (max_moves <= 64)
+        hipLaunchKernelGGL((apply_moves_kernel<T, 64>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
+                           list);
+    else if (max_moves <= reg_cap)
+        hipLaunchKernelGGL((apply_moves_kernel<T, (sizeof(T) == 8 ? 128 : 64)>), dim3(blocks), dim3(64), 0,
+                           ctx->stream, a, lda, c0, c1, ncols, list);
+    else {
+        DevBuf<T> scratch(ctx, (size_t)count * max_moves);
+        hipLaunchKernelGGL((apply_moves_slow_kernel<T>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
+                           list, scratch.get());
+    }
+}
+
+// Interchanges of panel rows [0, m) recorded in rowid: move the listed rows
+// in columns [0,c0) and [c1,ncols) of a (rows relative to the panel top).
+template <typename T>
+void sync_columns(Context* ctx, int m, T* a, int lda, int c0, int c1, int ncols, int* rowid, int* list, int max_moves)
+{
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, ctx->stream, list, 0);
+    hipLaunchKernelGGL(compact_moves_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, rowid, list);
+    apply_moves(ctx, a, lda, c0, c1, ncols, list, max_moves);
+}
+
+template <typename T, bool LOWER, int TR, bool UNIT>
+void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ldb)
+{
+    int blocks = (ncols + 63) / 64;
+    hipLaunchKernelGGL((trsm_diag_kernel<T, 64, LOWER, TR, UNIT>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, A,
+                       lda, B, ldb);
+}
+
+// op(A) X = B in place, A n x n triangular.
+template <typename T, bool LOWER, int TR, bool UNIT>
+void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    if (nrhs <= 32 || n <= 64) {
+        hipLaunchKernelGGL((trsv_kernel<T, LOWER, TR, UNIT>), dim3(nrhs), dim3(256), 0, ctx->stream, n, A, lda, B, ldb);
+        return;
+    }
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    const int nblk = (n + 63) / 64;
+    const int ta = TR;
+    for (int bq = 0; bq < nblk; bq++) {
+        const int blk = FORWARD ? bq : nblk - 1 - bq;
+        const int k0 = blk * 64;
+        const int kb = n - k0 < 64 ? n - k0 : 64;
+        if (TR != 0) {
+            // Left-looking: B_k -= op(A)[k, solved] X_solved.
+            if (FORWARD && k0 > 0)
+                Ops<T>::gemm(ctx, ta, kb, nrhs, k0, Ops<T>::neg_one(), A + (size_t)k0 * lda, lda, B, ldb, Ops<T>::one(),
+                             B + k0, ldb);
+            if (!FORWARD && k0 + kb < n)
+                Ops<T>::gemm(ctx, ta, kb, nrhs, n - k0 - kb, Ops<T>::neg_one(), A + (k0 + kb) + (size_t)k0 * lda, lda,
+                             B + k0 + kb, ldb, Ops<T>::one(), B + k0, ldb);
+        }
+        trsm_diag<T, LOWER, TR, UNIT>(ctx, kb, nrhs, A + k0 + (size_t)k0 * lda, lda, B + k0, ldb);
+        if (TR == 0) {
+            // Right-looking: B_rest -= A[rest, k] X_k.
+            if (FORWARD && k0 + kb < n)
+                Ops<T>::gemm(ctx, 0, n - k0 - kb, nrhs, kb, Ops<T>::neg_one(), A + (k0 + kb) + (size_t)k0 * lda, lda,
+                             B + k0, ldb, Ops<T>::one(), B + k0 + kb, ldb);
+            if (!FORWARD && k0 > 0)
+                Ops<T>::gemm(ctx, 0, k0, nrhs, kb, Ops<T>::neg_one(), A + (size_t)k0 * lda, lda, B + k0, ldb,
+                             Ops<T>::one(), B, ldb);
+        }
+    }
+}
+
+// U12 := L11^-1 A12 for a block step: one thread per column for w <= 64,
+// blocked substitution beyond.
+template <typename T>
+void trsm_llnu(Context* ctx, int w, int ncols, const T* L, int ldl, T* B, int ldb)
+{
+    if (w <= 0 || ncols <= 0) return;
+    int blocks = (ncols + 63) / 64;
+    if (w <= 16)
+        hipLaunchKernelGGL((trsm_llnu_kernel<T, 16>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
+    else if (w <= 32)
+        hipLaunchKernelGGL((trsm_llnu_kernel<T, 32>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
+    else if (w <= 64)
+        hipLaunchKernelGGL((trsm_llnu_kernel<T, 64>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
+    else
+        trsm_impl<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
+}
+
+template <typename T>
+void permute_rows(Context* ctx, int n, int ncols, const int* perm, const T* src, int lds, T* dst, int ldd, int scatter)
+{
+    size_t total = (size_t)n * ncols;
+    if (!total) return;
+    int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
+    hipLaunchKernelGGL((permute_rows_kernel<T>), dim3(blocks), dim3(256), 0, ctx->stream, n, ncols, perm, src, lds, dst,
+                       ldd, scatter);
+}
+
+template <typename T>
+void copy2d(Context* ctx, int m, int n, const T* src, int lds, T* dst, int ldd)
+{
+    if (m <= 0 || n <= 0) return;
+    TNS_HIP(hipMemcpy2DAsync(dst, sizeof(T) * (size_t)ldd, src, sizeof(T) * (size_t)lds, sizeof(T) * (size_t)m, n,
+                             hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+// Factor the m x w panel at a (global top row/column `off`): min(m, w)
+// pivot steps, with every one of the w columns receiving the interchanges
+// and the updates. Pivots and payloads are indexed from the panel top. A
+// panel wider than the register kernel holds is factored in column chunks,
+// right-looking: factor a chunk, move its interchanges to the panel's other
+// columns, solve and update the panel columns to its right.
+template <typename T>
+void panel(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rs, int* rc, int* info,
+           int* list)
+{
+    if (m <= 0 || w <= 0) return;
+    const int jl = m < w ? m : w;
+    const int cw = reg_panel_width<T>(m);
+    if (cw == 0) {
+        hipLaunchKernelGGL((getf2_global_kernel<T>), dim3(1), dim3(1024), 0, ctx->stream, m, w, a, lda, piv, off, perm,
+                           rs, (int*)nullptr, info);
+        return;
+    }
+    if (w <= cw) {
+        getf2_reg<T>(ctx, m, w, a, lda, piv, off, perm, rs, nullptr, info);
+        return;
+    }
+    for (int c0 = 0; c0 < jl; c0 += cw) {
+        const int cwid = cw < jl - c0 ? cw : jl - c0;
+        T* sub = a + c0 + (size_t)c0 * lda;
+        getf2_reg<T>(ctx, m - c0, cwid, sub, lda, piv + c0, off + c0, perm + c0, rs + c0, rc + c0, info);
+        sync_columns<T>(ctx, m - c0, a + c0, lda, c0, c0 + cwid, w, rc + c0, list, 2 * cwid);
+        if (c0 + cwid >= w) continue;
+        trsm_llnu<T>(ctx, cwid, w - c0 - cwid, sub, lda, sub + (size_t)cwid * lda, lda);
+        if (m > c0 + cwid)
+            Ops<T>::gemm(ctx, 0, m - c0 - cwid, w - c0 - cwid, cwid, Ops<T>::neg_one(), sub + cwid, lda,
+                         sub + (size_t)cwid * lda, lda, Ops<T>::one(), sub + cwid + (size_t)cwid * lda, lda);
+    }
+}
+
+template <typename T>
+void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+{
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info, INT_MAX);
+    const int limit = m < n ? m : n;
+    if (limit > 0) {
+        if (nb < 1) nb = limit < 1024 ? 32 : 64;
+        const int maxw = nb < limit ? nb : n;
+        DevBuf<int> rs(ctx, (size_t)m), rc(ctx, (size_t)m);
+        DevBuf<int> list(ctx, 4 * (size_t)maxw + 4);
+        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, d_perm, rs.get());
+        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, rc.get(), (int*)nullptr);
+        if (nb >= limit) {
+            // Small problems are all panel (Lu.cs:244-249).
+            panel<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, rs, rc, d_info, list);
+        } else {
+            for (int jb = 0; jb < limit; jb += nb) {
+                const int w = nb < limit - jb ? nb : limit - jb;
+                T* pa = a + jb + (size_t)jb * lda;
+                panel<T>(ctx, m - jb, w, pa, lda, d_piv + jb, jb, d_perm + jb, rs.get() + jb, rc.get() + jb, d_info,
+                         list);
+                // This step's interchanges, applied to the columns either
+                // side of the panel (Lu.cs:270-278).
+                sync_columns<T>(ctx, m - jb, a + jb, lda, jb, jb + w, n, rs.get() + jb, list, 2 * w);
+                if (jb + w >= n) continue;
+                trsm_llnu<T>(ctx, w, n - jb - w, pa, lda, pa + (size_t)w * lda, lda);
+                if (jb + w >= m) continue;
+                Ops<T>::gemm(ctx, 0, m - jb - w, n - jb - w, w, Ops<T>::neg_one(), pa + w, lda, pa + (size_t)w * lda,
+                             lda, Ops<T>::one(), pa + w + (size_t)w * lda, lda);
+            }
+        }
+    }
+    hipLaunchKernelGGL(finish_info_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info);
+    TNS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void getrs_impl(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu, const int* d_perm, T* b, int ldb)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    DevBuf<T> tmp(ctx, (size_t)n * nrhs);
+    if (!trans) {
+        // P b, then L forward, then U backward (Lu.cs:336-351).
+        permute_rows<T>(ctx, n, nrhs, d_perm, b, ldb, tmp, n, 0);
+        trsm_impl<T, true, 0, true>(ctx, n, nrhs, lu, ldlu, tmp, n);
+        trsm_impl<T, false, 0, false>(ctx, n, nrhs, lu, ldlu, tmp, n);
+        copy2d<T>(ctx, n, nrhs, tmp, n, b, ldb);
+    } else {
+        // U^H forward, L^H backward, then the permutation undone
+        // (Lu.cs:366-378, BlockedLu.cs:126-141).
+        copy2d<T>(ctx, n, nrhs, b, ldb, tmp, n);
+        constexpr int TR = sizeof(T) == 8 ? 1 : 2;
+        trsm_impl<T, false, TR, false>(ctx, n, nrhs, lu, ldlu, tmp, n);
+        trsm_impl<T, true, TR, true>(ctx, n, nrhs, lu, ldlu, tmp, n);
+        permute_rows<T>(ctx, n, nrhs, d_perm, tmp, n, b, ldb, 1);
+    }
+    TNS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+{
+    getrf_impl<double>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+}
+
+void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+{
+    getrf_impl<double2>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+}
+
+void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb)
+{
+    getrs_impl<double>(ctx, trans, n, nrhs, lu, ldlu, d_perm, b, ldb);
+}
+
+void zgetrs(Context* ctx, int trans, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm, cplx* b, int ldb)
+{
+    getrs_impl<double2>(ctx, trans, n, nrhs, lu, ldlu, d_perm, b, ldb);
+}
+
+void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb)
+{
+    switch (kind) {
+    case TNS_TRI_LOWER_UNIT: trsm_impl<double, true, 0, true>(ctx, m, n, a, lda, b, ldb); break;
+    case TNS_TRI_UPPER: trsm_impl<double, false, 0, false>(ctx, m, n, a, lda, b, ldb); break;
+    case TNS_TRI_LOWER: trsm_impl<double, true, 0, false>(ctx, m, n, a, lda, b, ldb); break;
+    case TNS_TRI_UPPER_T: trsm_impl<double, false, 1, false>(ctx, m, n, a, lda, b, ldb); break;
+    case TNS_TRI_LOWER_T: trsm_impl<double, true, 1, false>(ctx, m, n, a, lda, b, ldb); break;
+    case TNS_TRI_LOWER_UNIT_T: trsm_impl<double, true, 1, true>(ctx, m, n, a, lda, b, ldb); break;
+    default: fail(TNS_EINVAL, "unknown triangular solve kind");
+    }
+    TNS_HIP(hipGetLastError());
+}
+
+void dpivot_range(Context* ctx, int n, const double* a, int lda, double* d_out)
+{
+    hipLaunchKernelGGL((pivot_range_kernel<double>), dim3(1), dim3(256), 0, ctx->stream, n, a, lda, d_out);
+}
+
+void zpivot_range(Context* ctx, int n, const cplx* a, int lda, double* d_out)
+{
+    hipLaunchKernelGGL((pivot_range_kernel<double2>), dim3(1), dim3(256), 0, ctx->stream, n, a, lda, d_out);
+}
+
+void dcopy2d(Context* ctx, int m, int n, const double* src, int lds, double* dst, int ldd)
+{
+    copy2d<double>(ctx, m, n, src, lds, dst, ldd);
+}
+
+void zcopy2d(Context* ctx, int m, int n, const cplx* src, int lds, cplx* dst, int ldd)
+{
+    copy2d<double2>(ctx, m, n, src, lds, dst, ldd);
+}
+
+}  // namespace tns

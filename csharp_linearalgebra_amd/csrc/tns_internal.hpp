@@ -1,0 +1,170 @@
+// Internal declarations of the MI355X backend: the context (Tensile's
+// Workspace on the device), status plumbing, the device buffer pool, and the
+// device-side entry points each kernel file provides. Nothing here crosses
+// the C ABI; tensile_hip.h is the public surface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tensile_hip.h"
+
+namespace tns {
+
+using cplx = double2;  // (re, im), the layout of System.Numerics.Complex
+
+// A thrown status: the C ABI catches it at the boundary and returns the code,
+// so no C++ exception ever crosses extern "C".
+struct Error {
+    int status;
+    std::string message;
+};
+
+[[noreturn]] void fail(int status, const std::string& message);
+void check_hip(hipError_t e, const char* what);
+#define TNS_HIP(call) ::tns::check_hip((call), #call)
+
+// Grow-only caching pool of device buffers, keyed by rounded size. Every
+// internal temporary comes from here, so a run of calls (the block steps of
+// a factorization, the squarings of an exponential) allocates once. A buffer
+// returns to the pool when its handle dies; reuse is ordered by the one
+// stream every kernel of a context runs on.
+class Pool {
+public:
+    explicit Pool(size_t limit_bytes) : limit_(limit_bytes) {}
+    ~Pool();
+    void* take(size_t bytes);
+    void give(void* p, size_t bytes);
+    void trim();
+    size_t held() const { return held_; }
+    size_t limit() const { return limit_; }
+    void set_limit(size_t l) { limit_ = l; }
+
+private:
+    static size_t round(size_t bytes);
+    std::multimap<size_t, void*> free_;
+    std::map<void*, size_t> sizes_;
+    size_t held_ = 0;
+    size_t limit_;
+};
+
+struct Context;
+
+// A device buffer on loan from the pool.
+template <typename T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    DevBuf(Context* ctx, size_t count);
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        release();
+        ctx_ = o.ctx_; p_ = o.p_; bytes_ = o.bytes_;
+        o.ctx_ = nullptr; o.p_ = nullptr; o.bytes_ = 0;
+        return *this;
+    }
+    T* get() const { return p_; }
+    operator T*() const { return p_; }
+    void release();
+
+private:
+    Context* ctx_ = nullptr;
+    T* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pool pool{size_t(64) << 30};
+    // Small fixed device and pinned-host areas for reductions and flags.
+    double* d_scal = nullptr;   // 64 doubles
+    int* d_int = nullptr;       // 64 ints
+    double* h_scal = nullptr;   // pinned mirror, 64 doubles
+    int* h_int = nullptr;       // pinned mirror, 64 ints
+    std::string last_error;
+    std::mutex gate;            // the Workspace lock (Workspace.cs:136-139)
+    bool timing = false;
+    double phase_ms[8] = {0};   // LU phase split when timing is on
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+template <typename T>
+DevBuf<T>::DevBuf(Context* ctx, size_t count) : ctx_(ctx), bytes_(sizeof(T) * (count ? count : 1))
+{
+    p_ = static_cast<T*>(ctx->pool.take(bytes_));
+}
+
+template <typename T>
+void DevBuf<T>::release()
+{
+    if (ctx_ && p_) ctx_->pool.give(p_, bytes_);
+    ctx_ = nullptr;
+    p_ = nullptr;
+    bytes_ = 0;
+}
+
+// Whether p is device (or managed) memory this process can hand to a kernel.
+bool is_device_pointer(const void* p);
+
+void sync(Context* ctx);
+
+// ---------------------------------------------------------------------------
+// Device-side operations. Every pointer here is device memory; every launch
+// goes on ctx->stream and returns without waiting.
+// ---------------------------------------------------------------------------
+
+// C := beta*C + alpha*A*B, FP64 MFMA (gemm.hip).
+void dgemm(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda,
+           const double* b, int ldb, double beta, double* c, int ldc);
+// Complex C := beta*C + alpha*A*B by the 4M method on FP64 MFMA (gemm.hip).
+void zgemm(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda,
+           const cplx* b, int ldb, cplx beta, cplx* c, int ldc);
+// op(A) = A (ta 0), A^T (1) or A^H (2, complex only).
+void dgemm_ex(Context* ctx, int ta, int m, int n, int k, double alpha, const double* a, int lda,
+              const double* b, int ldb, double beta, double* c, int ldc);
+void zgemm_ex(Context* ctx, int ta, int m, int n, int k, cplx alpha, const cplx* a, int lda,
+              const cplx* b, int ldb, cplx beta, cplx* c, int ldc);
+// Tile configuration override for the autotuner: -1 = heuristic.
+void gemm_force_config(int real_cfg, int cplx_cfg);
+int gemm_config_count(bool complex_);
+
+// LU (lu.hip). d_piv: min(m,n) LAPACK-order interchanges (0-based);
+// d_perm: m entries, row i of P*A is row d_perm[i] of A; d_info: the first
+// exactly-zero pivot column or -1.
+void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info);
+void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info);
+// Solves against the factors; trans 1 = A^T (real) / A^H (complex).
+void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* d_perm,
+            double* b, int ldb);
+void zgetrs(Context* ctx, int trans, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm,
+            cplx* b, int ldb);
+// The KernelEntry triangular solves: kind as tns_tri_kind.
+void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb);
+// min/max |u_ii| into d_out[0], d_out[1] (and a NaN flag).
+void dpivot_range(Context* ctx, int n, const double* a, int lda, double* d_out);
+void zpivot_range(Context* ctx, int n, const cplx* a, int lda, double* d_out);
+
+// Level-1/2 helpers (blas1.hip).
+void dnorm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* d_out);
+void znorm(Context* ctx, int kind, int m, int n, const cplx* a, int lda, double* d_out);
+void dpanel(Context* ctx, int trans, int n, int t, const double* a, int lda, const double* x, int ldx,
+            double* y, int ldy);
+void zpanel(Context* ctx, int adjoint, int n, int t, const cplx* a, int lda, const cplx* x, int ldx,
+            cplx* y, int ldy);
+void dcopy2d(Context* ctx, int m, int n, const double* src, int lds, double* dst, int ldd);
+void zcopy2d(Context* ctx, int m, int n, const cplx* src, int lds, cplx* dst, int ldd);
+void dscale_c(Context* ctx, int m, int n, double beta, double* c, int ldc);
+void zscale_c(Context* ctx, int m, int n, cplx beta, cplx* c, int ldc);
+
+}  // namespace tns

@@ -1,0 +1,115 @@
+"""Seeded synthetic inputs for the benchmark workloads and their parity tests.
+
+Each generator builds one of the dense-path workloads from numpy's PCG64
+generator with an explicit seed, so the device and the CPU oracle see the
+same arrays. Shapes and distributions:
+
+  lu_system         n=1024 real: A i.i.d. U(-1,1) + n*I, B n x 16 U(-1,1)
+  expm_operator     complex n=1000: A = -D + i*W, D = diag(U(0.1,10)) plus a
+                    symmetric U(-0.05,0.05) coupling, W symmetric U(-1,1)
+                    scaled so ||A||_1 = target (50); B n x 8 complex U(-1,1)
+  conditioned       real n: A = U diag(sigma) V^T with U, V products of 64
+                    Householder reflectors, sigma log-spaced to condition
+                    kappa; B n x 32 Gaussian
+  frequency_sweep   count complex matrices Z(w) = R + i w L of uneven order,
+                    R = diag U(0.01,1), L = G G^T / n + I, w log-spaced
+  adjoint_problem   the loss data of the exp(tA)B gradient workload
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def lu_system(n: int = 1024, nrhs: int = 16, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1.0, 1.0, (n, n)) + n * np.eye(n)
+    b = rng.uniform(-1.0, 1.0, (n, nrhs))
+    return np.asfortranarray(a), np.asfortranarray(b)
+
+
+def _one_norm(a):
+    return np.abs(a).sum(axis=0).max()
+
+
+def expm_operator(n: int = 1000, target: float = 50.0, nrhs: int = 8, seed: int = 0):
+    """A = -D + iW with ||A||_1 = target (to rounding), and B."""
+    rng = np.random.default_rng(seed)
+    d = np.diag(rng.uniform(0.1, 10.0, n))
+    c = rng.uniform(-0.05, 0.05, (n, n))
+    c = np.triu(c, 1)
+    d = d + c + c.T
+    w = rng.uniform(-1.0, 1.0, (n, n))
+    w = np.triu(w) + np.triu(w, 1).T
+    lo, hi = 0.0, 1.0
+    while _one_norm(-d + 1j * hi * w) < target:
+        hi *= 2.0
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if _one_norm(-d + 1j * mid * w) < target:
+            lo = mid
+        else:
+            hi = mid
+    a = -d + 1j * (0.5 * (lo + hi)) * w
+    b = rng.uniform(-1.0, 1.0, (n, nrhs)) + 1j * rng.uniform(-1.0, 1.0, (n, nrhs))
+    return np.asfortranarray(a), np.asfortranarray(b)
+
+
+def _householder_product(n, count, rng):
+    q = np.eye(n)
+    for _ in range(count):
+        v = rng.standard_normal(n)
+        v /= np.linalg.norm(v)
+        q = q - 2.0 * np.outer(v, v @ q)
+    return q
+
+
+def conditioned(n: int, kappa: float, nrhs: int = 32, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    u = _householder_product(n, 64, rng)
+    v = _householder_product(n, 64, rng)
+    sigma = np.logspace(0.0, -np.log10(kappa), n)
+    a = (u * sigma) @ v.T
+    b = rng.standard_normal((n, nrhs))
+    return np.asfortranarray(a), np.asfortranarray(b)
+
+
+def frequency_sweep(count: int = 512, nmin: int = 64, nmax: int = 1024, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    orders = rng.integers(nmin, nmax + 1, count)
+    omegas = np.logspace(1.0, 6.0, count)
+    mats = []
+    for n, w in zip(orders, omegas):
+        r = np.diag(rng.uniform(0.01, 1.0, n))
+        g = rng.standard_normal((n, n))
+        l = g @ g.T / n + np.eye(n)
+        mats.append(np.asfortranarray(r + 1j * w * l))
+    return mats
+
+
+def adjoint_problem(n: int, complex_: bool, target: float = 20.0, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    if complex_:
+        a, _ = expm_operator(n, target, 1, seed)
+    else:
+        a, _ = expm_operator(n, target, 1, seed)
+        a = np.asfortranarray(a.real * (target / _one_norm(a.real)))
+    dt = np.complex128 if complex_ else np.float64
+    def g(*shape):
+        x = rng.standard_normal(shape)
+        if complex_:
+            x = x + 1j * rng.standard_normal(shape)
+        return np.asfortranarray(x.astype(dt))
+    return a, g(n, 4), g(8, n), g(8, 4)
+
+
+def expm_flops(n: int, squarings: int, complex_: bool = True) -> float:
+    """Algorithmic flops of a degree-13 Expm with s squarings: (6 + s)
+    products, the LU of the Pade denominator and its n-column solve."""
+    mul = 8.0 if complex_ else 2.0
+    lu = (8.0 / 3.0 if complex_ else 2.0 / 3.0) * n ** 3
+    solve = (8.0 if complex_ else 2.0) * n ** 3
+    return (6 + squarings) * mul * n ** 3 + lu + solve
+
+
+def getrf_flops(n: int) -> float:
+    return (2.0 / 3.0) * n ** 3 - n ** 2 / 2.0 - n / 6.0

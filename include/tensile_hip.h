@@ -1,0 +1,221 @@
+/*
+ * tensile_hip.h -- the C ABI of the MI355X backend for Tensile's dense hot
+ * path (xfmrexpert/CSharp_LinearAlgebra).
+ *
+ * The entry points are what Tensile's pinning seam, KernelEntry
+ * (src/Tensile.Kernels/KernelEntry.cs:31-270), and the public assembly's
+ * device-resident algorithms would bind through P/Invoke. Each declaration
+ * names the reference interface it replaces.
+ *
+ * Conventions (Operand.cs:15-97, MatrixShape.cs:20-152):
+ *   - matrices are column-major with int32 rows, columns and leading
+ *     dimension ld >= max(1, rows); complex elements are two doubles (re, im),
+ *     the layout of System.Numerics.Complex;
+ *   - every pointer may be host memory (the C# caller's pinned span) or
+ *     device memory. Host-memory calls stage through the device and return
+ *     when the results are back (the seam is synchronous, KernelEntry.cs:19-24).
+ *     Device-memory calls run on the context's stream; those that return a
+ *     value through a host pointer (norms, estimates, LU info, diagnostics)
+ *     wait for it, the rest return once the work is enqueued
+ *     (tns_synchronize waits);
+ *   - padding rows between rows and ld are never written (GemmTests.cs:107-129);
+ *   - on any failure the destination is left untouched (invariant I5,
+ *     ComplexKernels.cs:103-106) and the status says why; no C++ exception
+ *     crosses this boundary.
+ *
+ * A context is not thread-safe; the C# Workspace keeps its lock around it
+ * (Workspace.cs:136-139).
+ */
+#ifndef TENSILE_HIP_H
+#define TENSILE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define TNS_API __declspec(dllexport)
+#else
+#define TNS_API __attribute__((visibility("default")))
+#endif
+
+/* Status codes; the C# binding maps each to the reference's exception. */
+#define TNS_OK 0
+#define TNS_EINVAL 1      /* ArgumentException: operands disagree (KernelEntry.cs:250-269) */
+#define TNS_ERANGE 2      /* ArgumentOutOfRangeException (MatrixExponentialAction.cs:215-220, 385-393) */
+#define TNS_ESTATE 3      /* InvalidOperationException: singular or non-square solve (Lu.cs:338-342),
+                             singular Pade denominator (MatrixExponential.cs:397-402) */
+#define TNS_ENOMEM 4      /* AllocationLimitException (TensileLimits.cs:78-96) */
+#define TNS_EDISPOSED 5   /* ObjectDisposedException (GemmDispatch.cs:107) */
+#define TNS_EDEVICE 6     /* the HIP runtime failed */
+#define TNS_ECALLBACK 7   /* a caller's operator callback returned non-zero (the binding rethrows its exception) */
+
+typedef struct tns_context tns_context;
+typedef struct { double re, im; } tns_complex;
+
+/* Triangular solve kinds: the six KernelEntry.Solve* (KernelEntry.cs:98-173). */
+enum {
+    TNS_TRI_LOWER_UNIT = 0,   /* SolveLowerUnit */
+    TNS_TRI_UPPER = 1,        /* SolveUpper */
+    TNS_TRI_LOWER = 2,        /* SolveLower */
+    TNS_TRI_UPPER_T = 3,      /* SolveUpperTransposed */
+    TNS_TRI_LOWER_T = 4,      /* SolveLowerTransposed */
+    TNS_TRI_LOWER_UNIT_T = 5  /* SolveLowerUnitTransposed */
+};
+
+/* LuFactorization's diagnostics (Lu.cs:17-67). */
+typedef struct {
+    int singular_column;     /* first exactly-zero pivot column, or -1 */
+    double smallest_pivot;   /* min |u_ii| */
+    double largest_pivot;    /* max |u_ii| */
+} tns_lu_info;
+
+/* NormEstimateResult (NormEstimate.cs:9). */
+typedef struct {
+    double value;
+    int iterations;
+    int products;
+} tns_normest;
+
+/* ExpmDiagnostics (MatrixExponential.cs:575-582). */
+typedef struct {
+    int degree;
+    int squarings;
+} tns_expm_diag;
+
+/* ExpmvDiagnostics (MatrixExponentialAction.cs:498-511). */
+typedef struct {
+    int degree;
+    int scaling;
+    int applications;
+    int early_exits;
+} tns_expmv_diag;
+
+/* ---- context: the device side of Tensile's Workspace (Workspace.cs:36-268) */
+TNS_API int tns_context_create(int device, tns_context** out);
+TNS_API int tns_context_destroy(tns_context* ctx);
+/* Message for the last failed call on this context (owned by the context). */
+TNS_API const char* tns_last_error(const tns_context* ctx);
+/* Ceiling on the device memory the context's pool may hold, the analogue of
+ * TensileLimits.MaxElements (TensileLimits.cs:27-147). */
+TNS_API int tns_context_set_limit(tns_context* ctx, size_t bytes);
+/* The context's HIP stream (hipStream_t), for callers chaining device work. */
+TNS_API int tns_context_stream(tns_context* ctx, void** stream);
+TNS_API int tns_synchronize(tns_context* ctx);
+/* Library build tag. */
+TNS_API const char* tns_version(void);
+
+/* ---- GEMM: KernelEntry.Multiply (KernelEntry.cs:34-56) ------------------
+ * C := beta*C + alpha*A*B; A m x k, B k x n, C m x n. beta == 0 overwrites
+ * (a NaN in C does not survive); k == 0 or alpha == 0 only scales C; an empty
+ * C returns at once (KernelEntry.cs:41-44, Gemm.cs:117-203). */
+TNS_API int tns_dgemm(tns_context* ctx, int m, int n, int k, double alpha, const double* a, int lda,
+                      const double* b, int ldb, double beta, double* c, int ldc);
+/* Complex product by the 4M method (ComplexKernels.Multiply,
+ * ComplexKernels.cs:49-108), with Combine's special cases for alpha = 1,
+ * real alpha, beta = 0 and beta = 1 (ComplexKernels.cs:213-240). */
+TNS_API int tns_zgemm(tns_context* ctx, int m, int n, int k, tns_complex alpha, const tns_complex* a, int lda,
+                      const tns_complex* b, int ldb, tns_complex beta, tns_complex* c, int ldc);
+
+/* ---- LU: KernelEntry.FactorLu (KernelEntry.cs:59-67) / Lu.Factor
+ * (Lu.cs:230-309); complex: BlockedLu.Factor (BlockedLu.cs:45-101) ---------
+ * Factors A in place as P*A = L*U. nb < 1 selects Lu.DefaultBlockSizeFor
+ * (32 below min(m,n) = 1024, else 64; Lu.cs:221-222). pivots receives
+ * min(m,n) 0-based LAPACK-order interchanges. */
+TNS_API int tns_dgetrf(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots,
+                       tns_lu_info* info);
+TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
+                       tns_lu_info* info);
+
+/* ---- LU solves: KernelEntry.SolveLu / SolveLuTransposed (KernelEntry.cs:70-95),
+ * Lu.Solve / SolveTransposed (Lu.cs:336-378); complex BlockedLu.Solve /
+ * SolveAdjoint (BlockedLu.cs:108-141) -------------------------------------
+ * trans 0: A X = B; 1: A^T X = B (real) or A^H X = B (complex). B n x nrhs is
+ * overwritten. singular_column >= 0 refuses with TNS_ESTATE. */
+TNS_API int tns_dgetrs(tns_context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu,
+                       const int* pivots, int singular_column, double* b, int ldb);
+TNS_API int tns_zgetrs(tns_context* ctx, int trans, int n, int nrhs, const tns_complex* lu, int ldlu,
+                       const int* pivots, int singular_column, tns_complex* b, int ldb);
+
+/* ---- triangular solves: KernelEntry.Solve* (KernelEntry.cs:98-173),
+ * Triangular.cs:26-206. A m x m, B m x n overwritten; kind TNS_TRI_*. */
+TNS_API int tns_dtrsm(tns_context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb);
+
+/* ---- panel products: KernelEntry.MultiplyPanel / MultiplyPanelTransposed
+ * (KernelEntry.cs:176-201), PanelProduct.cs:25-55. Y := A X or A^T X for a
+ * square A and an n x t panel X. Complex: A X or A^H X, the product of
+ * ComplexDenseOperator (LinearOperators.cs:232-283). */
+TNS_API int tns_dpanel(tns_context* ctx, int trans, int n, int t, const double* a, int lda, const double* x,
+                       int ldx, double* y, int ldy);
+TNS_API int tns_zpanel(tns_context* ctx, int adjoint, int n, int t, const tns_complex* a, int lda,
+                       const tns_complex* x, int ldx, tns_complex* y, int ldy);
+
+/* ---- norms: KernelEntry.OneNorm / InfinityNorm / FrobeniusNorm
+ * (KernelEntry.cs:204-234), Norms.cs:11-66; complex ComplexKernels.OneNorm /
+ * InfinityNorm (ComplexKernels.cs:144-180). kind '1', 'I' or 'F' (complex:
+ * '1' or 'I'). An empty operand gives 0. */
+TNS_API int tns_dnorm(tns_context* ctx, char kind, int m, int n, const double* a, int lda, double* out);
+TNS_API int tns_znorm(tns_context* ctx, char kind, int m, int n, const tns_complex* a, int lda, double* out);
+
+/* ---- 1-norm estimation: NormEstimate.Of over DenseMatrixOperator(a, power)
+ * (NormEstimate.cs:69-274, LinearOperators.cs:89-170); MatrixOperations.
+ * EstimateOneNorm. Complex: ComplexDenseOperator (LinearOperators.cs:194-293).
+ * columns/max_iterations/seed as NormEstimate.DefaultColumns (2),
+ * DefaultMaxIterations (5), DefaultSeed (20260915). */
+TNS_API int tns_dnormest_power(tns_context* ctx, int n, const double* a, int lda, int power, int columns,
+                               int max_iterations, int seed, tns_normest* out);
+TNS_API int tns_znormest_power(tns_context* ctx, int n, const tns_complex* a, int lda, int power, int columns,
+                               int max_iterations, int seed, tns_normest* out);
+
+/* ---- condition: LuDecomposition.ReciprocalCondition (MatrixOperations.cs:106-114),
+ * Condition.ReciprocalOne (NormEstimate.cs:432-466): 1/(||A||_1 ||A^-1||_1)
+ * with ||A^-1||_1 estimated through the LU inverse operator
+ * (LinearOperators.cs:303-350). anorm is ||A||_1 of the matrix before it was
+ * factored. 0 for a singular factorization or a zero norm. */
+TNS_API int tns_dgecon(tns_context* ctx, int n, const double* lu, int ldlu, const int* pivots,
+                       int singular_column, double anorm, int columns, double* rcond);
+TNS_API int tns_zgecon(tns_context* ctx, int n, const tns_complex* lu, int ldlu, const int* pivots,
+                       int singular_column, double anorm, int columns, double* rcond);
+
+/* ---- matrix exponential: MatrixExponential.Expm (MatrixExponential.cs:143-270,
+ * MatrixExponential.Complex.cs:35-53). E := exp(A), Al-Mohy & Higham 2009
+ * scaling and squaring with Pade degrees 3/5/7/9/13. TNS_ESTATE if the Pade
+ * denominator is exactly singular. */
+TNS_API int tns_dexpm(tns_context* ctx, int n, const double* a, int lda, double* e, int lde,
+                      tns_expm_diag* diag);
+TNS_API int tns_zexpm(tns_context* ctx, int n, const tns_complex* a, int lda, tns_complex* e, int lde,
+                      tns_expm_diag* diag);
+
+/* ---- exponential action: MatrixExponentialAction.Expmv
+ * (MatrixExponentialAction.cs:102-178, MatrixExponentialAction.Complex.cs:34-63).
+ * F := exp(tA) B, B n x nb; Al-Mohy & Higham 2011 truncated Taylor with the
+ * trace shift and the sharp (m, s) selection. TNS_ERANGE if the selected
+ * cost is not a finite computation. */
+TNS_API int tns_dexpmv(tns_context* ctx, int n, int nb, const double* a, int lda, const double* b, int ldb,
+                       double t, double* f, int ldf, tns_expmv_diag* diag);
+TNS_API int tns_zexpmv(tns_context* ctx, int n, int nb, const tns_complex* a, int lda, const tns_complex* b,
+                       int ldb, double t, tns_complex* f, int ldf, tns_expmv_diag* diag);
+
+/* ---- matrix-free action over an ILinearOperator<T> (LinearOperators.cs:37-57,
+ * MatrixExponentialAction.cs:203-230, MatrixExponentialAction.Complex.cs:85-111).
+ * The operator is the caller's callback: Y := Op * X for an n x t panel,
+ * returning 0 on success. With device_operator == 0 the panels are host
+ * memory (a C# ILinearOperator); with 1 they are device memory on the
+ * context's stream (a device-resident operator). No shift, no sharp bound:
+ * the caller's one_norm_bound >= ||A||_1 drives the selection; it must be
+ * finite and not negative (TNS_ERANGE). */
+typedef int (*tns_dapply_fn)(void* user, int n, int t, const double* x, int ldx, double* y, int ldy);
+typedef int (*tns_zapply_fn)(void* user, int n, int t, const tns_complex* x, int ldx, tns_complex* y, int ldy);
+TNS_API int tns_dexpmv_op(tns_context* ctx, int n, tns_dapply_fn apply, void* user, int device_operator, int nb,
+                          const double* b, int ldb, double t, double one_norm_bound, double* f, int ldf,
+                          tns_expmv_diag* diag);
+TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* user, int device_operator, int nb,
+                          const tns_complex* b, int ldb, double t, double one_norm_bound, tns_complex* f, int ldf,
+                          tns_expmv_diag* diag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
