@@ -147,8 +147,8 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
         __syncthreads();
     }
 
-    // D block layout of v_mfma_f64_16x16x4_f64: lane l, register r holds
-    // row 4*(l/16)+r, column l%16 of the 16x16 block.
+    // D block layout of v_mfma_f64_16x16x4_f64 (one-element groups): lane l,
+    // register r holds row 4*r + l/16, column l%16 of the 16x16 block.
 #pragma unroll
     for (int i = 0; i < S::TM; i++)
 #pragma unroll
@@ -157,7 +157,7 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
             if (col >= n) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * WM + i * 16 + 4 * kq + r;
+                const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
                 if (row >= m) continue;
                 double* p = C + row + (size_t)col * ldc;
                 double v = alpha * acc[i][j][r];
@@ -291,7 +291,7 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
             if (col >= n) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * WM + i * 16 + 4 * kq + r;
+                const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
                 if (row >= m) continue;
                 const double p = re[i][j][r], q = im[i][j][r];
                 double pr, pi;

@@ -307,18 +307,22 @@ __global__ void finish_info_kernel(int* info)
     if (*info == INT_MAX) *info = -1;
 }
 
-// Rows of [0, count) whose payload says they moved: append (row, source),
-// and reset the payload for the next block step. list[0] holds the count.
-__global__ void compact_moves_kernel(int count, int* __restrict__ rowid, int* __restrict__ list)
+// Rows of a slice [0, count) whose payload (a global row index; the slice
+// starts at global row `base`) says they moved: append (row, source) in
+// slice-local indices and reset the payload for the next step. list[0]
+// holds the count; at most `cap` moves are stored.
+__global__ void compact_moves_kernel(int count, int base, int* __restrict__ rowid, int* __restrict__ list, int cap)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    int src = rowid[i];
+    int src = rowid[i] - base;
     if (src != i) {
         int at = atomicAdd(list, 1);
-        list[1 + 2 * at] = i;
-        list[2 + 2 * at] = src;
-        rowid[i] = i;
+        if (at < cap) {
+            list[1 + 2 * at] = i;
+            list[2 + 2 * at] = src;
+        }
+        rowid[i] = i + base;
     }
 }
 
@@ -332,7 +336,7 @@ apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const 
     int left = c0, right = ncols - c1;
     if (idx >= left + right) return;
     int j = idx < left ? idx : c1 + (idx - left);
-    const int count = list[0];
+    const int count = list[0] < MAXMOV ? list[0] : MAXMOV;
     T* col = a + (size_t)j * lda;
     T vals[MAXMOV];
 #pragma unroll
@@ -346,13 +350,13 @@ apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const 
 // Larger move lists (never produced by nb <= 64, kept for generality).
 template <typename T>
 __global__ void apply_moves_slow_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols,
-                                        const int* __restrict__ list, T* __restrict__ scratch)
+                                        const int* __restrict__ list, T* __restrict__ scratch, int cap)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     int left = c0, right = ncols - c1;
     if (idx >= left + right) return;
     int j = idx < left ? idx : c1 + (idx - left);
-    const int count = list[0];
+    const int count = list[0] < cap ? list[0] : cap;
     T* col = a + (size_t)j * lda;
     T* s = scratch + (size_t)idx * count;
     for (int q = 0; q < count; q++) s[q] = col[list[2 + 2 * q]];
@@ -671,17 +675,21 @@ void apply_moves(Context* ctx, T* a, int lda, int c0, int c1, int ncols, const i
     else {
         DevBuf<T> scratch(ctx, (size_t)count * max_moves);
         hipLaunchKernelGGL((apply_moves_slow_kernel<T>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
-                           list, scratch.get());
+                           list, scratch.get(), max_moves);
     }
 }
 
-// Interchanges of panel rows [0, m) recorded in rowid: move the listed rows
-// in columns [0,c0) and [c1,ncols) of a (rows relative to the panel top).
+// Interchanges of rows [0, m) of a slice starting at global row `base`,
+// recorded in rowid: move the listed rows in columns [0,c0) and [c1,ncols)
+// of a (rows relative to the slice top). At most max_moves rows can have
+// moved (two per interchange); the list buffer holds 1 + 2*max_moves ints.
 template <typename T>
-void sync_columns(Context* ctx, int m, T* a, int lda, int c0, int c1, int ncols, int* rowid, int* list, int max_moves)
+void sync_columns(Context* ctx, int m, int base, T* a, int lda, int c0, int c1, int ncols, int* rowid, int* list,
+                  int max_moves)
 {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, ctx->stream, list, 0);
-    hipLaunchKernelGGL(compact_moves_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, rowid, list);
+    hipLaunchKernelGGL(compact_moves_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, base, rowid, list,
+                       max_moves);
     apply_moves(ctx, a, lda, c0, c1, ncols, list, max_moves);
 }
 
@@ -792,7 +800,7 @@ void panel(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* pe
         const int cwid = cw < jl - c0 ? cw : jl - c0;
         T* sub = a + c0 + (size_t)c0 * lda;
         getf2_reg<T>(ctx, m - c0, cwid, sub, lda, piv + c0, off + c0, perm + c0, rs + c0, rc + c0, info);
-        sync_columns<T>(ctx, m - c0, a + c0, lda, c0, c0 + cwid, w, rc + c0, list, 2 * cwid);
+        sync_columns<T>(ctx, m - c0, off + c0, a + c0, lda, c0, c0 + cwid, w, rc + c0, list, 2 * cwid);
         if (c0 + cwid >= w) continue;
         trsm_llnu<T>(ctx, cwid, w - c0 - cwid, sub, lda, sub + (size_t)cwid * lda, lda);
         if (m > c0 + cwid)
@@ -824,7 +832,7 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
                          list);
                 // This step's interchanges, applied to the columns either
                 // side of the panel (Lu.cs:270-278).
-                sync_columns<T>(ctx, m - jb, a + jb, lda, jb, jb + w, n, rs.get() + jb, list, 2 * w);
+                sync_columns<T>(ctx, m - jb, jb, a + jb, lda, jb, jb + w, n, rs.get() + jb, list, 2 * w);
                 if (jb + w >= n) continue;
                 trsm_llnu<T>(ctx, w, n - jb - w, pa, lda, pa + (size_t)w * lda, lda);
                 if (jb + w >= m) continue;
