@@ -55,6 +55,8 @@ SIGNATURES = {
     "tns_context_stream": (_I, [_P, C.POINTER(C.c_void_p)]),
     "tns_synchronize": (_I, [_P]),
     "tns_version": (C.c_char_p, []),
+    "tns_gemm_set_config": (_I, [_I, _I]),
+    "tns_gemm_config_count": (_I, [_I]),
     "tns_dgemm": (_I, [_P, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I]),
     "tns_zgemm": (_I, [_P, _I, _I, _I, Complex, _P, _I, _P, _I, Complex, _P, _I]),
     "tns_dgetrf": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo)]),
@@ -81,10 +83,25 @@ SIGNATURES = {
 _lib = None
 
 
+def _share_torch_runtime():
+    """PyTorch's ROCm wheel carries its own libamdhip64 and asks for it by a
+    name ("libamdhip64.so") that differs from the soname this library asks
+    for ("libamdhip64.so.7"). Loaded in the other order the process would
+    hold two HIP runtimes, and the second one finds no device; with torch's
+    loaded first the dynamic linker resolves this library's request to it by
+    soname, so both share one runtime, one device context and one address
+    space of device pointers."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """The loaded library; raises OSError if it is not built."""
     global _lib
     if _lib is None:
+        _share_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise OSError(
                 f"libtensile_hip.so not found at {LIB_PATH}; build it with "

@@ -206,20 +206,20 @@ void sort_keys_items(int n, double* keys, int* items)
 // Host element rules of the estimator (ElementKernels.cs:125-170,
 // ComplexKernels.cs:110-130).
 inline double mag_h(double x) { return std::fabs(x); }
-inline double mag_h(double2 x) { return s_abs(x); }
+inline double mag_h(zd x) { return s_abs(x); }
 inline double sign_h(double x) { return x >= 0.0 ? 1.0 : -1.0; }
-inline double2 sign_h(double2 x)
+inline zd sign_h(zd x)
 {
-    if (x.x == 0.0 && x.y == 0.0) return make_double2(1.0, 0.0);
+    if (x.x == 0.0 && x.y == 0.0) return make_zd(1.0, 0.0);
     double m = s_abs(x);
-    return make_double2(x.x / m, x.y / m);
+    return make_zd(x.x / m, x.y / m);
 }
 inline bool eq_h(double a, double b) { return a == b; }
-inline bool eq_h(double2 a, double2 b) { return a.x == b.x && a.y == b.y; }
+inline bool eq_h(zd a, zd b) { return a.x == b.x && a.y == b.y; }
 inline double neg_h(double a) { return -a; }
-inline double2 neg_h(double2 a) { return make_double2(-a.x, -a.y); }
+inline zd neg_h(zd a) { return make_zd(-a.x, -a.y); }
 inline double from_h(double x, double) { return x; }
-inline double2 from_h(double x, double2) { return make_double2(x, 0.0); }
+inline zd from_h(double x, zd) { return make_zd(x, 0.0); }
 template <typename T>
 constexpr bool discrete_signs()
 {
@@ -280,15 +280,15 @@ void gemm_t(Context* ctx, int m, int n, int k, double alpha, const T* a, int lda
     if constexpr (sizeof(T) == 8)
         dgemm(ctx, m, n, k, alpha, (const double*)a, lda, (const double*)b, ldb, beta, (double*)c, ldc);
     else
-        zgemm(ctx, m, n, k, make_double2(alpha, 0.0), (const double2*)a, lda, (const double2*)b, ldb,
-              make_double2(beta, 0.0), (double2*)c, ldc);
+        zgemm(ctx, m, n, k, make_zd(alpha, 0.0), (const zd*)a, lda, (const zd*)b, ldb,
+              make_zd(beta, 0.0), (zd*)c, ldc);
 }
 
 template <typename T>
 void norm_t(Context* ctx, int kind, int m, int n, const T* a, int lda, double* d_out)
 {
     if constexpr (sizeof(T) == 8) dnorm(ctx, kind, m, n, (const double*)a, lda, d_out);
-    else znorm(ctx, kind, m, n, (const double2*)a, lda, d_out);
+    else znorm(ctx, kind, m, n, (const zd*)a, lda, d_out);
 }
 
 template <typename T>
@@ -297,28 +297,28 @@ void panel_t(Context* ctx, int trans, int n, int t, const T* a, int lda, const T
     if constexpr (sizeof(T) == 8)
         dpanel(ctx, trans, n, t, (const double*)a, lda, (const double*)x, ldx, (double*)y, ldy);
     else
-        zpanel(ctx, trans, n, t, (const double2*)a, lda, (const double2*)x, ldx, (double2*)y, ldy);
+        zpanel(ctx, trans, n, t, (const zd*)a, lda, (const zd*)x, ldx, (zd*)y, ldy);
 }
 
 template <typename T>
 void copy_t(Context* ctx, int m, int n, const T* src, int lds, T* dst, int ldd)
 {
     if constexpr (sizeof(T) == 8) dcopy2d(ctx, m, n, (const double*)src, lds, (double*)dst, ldd);
-    else zcopy2d(ctx, m, n, (const double2*)src, lds, (double2*)dst, ldd);
+    else zcopy2d(ctx, m, n, (const zd*)src, lds, (zd*)dst, ldd);
 }
 
 template <typename T>
 void getrf_t(Context* ctx, int n, T* a, int lda, int* piv, int* perm, int* info)
 {
     if constexpr (sizeof(T) == 8) dgetrf(ctx, n, n, (double*)a, lda, 0, piv, perm, info);
-    else zgetrf(ctx, n, n, (double2*)a, lda, 0, piv, perm, info);
+    else zgetrf(ctx, n, n, (zd*)a, lda, 0, piv, perm, info);
 }
 
 template <typename T>
 void getrs_t(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu, const int* perm, T* b, int ldb)
 {
     if constexpr (sizeof(T) == 8) dgetrs(ctx, trans, n, nrhs, (const double*)lu, ldlu, perm, (double*)b, ldb);
-    else zgetrs(ctx, trans, n, nrhs, (const double2*)lu, ldlu, perm, (double2*)b, ldb);
+    else zgetrs(ctx, trans, n, nrhs, (const zd*)lu, ldlu, perm, (zd*)b, ldb);
 }
 
 int ceil_int(double x)
@@ -668,7 +668,7 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
         if constexpr (sizeof(T) == 8) r = std::exp(h);
         else {
             double ex = std::exp(h.x);
-            r = make_double2(ex * std::cos(h.y), ex * std::sin(h.y));
+            r = make_zd(ex * std::cos(h.y), ex * std::sin(h.y));
         }
         TNS_HIP(hipMemcpyAsync(e, &r, sizeof(T), hipMemcpyHostToDevice, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
@@ -797,7 +797,7 @@ T exp_scalar(T z)
     if constexpr (sizeof(T) == 8) return std::exp(z);
     else {
         double e = std::exp(z.x);
-        return make_double2(e * std::cos(z.y), e * std::sin(z.y));
+        return make_zd(e * std::cos(z.y), e * std::sin(z.y));
     }
 }
 
@@ -832,7 +832,7 @@ void run(Context* ctx, const DevOp<T>& op, const T* dense, int lda, int nb, cons
 
     T eta;
     if constexpr (sizeof(T) == 8) eta = exp_scalar<T>((mu * t) / (double)s);
-    else eta = exp_scalar<T>(s_div(s_scale(mu, t), make_double2((double)s, 0.0)));
+    else eta = exp_scalar<T>(s_div(s_scale(mu, t), make_zd((double)s, 0.0)));
     const bool unit_eta = eq_h(eta, from_h(1.0, T{}));
 
     const bool fused = dense != nullptr && (sizeof(T) == 8 ? nb <= 16 : nb <= 8);
@@ -890,7 +890,7 @@ void expmv(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb
     TNS_HIP(hipStreamSynchronize(ctx->stream));
     T mu;
     if constexpr (sizeof(T) == 8) mu = tr / n;
-    else mu = make_double2(tr.x / n, tr.y / n);
+    else mu = make_zd(tr.x / n, tr.y / n);
     shift_diag<T>(ctx, n, shifted.p(), n, neg_h(mu));
     const double anorm1 = one_norm<T>(ctx, n, shifted.p());
 
@@ -925,22 +925,22 @@ void expmv_op(Context* ctx, const DevOp<T>& op, int nb, const T* b, int ldb, dou
 }
 
 template NormEst normest<double>(Context*, const DevOp<double>&, int, int, int);
-template NormEst normest<double2>(Context*, const DevOp<double2>&, int, int, int);
+template NormEst normest<zd>(Context*, const DevOp<zd>&, int, int, int);
 template DevOp<double> power_operator<double>(Context*, int, const double*, int, int);
-template DevOp<double2> power_operator<double2>(Context*, int, const double2*, int, int);
+template DevOp<zd> power_operator<zd>(Context*, int, const zd*, int, int);
 template DevOp<double> inverse_operator<double>(Context*, int, const double*, int, const int*);
-template DevOp<double2> inverse_operator<double2>(Context*, int, const double2*, int, const int*);
+template DevOp<zd> inverse_operator<zd>(Context*, int, const zd*, int, const int*);
 template double gecon<double>(Context*, int, const double*, int, const int*, int, double, int);
-template double gecon<double2>(Context*, int, const double2*, int, const int*, int, double, int);
+template double gecon<zd>(Context*, int, const zd*, int, const int*, int, double, int);
 template void expm<double>(Context*, int, const double*, int, double*, int, ExpmDiag*);
-template void expm<double2>(Context*, int, const double2*, int, double2*, int, ExpmDiag*);
+template void expm<zd>(Context*, int, const zd*, int, zd*, int, ExpmDiag*);
 template void expmv<double>(Context*, int, int, const double*, int, const double*, int, double, double*, int,
                             ExpmvDiag*);
-template void expmv<double2>(Context*, int, int, const double2*, int, const double2*, int, double, double2*, int,
+template void expmv<zd>(Context*, int, int, const zd*, int, const zd*, int, double, zd*, int,
                              ExpmvDiag*);
 template void expmv_op<double>(Context*, const DevOp<double>&, int, const double*, int, double, double, double*, int,
                                ExpmvDiag*);
-template void expmv_op<double2>(Context*, const DevOp<double2>&, int, const double2*, int, double, double, double2*,
+template void expmv_op<zd>(Context*, const DevOp<zd>&, int, const zd*, int, double, double, zd*,
                                 int, ExpmvDiag*);
 
 }  // namespace tns

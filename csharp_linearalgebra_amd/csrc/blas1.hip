@@ -35,9 +35,9 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-__device__ __forceinline__ double2 wave_sum(double2 v)
+__device__ __forceinline__ zd wave_sum(zd v)
 {
-    return make_double2(wave_sum(v.x), wave_sum(v.y));
+    return make_zd(wave_sum(v.x), wave_sum(v.y));
 }
 
 template <typename T>
@@ -45,7 +45,7 @@ __device__ __forceinline__ double sq(T x);
 template <>
 __device__ __forceinline__ double sq<double>(double x) { return x * x; }
 template <>
-__device__ __forceinline__ double sq<double2>(double2 x) { return x.x * x.x + x.y * x.y; }
+__device__ __forceinline__ double sq<zd>(zd x) { return x.x * x.x + x.y * x.y; }
 
 // One wavefront per column: sum of |a_ij| (kind 0) or a_ij^2 (kind 2).
 template <typename T>
@@ -145,11 +145,12 @@ __global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __r
     if (kl == 0 && i < n) {
 #pragma unroll
         for (int c = 0; c < TMAX; c++) {
-            if (c >= t) break;
-            T s = red[0][c][lane];
+            if (c < t) {
+                T s = red[0][c][lane];
 #pragma unroll
-            for (int q = 1; q < PN_KL; q++) s = s_add(s, red[q][c][lane]);
-            part[((size_t)ks * t + c) * n + i] = s;
+                for (int q = 1; q < PN_KL; q++) s = s_add(s, red[q][c][lane]);
+                part[((size_t)ks * t + c) * n + i] = s;
+            }
         }
     }
 }
@@ -187,9 +188,10 @@ __global__ void __launch_bounds__(256) panel_t_kernel(int n, int t, const T* __r
     }
 #pragma unroll
     for (int c = 0; c < TMAX; c++) {
-        if (c >= t) break;
-        T s = wave_sum(acc[c]);
-        if (lane == 0) y[j + (size_t)c * ldy] = s;
+        if (c < t) {
+            T s = wave_sum(acc[c]);
+            if (lane == 0) y[j + (size_t)c * ldy] = s;
+        }
     }
 }
 
@@ -406,8 +408,8 @@ void panel_impl(Context* ctx, int trans, int n, int t, const T* a, int lda, cons
             dgemm_ex(ctx, trans ? 1 : 0, n, t, n, 1.0, (const double*)a, lda, (const double*)x, ldx, 0.0, (double*)y,
                      ldy);
         else
-            zgemm_ex(ctx, trans ? 2 : 0, n, t, n, make_double2(1.0, 0.0), (const double2*)a, lda, (const double2*)x,
-                     ldx, make_double2(0.0, 0.0), (double2*)y, ldy);
+            zgemm_ex(ctx, trans ? 2 : 0, n, t, n, make_zd(1.0, 0.0), (const zd*)a, lda, (const zd*)x,
+                     ldx, make_zd(0.0, 0.0), (zd*)y, ldy);
         return;
     }
     if (!trans) {
@@ -441,7 +443,7 @@ void dnorm(Context* ctx, int kind, int m, int n, const double* a, int lda, doubl
 
 void znorm(Context* ctx, int kind, int m, int n, const cplx* a, int lda, double* d_out)
 {
-    norm_impl<double2>(ctx, kind, m, n, a, lda, d_out);
+    norm_impl<zd>(ctx, kind, m, n, a, lda, d_out);
 }
 
 void dpanel(Context* ctx, int trans, int n, int t, const double* a, int lda, const double* x, int ldx, double* y,
@@ -452,7 +454,7 @@ void dpanel(Context* ctx, int trans, int n, int t, const double* a, int lda, con
 
 void zpanel(Context* ctx, int adjoint, int n, int t, const cplx* a, int lda, const cplx* x, int ldx, cplx* y, int ldy)
 {
-    panel_impl<double2>(ctx, adjoint, n, t, a, lda, x, ldx, y, ldy);
+    panel_impl<zd>(ctx, adjoint, n, t, a, lda, x, ldx, y, ldy);
 }
 
 template <typename T>
@@ -463,7 +465,7 @@ void lincomb(Context* ctx, int n, double c0, int terms, const LinTerms<T>& lt, T
     TNS_HIP(hipGetLastError());
 }
 template void lincomb<double>(Context*, int, double, int, const LinTerms<double>&, double*);
-template void lincomb<double2>(Context*, int, double, int, const LinTerms<double2>&, double2*);
+template void lincomb<zd>(Context*, int, double, int, const LinTerms<zd>&, zd*);
 
 template <typename T>
 void sub_add(Context* ctx, size_t count, const T* u, const T* v, T* den, T* num)
@@ -471,7 +473,7 @@ void sub_add(Context* ctx, size_t count, const T* u, const T* v, T* den, T* num)
     hipLaunchKernelGGL((sub_add_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, u, v, den, num);
 }
 template void sub_add<double>(Context*, size_t, const double*, const double*, double*, double*);
-template void sub_add<double2>(Context*, size_t, const double2*, const double2*, double2*, double2*);
+template void sub_add<zd>(Context*, size_t, const zd*, const zd*, zd*, zd*);
 
 template <typename T>
 void scale_to(Context* ctx, size_t count, const T* src, double f, T* dst)
@@ -480,7 +482,7 @@ void scale_to(Context* ctx, size_t count, const T* src, double f, T* dst)
     hipLaunchKernelGGL((scale_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, src, f, dst);
 }
 template void scale_to<double>(Context*, size_t, const double*, double, double*);
-template void scale_to<double2>(Context*, size_t, const double2*, double, double2*);
+template void scale_to<zd>(Context*, size_t, const zd*, double, zd*);
 
 template <typename T>
 void mul_scalar(Context* ctx, size_t count, T* x, T s)
@@ -489,7 +491,7 @@ void mul_scalar(Context* ctx, size_t count, T* x, T s)
     hipLaunchKernelGGL((mul_scalar_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, x, s);
 }
 template void mul_scalar<double>(Context*, size_t, double*, double);
-template void mul_scalar<double2>(Context*, size_t, double2*, double2);
+template void mul_scalar<zd>(Context*, size_t, zd*, zd);
 
 template <typename T>
 void shift_diag(Context* ctx, int n, T* a, int lda, T delta)
@@ -497,7 +499,7 @@ void shift_diag(Context* ctx, int n, T* a, int lda, T delta)
     hipLaunchKernelGGL((shift_diag_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, a, lda, delta);
 }
 template void shift_diag<double>(Context*, int, double*, int, double);
-template void shift_diag<double2>(Context*, int, double2*, int, double2);
+template void shift_diag<zd>(Context*, int, zd*, int, zd);
 
 template <typename T>
 void trace(Context* ctx, int n, const T* a, int lda, T* d_out)
@@ -505,7 +507,7 @@ void trace(Context* ctx, int n, const T* a, int lda, T* d_out)
     hipLaunchKernelGGL((trace_kernel<T>), dim3(1), dim3(1), 0, ctx->stream, n, a, lda, d_out);
 }
 template void trace<double>(Context*, int, const double*, int, double*);
-template void trace<double2>(Context*, int, const double2*, int, double2*);
+template void trace<zd>(Context*, int, const zd*, int, zd*);
 
 template <typename T>
 void magnitudes(Context* ctx, size_t count, const T* a, double* out)
@@ -513,7 +515,7 @@ void magnitudes(Context* ctx, size_t count, const T* a, double* out)
     hipLaunchKernelGGL((abs_kernel<T>), dim3(grid_for(count)), dim3(256), 0, ctx->stream, count, a, out);
 }
 template void magnitudes<double>(Context*, size_t, const double*, double*);
-template void magnitudes<double2>(Context*, size_t, const double2*, double*);
+template void magnitudes<zd>(Context*, size_t, const zd*, double*);
 
 void rowvec(Context* ctx, int n, const double* mags, const double* row, double* next)
 {
@@ -541,7 +543,7 @@ int panel_ksplit(int n)
     return ksplit_for(n);
 }
 template int panel_ksplit<double>(int);
-template int panel_ksplit<double2>(int);
+template int panel_ksplit<zd>(int);
 
 template <typename T>
 void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks)
@@ -549,7 +551,7 @@ void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* 
     panel_partials<T>(ctx, n, t, a, lda, x, ldx, part, ks);
 }
 template void panel_n_partials<double>(Context*, int, int, const double*, int, const double*, int, double*, int);
-template void panel_n_partials<double2>(Context*, int, int, const double2*, int, const double2*, int, double2*, int);
+template void panel_n_partials<zd>(Context*, int, int, const zd*, int, const zd*, int, zd*, int);
 
 template <typename T>
 void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms)
@@ -558,7 +560,7 @@ void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef
                        coef, work, f, d_norms);
 }
 template void taylor_step<double>(Context*, int, int, int, const double*, double, double*, double*, double*);
-template void taylor_step<double2>(Context*, int, int, int, const double2*, double, double2*, double2*, double*);
+template void taylor_step<zd>(Context*, int, int, int, const zd*, double, zd*, zd*, double*);
 
 template <typename T>
 void panel_infnorm(Context* ctx, int n, int nb, const T* x, double* d_out)
@@ -566,6 +568,6 @@ void panel_infnorm(Context* ctx, int n, int nb, const T* x, double* d_out)
     hipLaunchKernelGGL((panel_infnorm_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nb, x, d_out);
 }
 template void panel_infnorm<double>(Context*, int, int, const double*, double*);
-template void panel_infnorm<double2>(Context*, int, int, const double2*, double*);
+template void panel_infnorm<zd>(Context*, int, int, const zd*, double*);
 
 }  // namespace tns

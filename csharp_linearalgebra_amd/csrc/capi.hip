@@ -112,6 +112,8 @@ namespace {
 using tns::Context;
 using tns::DevBuf;
 using tns::fail;
+using tns::zd;
+using tns::make_zd;
 
 // A caller's matrix as the device sees it: the caller's own pointer when it
 // is device memory, else a packed device copy (ld = rows).
@@ -223,7 +225,7 @@ DevBuf<int> upload_ints(Context* ctx, const std::vector<int>& h)
     return d;
 }
 
-inline double2 c2(tns_complex z) { return make_double2(z.re, z.im); }
+inline zd c2(tns_complex z) { return make_zd(z.re, z.im); }
 
 // ---- generic bodies -----------------------------------------------------
 
@@ -236,10 +238,10 @@ void getrf_body(Context* ctx, int m, int n, T* a, int lda, int nb, int* pivots, 
     View<T> va = bring(ctx, a, m, n, lda, true);
     DevBuf<int> piv(ctx, (size_t)std::max(limit, 1)), perm(ctx, (size_t)std::max(m, 1));
     if constexpr (sizeof(T) == 8) tns::dgetrf(ctx, m, n, (double*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
-    else tns::zgetrf(ctx, m, n, (double2*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
+    else tns::zgetrf(ctx, m, n, (zd*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
     if (limit > 0) {
         if constexpr (sizeof(T) == 8) tns::dpivot_range(ctx, limit, (const double*)va.d, va.ld, ctx->d_scal);
-        else tns::zpivot_range(ctx, limit, (const double2*)va.d, va.ld, ctx->d_scal);
+        else tns::zpivot_range(ctx, limit, (const zd*)va.d, va.ld, ctx->d_scal);
         if (tns::is_device_pointer(pivots))
             TNS_HIP(hipMemcpyAsync(pivots, piv.get(), sizeof(int) * limit, hipMemcpyDeviceToDevice, ctx->stream));
         else
@@ -275,7 +277,7 @@ void getrs_body(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu,
     if constexpr (sizeof(T) == 8)
         tns::dgetrs(ctx, trans, n, nrhs, (const double*)vlu.d, vlu.ld, perm.get(), (double*)vb.d, vb.ld);
     else
-        tns::zgetrs(ctx, trans, n, nrhs, (const double2*)vlu.d, vlu.ld, perm.get(), (double2*)vb.d, vb.ld);
+        tns::zgetrs(ctx, trans, n, nrhs, (const zd*)vlu.d, vlu.ld, perm.get(), (zd*)vb.d, vb.ld);
     send_back(ctx, vb, b, n, nrhs, ldb);
     tns::sync(ctx);
 }
@@ -300,7 +302,7 @@ void gecon_body(Context* ctx, int n, const T* lu, int ldlu, const int* pivots, i
         packed.d = packed.buf.get();
         packed.ld = n;
         if constexpr (sizeof(T) == 8) tns::dcopy2d(ctx, n, n, (const double*)vlu.d, vlu.ld, (double*)packed.d, n);
-        else tns::zcopy2d(ctx, n, n, (const double2*)vlu.d, vlu.ld, (double2*)packed.d, n);
+        else tns::zcopy2d(ctx, n, n, (const zd*)vlu.d, vlu.ld, (zd*)packed.d, n);
         *rcond = tns::gecon<T>(ctx, n, packed.d, n, perm.get(), singular, anorm, columns);
     } else {
         *rcond = tns::gecon<T>(ctx, n, vlu.d, vlu.ld, perm.get(), singular, anorm, columns);
@@ -424,7 +426,7 @@ void panel_body(Context* ctx, int trans, int n, int t, const T* a, int lda, cons
     if constexpr (sizeof(T) == 8)
         tns::dpanel(ctx, trans, n, t, (const double*)va.d, va.ld, (const double*)vx.d, vx.ld, (double*)vy.d, vy.ld);
     else
-        tns::zpanel(ctx, trans, n, t, (const double2*)va.d, va.ld, (const double2*)vx.d, vx.ld, (double2*)vy.d,
+        tns::zpanel(ctx, trans, n, t, (const zd*)va.d, va.ld, (const zd*)vx.d, vx.ld, (zd*)vy.d,
                     vy.ld);
     send_back(ctx, vy, y, n, t, ldy);
     if (va.staged || vx.staged || vy.staged) tns::sync(ctx);
@@ -443,7 +445,7 @@ void norm_body(Context* ctx, char kind, int m, int n, const T* a, int lda, doubl
     }
     View<T> va = bring(ctx, a, m, n, lda, true);
     if constexpr (sizeof(T) == 8) tns::dnorm(ctx, kind, m, n, (const double*)va.d, va.ld, ctx->d_scal);
-    else tns::znorm(ctx, kind, m, n, (const double2*)va.d, va.ld, ctx->d_scal);
+    else tns::znorm(ctx, kind, m, n, (const zd*)va.d, va.ld, ctx->d_scal);
     *out = tns::read_double(ctx, ctx->d_scal);
 }
 
@@ -455,6 +457,16 @@ void norm_body(Context* ctx, char kind, int m, int n, const T* a, int lda, doubl
 extern "C" {
 
 TNS_API const char* tns_version(void) { return "tensile-hip 0.1 gfx950"; }
+
+TNS_API int tns_gemm_set_config(int real_config, int complex_config)
+{
+    if (real_config < -1 || real_config >= tns::gemm_config_count(false)) return TNS_EINVAL;
+    if (complex_config < -1 || complex_config >= tns::gemm_config_count(true)) return TNS_EINVAL;
+    tns::gemm_force_config(real_config, complex_config);
+    return TNS_OK;
+}
+
+TNS_API int tns_gemm_config_count(int complex_) { return tns::gemm_config_count(complex_ != 0); }
 
 TNS_API int tns_context_create(int device, tns_context** out)
 {
@@ -570,11 +582,11 @@ TNS_API int tns_zgemm(tns_context* ctx, int m, int n, int k, tns_complex alpha, 
         check_operand(a, m, k, lda, "A");
         check_operand(b, k, n, ldb, "B");
         check_operand(c, m, n, ldc, "C");
-        const double2 *pa = (const double2*)a, *pb = (const double2*)b;
-        double2* pc = (double2*)c;
-        View<double2> va = bring(cx, pa, m, k, lda, true);
-        View<double2> vb = bring(cx, pb, k, n, ldb, true);
-        View<double2> vc = bring(cx, (const double2*)pc, m, n, ldc, !(beta.re == 0.0 && beta.im == 0.0));
+        const zd *pa = (const zd*)a, *pb = (const zd*)b;
+        zd* pc = (zd*)c;
+        View<zd> va = bring(cx, pa, m, k, lda, true);
+        View<zd> vb = bring(cx, pb, k, n, ldb, true);
+        View<zd> vc = bring(cx, (const zd*)pc, m, n, ldc, !(beta.re == 0.0 && beta.im == 0.0));
         tns::zgemm(cx, m, n, k, c2(alpha), va.d, va.ld, vb.d, vb.ld, c2(beta), vc.d, vc.ld);
         send_back(cx, vc, pc, m, n, ldc);
         if (va.staged || vb.staged || vc.staged) tns::sync(cx);
@@ -589,7 +601,7 @@ TNS_API int tns_dgetrf(tns_context* ctx, int m, int n, double* a, int lda, int n
 TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
                        tns_lu_info* info)
 {
-    return guarded(ctx, [&](Context* c) { getrf_body<double2>(c, m, n, (double2*)a, lda, nb, pivots, info); });
+    return guarded(ctx, [&](Context* c) { getrf_body<zd>(c, m, n, (zd*)a, lda, nb, pivots, info); });
 }
 
 TNS_API int tns_dgetrs(tns_context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* pivots,
@@ -604,7 +616,7 @@ TNS_API int tns_zgetrs(tns_context* ctx, int trans, int n, int nrhs, const tns_c
                        const int* pivots, int singular_column, tns_complex* b, int ldb)
 {
     return guarded(ctx, [&](Context* c) {
-        getrs_body<double2>(c, trans, n, nrhs, (const double2*)lu, ldlu, pivots, singular_column, (double2*)b, ldb);
+        getrs_body<zd>(c, trans, n, nrhs, (const zd*)lu, ldlu, pivots, singular_column, (zd*)b, ldb);
     });
 }
 
@@ -633,7 +645,7 @@ TNS_API int tns_zpanel(tns_context* ctx, int adjoint, int n, int t, const tns_co
                        const tns_complex* x, int ldx, tns_complex* y, int ldy)
 {
     return guarded(ctx, [&](Context* c) {
-        panel_body<double2>(c, adjoint, n, t, (const double2*)a, lda, (const double2*)x, ldx, (double2*)y, ldy);
+        panel_body<zd>(c, adjoint, n, t, (const zd*)a, lda, (const zd*)x, ldx, (zd*)y, ldy);
     });
 }
 
@@ -644,7 +656,7 @@ TNS_API int tns_dnorm(tns_context* ctx, char kind, int m, int n, const double* a
 
 TNS_API int tns_znorm(tns_context* ctx, char kind, int m, int n, const tns_complex* a, int lda, double* out)
 {
-    return guarded(ctx, [&](Context* c) { norm_body<double2>(c, kind, m, n, (const double2*)a, lda, out); });
+    return guarded(ctx, [&](Context* c) { norm_body<zd>(c, kind, m, n, (const zd*)a, lda, out); });
 }
 
 TNS_API int tns_dnormest_power(tns_context* ctx, int n, const double* a, int lda, int power, int columns,
@@ -659,7 +671,7 @@ TNS_API int tns_znormest_power(tns_context* ctx, int n, const tns_complex* a, in
                                int max_iterations, int seed, tns_normest* out)
 {
     return guarded(ctx, [&](Context* c) {
-        normest_body<double2>(c, n, (const double2*)a, lda, power, columns, max_iterations, seed, out);
+        normest_body<zd>(c, n, (const zd*)a, lda, power, columns, max_iterations, seed, out);
     });
 }
 
@@ -675,7 +687,7 @@ TNS_API int tns_zgecon(tns_context* ctx, int n, const tns_complex* lu, int ldlu,
                        int singular_column, double anorm, int columns, double* rcond)
 {
     return guarded(ctx, [&](Context* c) {
-        gecon_body<double2>(c, n, (const double2*)lu, ldlu, pivots, singular_column, anorm, columns, rcond);
+        gecon_body<zd>(c, n, (const zd*)lu, ldlu, pivots, singular_column, anorm, columns, rcond);
     });
 }
 
@@ -688,7 +700,7 @@ TNS_API int tns_zexpm(tns_context* ctx, int n, const tns_complex* a, int lda, tn
                       tns_expm_diag* diag)
 {
     return guarded(ctx, [&](Context* c) {
-        expm_body<double2>(c, n, (const double2*)a, lda, (double2*)e, lde, diag);
+        expm_body<zd>(c, n, (const zd*)a, lda, (zd*)e, lde, diag);
     });
 }
 
@@ -702,7 +714,7 @@ TNS_API int tns_zexpmv(tns_context* ctx, int n, int nb, const tns_complex* a, in
                        double t, tns_complex* f, int ldf, tns_expmv_diag* diag)
 {
     return guarded(ctx, [&](Context* c) {
-        expmv_body<double2>(c, n, nb, (const double2*)a, lda, (const double2*)b, ldb, t, (double2*)f, ldf, diag);
+        expmv_body<zd>(c, n, nb, (const zd*)a, lda, (const zd*)b, ldb, t, (zd*)f, ldf, diag);
     });
 }
 
@@ -720,8 +732,8 @@ TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* us
                           tns_expmv_diag* diag)
 {
     return guarded(ctx, [&](Context* c) {
-        expmv_op_body<double2>(c, n, apply, user, device_operator, nb, (const double2*)b, ldb, t, one_norm_bound,
-                               (double2*)f, ldf, diag);
+        expmv_op_body<zd>(c, n, apply, user, device_operator, nb, (const zd*)b, ldb, t, one_norm_bound,
+                               (zd*)f, ldf, diag);
     });
 }
 

@@ -5,9 +5,10 @@
 //
 // Design, per workgroup: a BM x BN tile of C, 4 waves each owning a WM x WN
 // sub-tile built from 16x16 v_mfma_f64_16x16x4_f64 blocks held in
-// registers. A and B stream through LDS in BK-deep slabs, double-buffered:
-// the next slab is fetched into registers while the current one feeds the
-// matrix cores, then written to the other LDS buffer, one barrier per slab.
+// registers. A and B stream through LDS in 16-deep slabs: the next slab is
+// fetched into registers (clamped, branch-free loads, all in flight at once)
+// while the current one feeds the matrix cores, then written to LDS between
+// two barriers.
 // Complex operands are split into real and imaginary LDS planes as they are
 // staged, and each complex 16x16 block takes four real MFMAs,
 //     Re += Ar*Br - Ai*Bi,   Im += Ar*Bi + Ai*Br,
@@ -55,19 +56,50 @@ struct Shape {
     static constexpr int TN = WN / 16;
     static constexpr int LA = BM * BK / NT;
     static constexpr int LB = BK * BN / NT;
+    static constexpr int KP = BK + 2;  // padded k-stride of the k-contiguous LDS planes
     static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "slab must split evenly over threads");
     static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA granularity");
+    static_assert(LA % 2 == 0 && LB % 2 == 0, "paired loads need an even count per thread");
 };
 
+// LDS planes of one slab. A is held [k][i] (i contiguous) when it is read
+// untransposed and [i][k] (k contiguous, padded) when op(A) = A^T, so the
+// global loads stay coalesced either way; B is [j][k] (k contiguous,
+// padded). With the k-stride BK + 2 the 16 lanes of a fragment read touch
+// distinct bank pairs.
+template <int BM, int BN, int BK, int TA>
+struct Planes {
+    static constexpr int KP = BK + 2;
+    static constexpr int A_SIZE = TA == 0 ? BK * BM : BM * KP;
+    static constexpr int B_SIZE = BN * KP;
+    __device__ static int a_index(int i, int k) { return TA == 0 ? k * BM + i : i * KP + k; }
+    __device__ static int b_index(int k, int j) { return j * KP + k; }
+};
+
+// Global index of the e-th element of an A slab (rows i, depth kk), laid
+// out so consecutive threads read consecutive memory.
+template <int BM, int BK, int TA>
+__device__ __forceinline__ void a_coords(int e, int& i, int& kk)
+{
+    if (TA == 0) {
+        i = e % BM;
+        kk = e / BM;
+    } else {
+        kk = e % BK;
+        i = e / BK;
+    }
+}
+
 template <int BM, int BN, int BK, int WM, int WN, int TA>
-__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, (BM / WM) * (BN / WN) * 64 > 256 ? 1 : 2)
 dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, int lda,
              const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
-             int tiles_m, int tiles_n)
+             int tiles_m, int tiles_n, int aligned)
 {
     using S = Shape<BM, BN, BK, WM, WN>;
-    __shared__ double As[2][BK][BM + 1];
-    __shared__ double Bs[2][BK][BN + 1];
+    using P = Planes<BM, BN, BK, TA>;
+    __shared__ double As[P::A_SIZE];
+    __shared__ double Bs[P::B_SIZE];
 
     int tmi, tni;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
@@ -83,45 +115,100 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
 
     double ra[S::LA], rb[S::LB];
 
+    // Interior slabs of aligned operands load 16 bytes per lane (two
+    // neighbours along the contiguous dimension); elsewhere loads are
+    // clamped into the matrix and zeroed after, never branched around, so
+    // every load of a slab is in flight at once.
+    const bool m_inside = m0 + BM <= m, n_inside = n0 + BN <= n;
+    bool vec = false, vec_a = false, vec_b = false;
     auto gload = [&](int k0) {
+        const bool k_inside = k0 + BK <= k;
+        if (aligned && k_inside && (TA == 0 ? m_inside : m_inside)) {
 #pragma unroll
-        for (int r = 0; r < S::LA; r++) {
-            int e = tid + r * S::NT;
-            if (TA == 0) {
-                int i = e % BM, kk = e / BM;
-                int gi = m0 + i, gk = k0 + kk;
-                ra[r] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : 0.0;
-            } else {
-                int kk = e % BK, i = e / BK;
-                int gi = m0 + i, gk = k0 + kk;
-                ra[r] = (gi < m && gk < k) ? A[gk + (size_t)gi * lda] : 0.0;
+            for (int r = 0; r < S::LA / 2; r++) {
+                const int e = tid + r * S::NT;
+                int i, kk;
+                if (TA == 0) { i = 2 * (e % (BM / 2)); kk = e / (BM / 2); }
+                else { kk = 2 * (e % (BK / 2)); i = e / (BK / 2); }
+                const double2 v = TA == 0 ? *reinterpret_cast<const double2*>(A + (m0 + i) + (size_t)(k0 + kk) * lda)
+                                          : *reinterpret_cast<const double2*>(A + (k0 + kk) + (size_t)(m0 + i) * lda);
+                ra[2 * r] = v.x;
+                ra[2 * r + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < S::LA; r++) {
+                int i, kk;
+                a_coords<BM, BK, TA>(tid + r * S::NT, i, kk);
+                const int gi = m0 + i, gk = k0 + kk;
+                const int ci = gi < m ? gi : m - 1, ck = gk < k ? gk : k - 1;
+                const double v = TA == 0 ? A[ci + (size_t)ck * lda] : A[ck + (size_t)ci * lda];
+                ra[r] = (gi < m && gk < k) ? v : 0.0;
             }
         }
+        if (aligned && k_inside && n_inside) {
 #pragma unroll
-        for (int r = 0; r < S::LB; r++) {
-            int e = tid + r * S::NT;
-            int kk = e % BK, j = e / BK;
-            int gk = k0 + kk, gj = n0 + j;
-            rb[r] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : 0.0;
+            for (int r = 0; r < S::LB / 2; r++) {
+                const int e = tid + r * S::NT;
+                const int kk = 2 * (e % (BK / 2)), j = e / (BK / 2);
+                const double2 v = *reinterpret_cast<const double2*>(B + (k0 + kk) + (size_t)(n0 + j) * ldb);
+                rb[2 * r] = v.x;
+                rb[2 * r + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < S::LB; r++) {
+                const int e = tid + r * S::NT;
+                const int kk = e % BK, j = e / BK;
+                const int gk = k0 + kk, gj = n0 + j;
+                const int ck = gk < k ? gk : k - 1, cj = gj < n ? gj : n - 1;
+                const double v = B[ck + (size_t)cj * ldb];
+                rb[r] = (gk < k && gj < n) ? v : 0.0;
+            }
         }
+        vec = aligned && k_inside;
+        vec_a = vec && m_inside;
+        vec_b = vec && n_inside;
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&]() {
+        if (vec_a) {
 #pragma unroll
-        for (int r = 0; r < S::LA; r++) {
-            int e = tid + r * S::NT;
-            if (TA == 0) As[buf][e / BM][e % BM] = ra[r];
-            else As[buf][e % BK][e / BK] = ra[r];
+            for (int r = 0; r < S::LA / 2; r++) {
+                const int e = tid + r * S::NT;
+                int i, kk;
+                if (TA == 0) { i = 2 * (e % (BM / 2)); kk = e / (BM / 2); }
+                else { kk = 2 * (e % (BK / 2)); i = e / (BK / 2); }
+                As[P::a_index(i, kk)] = ra[2 * r];
+                As[P::a_index(TA == 0 ? i + 1 : i, TA == 0 ? kk : kk + 1)] = ra[2 * r + 1];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < S::LA; r++) {
+                int i, kk;
+                a_coords<BM, BK, TA>(tid + r * S::NT, i, kk);
+                As[P::a_index(i, kk)] = ra[r];
+            }
         }
+        if (vec_b) {
 #pragma unroll
-        for (int r = 0; r < S::LB; r++) {
-            int e = tid + r * S::NT;
-            Bs[buf][e % BK][e / BK] = rb[r];
+            for (int r = 0; r < S::LB / 2; r++) {
+                const int e = tid + r * S::NT;
+                const int kk = 2 * (e % (BK / 2)), j = e / (BK / 2);
+                Bs[P::b_index(kk, j)] = rb[2 * r];
+                Bs[P::b_index(kk + 1, j)] = rb[2 * r + 1];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < S::LB; r++) {
+                const int e = tid + r * S::NT;
+                Bs[P::b_index(e % BK, e / BK)] = rb[r];
+            }
         }
     };
 
     const int ktiles = (k + BK - 1) / BK;
     gload(0);
-    sstore(0);
+    sstore();
     __syncthreads();
 
     const int arow = wm * WM + (lane & 15);
@@ -129,22 +216,24 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
     const int kq = lane >> 4;
 
     for (int t = 0; t < ktiles; ++t) {
-        const int cur = t & 1;
         if (t + 1 < ktiles) gload((t + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double af[S::TM], bf[S::TN];
 #pragma unroll
-            for (int i = 0; i < S::TM; i++) af[i] = As[cur][kk + kq][arow + i * 16];
+            for (int i = 0; i < S::TM; i++) af[i] = As[P::a_index(arow + i * 16, kk + kq)];
 #pragma unroll
-            for (int j = 0; j < S::TN; j++) bf[j] = Bs[cur][kk + kq][bcol + j * 16];
+            for (int j = 0; j < S::TN; j++) bf[j] = Bs[P::b_index(kk + kq, bcol + j * 16)];
 #pragma unroll
             for (int i = 0; i < S::TM; i++)
 #pragma unroll
                 for (int j = 0; j < S::TN; j++) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
         }
-        if (t + 1 < ktiles) sstore(cur ^ 1);
         __syncthreads();
+        if (t + 1 < ktiles) {
+            sstore();
+            __syncthreads();
+        }
     }
 
     // D block layout of v_mfma_f64_16x16x4_f64 (one-element groups): lane l,
@@ -168,18 +257,19 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
 
 // Complex, with ComplexKernels.Combine's scalar special cases folded into
 // the write-back (ComplexKernels.cs:213-240). flags: 1 unit alpha, 2 real
-// alpha, 4 zero beta, 8 unit beta.
+// alpha, 4 zero beta, 8 unit beta. TA: 0 A, 1 A^T, 2 A^H.
 template <int BM, int BN, int BK, int WM, int WN, int TA>
-__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, 2)
 zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A, int lda,
              const double2* __restrict__ B, int ldb, double br_, double bi_, double2* __restrict__ C,
              int ldc, int flags, int tiles_m, int tiles_n)
 {
     using S = Shape<BM, BN, BK, WM, WN>;
-    __shared__ double Ar[2][BK][BM + 1];
-    __shared__ double Ai[2][BK][BM + 1];
-    __shared__ double Br[2][BK][BN + 1];
-    __shared__ double Bi[2][BK][BN + 1];
+    using P = Planes<BM, BN, BK, TA>;
+    __shared__ double Ar[P::A_SIZE];
+    __shared__ double Ai[P::A_SIZE];
+    __shared__ double Br[P::B_SIZE];
+    __shared__ double Bi[P::B_SIZE];
 
     int tmi, tni;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
@@ -196,55 +286,51 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
             im[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
         }
 
-    double2 ra[S::LA], rb[S::LB];
-    const double2 zero2 = make_double2(0.0, 0.0);
+    double ra_r[S::LA], ra_i[S::LA], rb_r[S::LB], rb_i[S::LB];
 
     auto gload = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < S::LA; r++) {
-            int e = tid + r * S::NT;
-            if (TA == 0) {
-                int i = e % BM, kk = e / BM;
-                int gi = m0 + i, gk = k0 + kk;
-                ra[r] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : zero2;
-            } else {
-                int kk = e % BK, i = e / BK;
-                int gi = m0 + i, gk = k0 + kk;
-                ra[r] = (gi < m && gk < k) ? A[gk + (size_t)gi * lda] : zero2;
-                if (TA == 2) ra[r].y = -ra[r].y;
-            }
+            int i, kk;
+            a_coords<BM, BK, TA>(tid + r * S::NT, i, kk);
+            const int gi = m0 + i, gk = k0 + kk;
+            const int ci = gi < m ? gi : m - 1, ck = gk < k ? gk : k - 1;
+            const double2 v = TA == 0 ? A[ci + (size_t)ck * lda] : A[ck + (size_t)ci * lda];
+            const bool ok = gi < m && gk < k;
+            ra_r[r] = ok ? v.x : 0.0;
+            ra_i[r] = ok ? (TA == 2 ? -v.y : v.y) : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < S::LB; r++) {
-            int e = tid + r * S::NT;
-            int kk = e % BK, j = e / BK;
-            int gk = k0 + kk, gj = n0 + j;
-            rb[r] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : zero2;
+            const int e = tid + r * S::NT;
+            const int kk = e % BK, j = e / BK;
+            const int gk = k0 + kk, gj = n0 + j;
+            const int ck = gk < k ? gk : k - 1, cj = gj < n ? gj : n - 1;
+            const double2 v = B[ck + (size_t)cj * ldb];
+            const bool ok = gk < k && gj < n;
+            rb_r[r] = ok ? v.x : 0.0;
+            rb_i[r] = ok ? v.y : 0.0;
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&]() {
 #pragma unroll
         for (int r = 0; r < S::LA; r++) {
-            int e = tid + r * S::NT;
-            if (TA == 0) {
-                Ar[buf][e / BM][e % BM] = ra[r].x;
-                Ai[buf][e / BM][e % BM] = ra[r].y;
-            } else {
-                Ar[buf][e % BK][e / BK] = ra[r].x;
-                Ai[buf][e % BK][e / BK] = ra[r].y;
-            }
+            int i, kk;
+            a_coords<BM, BK, TA>(tid + r * S::NT, i, kk);
+            Ar[P::a_index(i, kk)] = ra_r[r];
+            Ai[P::a_index(i, kk)] = ra_i[r];
         }
 #pragma unroll
         for (int r = 0; r < S::LB; r++) {
-            int e = tid + r * S::NT;
-            Br[buf][e % BK][e / BK] = rb[r].x;
-            Bi[buf][e % BK][e / BK] = rb[r].y;
+            const int e = tid + r * S::NT;
+            Br[P::b_index(e % BK, e / BK)] = rb_r[r];
+            Bi[P::b_index(e % BK, e / BK)] = rb_i[r];
         }
     };
 
     const int ktiles = (k + BK - 1) / BK;
     gload(0);
-    sstore(0);
+    sstore();
     __syncthreads();
 
     const int arow = wm * WM + (lane & 15);
@@ -252,21 +338,20 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
     const int kq = lane >> 4;
 
     for (int t = 0; t < ktiles; ++t) {
-        const int cur = t & 1;
         if (t + 1 < ktiles) gload((t + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double a_r[S::TM], a_i[S::TM], a_n[S::TM], b_r[S::TN], b_i[S::TN];
 #pragma unroll
             for (int i = 0; i < S::TM; i++) {
-                a_r[i] = Ar[cur][kk + kq][arow + i * 16];
-                a_i[i] = Ai[cur][kk + kq][arow + i * 16];
+                a_r[i] = Ar[P::a_index(arow + i * 16, kk + kq)];
+                a_i[i] = Ai[P::a_index(arow + i * 16, kk + kq)];
                 a_n[i] = -a_i[i];
             }
 #pragma unroll
             for (int j = 0; j < S::TN; j++) {
-                b_r[j] = Br[cur][kk + kq][bcol + j * 16];
-                b_i[j] = Bi[cur][kk + kq][bcol + j * 16];
+                b_r[j] = Br[P::b_index(kk + kq, bcol + j * 16)];
+                b_i[j] = Bi[P::b_index(kk + kq, bcol + j * 16)];
             }
 #pragma unroll
             for (int i = 0; i < S::TM; i++)
@@ -278,8 +363,11 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
                     im[i][j] = mfma_f64(a_i[i], b_r[j], im[i][j]);
                 }
         }
-        if (t + 1 < ktiles) sstore(cur ^ 1);
         __syncthreads();
+        if (t + 1 < ktiles) {
+            sstore();
+            __syncthreads();
+        }
     }
 
     const bool unit_alpha = flags & 1, real_alpha = flags & 2, zero_beta = flags & 4, unit_beta = flags & 8;
@@ -337,8 +425,9 @@ void launch_d(Context* ctx, int m, int n, int k, double alpha, const double* a, 
               double beta, double* c, int ldc)
 {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    const int aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0;
     hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tm * tn), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
-                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn);
+                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn, aligned);
 }
 
 template <int BM, int BN, int BK, int WM, int WN, int TA>
@@ -347,7 +436,9 @@ void launch_z(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int 
 {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     hipLaunchKernelGGL((zgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tm * tn), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
-                       ctx->stream, m, n, k, alpha.x, alpha.y, a, lda, b, ldb, beta.x, beta.y, c, ldc, flags, tm, tn);
+                       ctx->stream, m, n, k, alpha.x, alpha.y, reinterpret_cast<const double2*>(a), lda,
+                       reinterpret_cast<const double2*>(b), ldb, beta.x, beta.y, reinterpret_cast<double2*>(c), ldc,
+                       flags, tm, tn);
 }
 
 long tiles(int m, int n, int bm, int bn) { return (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn); }
@@ -375,12 +466,14 @@ void zscale_c(Context* ctx, int m, int n, cplx beta, cplx* c, int ldc)
     if ((beta.x == 1.0 && beta.y == 0.0) || m == 0 || n == 0) return;
     size_t total = (size_t)m * n;
     int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
-    hipLaunchKernelGGL(zscale_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, beta.x, beta.y, c, ldc);
+    hipLaunchKernelGGL(zscale_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, beta.x, beta.y,
+                       reinterpret_cast<double2*>(c), ldc);
 }
 
 // Real: 0 = 128x128 (4 waves of 64x64), 1 = 128x64 (4 of 64x32),
-// 2 = 64x64 (4 of 32x32). The largest tile that still gives every CU a
-// workgroup wins; small problems take the 64x64 tile.
+// 2 = 64x64 (4 of 32x32). Measured on MI355X the 64x64 tile is the fastest
+// or level up to n = 4096 and for the thin-k updates of the factorisation;
+// the 128x128 tile pays only once there are four of its tiles per CU.
 void dgemm_ex(Context* ctx, int ta, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
               int ldb, double beta, double* c, int ldc)
 {
@@ -391,14 +484,12 @@ void dgemm_ex(Context* ctx, int ta, int m, int n, int k, double alpha, const dou
     }
     int cfg = g_real_cfg;
     if (cfg < 0) {
-        if (tiles(m, n, 128, 128) >= 256) cfg = 0;
-        else if (tiles(m, n, 128, 64) >= 256) cfg = 1;
-        else cfg = 2;
+        cfg = tiles(m, n, 128, 128) >= 1024 && k >= 1024 ? 0 : 2;
     }
 #define TNS_D(TA_)                                                                                          \
     switch (cfg) {                                                                                         \
-    case 0: launch_d<128, 128, 8, 64, 64, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break; \
-    case 1: launch_d<128, 64, 8, 64, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break;  \
+    case 0: launch_d<128, 128, 16, 64, 64, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break; \
+    case 1: launch_d<128, 64, 16, 64, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break;  \
     default: launch_d<64, 64, 16, 32, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); break; \
     }
     if (ta == 0) { TNS_D(0) } else { TNS_D(1) }
@@ -423,15 +514,15 @@ void zgemm_ex(Context* ctx, int ta, int m, int n, int k, cplx alpha, const cplx*
     if (beta.x == 1.0 && beta.y == 0.0) flags |= 8;
     int cfg = g_cplx_cfg;
     if (cfg < 0) {
-        if (tiles(m, n, 128, 64) >= 256) cfg = 0;
-        else if (tiles(m, n, 64, 64) >= 160) cfg = 1;
+        if (tiles(m, n, 128, 64) >= 512) cfg = 0;
+        else if (tiles(m, n, 64, 64) >= 512) cfg = 1;
         else cfg = 2;
     }
 #define TNS_Z(TA_)                                                                                                  \
     switch (cfg) {                                                                                                 \
-    case 0: launch_z<128, 64, 8, 64, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;  \
-    case 1: launch_z<64, 64, 8, 32, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;   \
-    default: launch_z<64, 32, 8, 32, 16, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;  \
+    case 0: launch_z<128, 64, 16, 64, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;  \
+    case 1: launch_z<64, 64, 16, 32, 32, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;   \
+    default: launch_z<64, 32, 16, 32, 16, TA_>(ctx, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, flags); break;  \
     }
     if (ta == 0) { TNS_Z(0) } else if (ta == 1) { TNS_Z(1) } else { TNS_Z(2) }
 #undef TNS_Z

@@ -75,16 +75,59 @@ __device__ __forceinline__ double shfl_t<double>(double v, int src)
     return __shfl(v, src, 64);
 }
 template <>
-__device__ __forceinline__ double2 shfl_t<double2>(double2 v, int src)
+__device__ __forceinline__ zd shfl_t<zd>(zd v, int src)
 {
-    return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+    return make_zd(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
 __device__ __forceinline__ double shfl_xor_t(double v, int off) { return __shfl_xor(v, off, 64); }
-__device__ __forceinline__ double2 shfl_xor_t(double2 v, int off)
+__device__ __forceinline__ zd shfl_xor_t(zd v, int off)
 {
-    return make_double2(__shfl_xor(v.x, off, 64), __shfl_xor(v.y, off, 64));
+    return make_zd(__shfl_xor(v.x, off, 64), __shfl_xor(v.y, off, 64));
 }
+
+// v[c] for a runtime c, and v[c] := x, by bitwise selection over
+// compile-time indices, so the array stays in registers. (A chain of
+// conditional moves is folded by the compiler back into an indexed load,
+// which demotes the array to scratch memory; masks are not.)
+__device__ __forceinline__ unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ double double_of(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+template <int N>
+__device__ __forceinline__ double pick(const double (&v)[N], int c)
+{
+    unsigned long long acc = 0;
+#pragma unroll
+    for (int q = 0; q < N; q++) acc |= bits_of(v[q]) & (0ull - (unsigned long long)(q == c));
+    return double_of(acc);
+}
+
+template <int N>
+__device__ __forceinline__ zd pick(const zd (&v)[N], int c)
+{
+    unsigned long long re = 0, im = 0;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        const unsigned long long mask = 0ull - (unsigned long long)(q == c);
+        re |= bits_of(v[q].x) & mask;
+        im |= bits_of(v[q].y) & mask;
+    }
+    return zd{double_of(re), double_of(im)};
+}
+
+// Diagonal block order of the triangular solves: 64 real, 32 complex (a
+// 64-element complex column does not fit the register file, nor a 64 x 64
+// complex block in one workgroup's LDS).
+template <typename T>
+constexpr int tri_block() { return sizeof(T) == 8 ? 64 : 32; }
+
+// y if take, else x, by masks.
+__device__ __forceinline__ double blend(double x, double y, bool take)
+{
+    const unsigned long long mask = 0ull - (unsigned long long)take;
+    return double_of((bits_of(x) & ~mask) | (bits_of(y) & mask));
+}
+__device__ __forceinline__ zd blend(zd x, zd y, bool take) { return zd{blend(x.x, y.x, take), blend(x.y, y.y, take)}; }
 
 // ---------------------------------------------------------------------------
 // Panel factorization, register resident. m <= blockDim.x * R, w <= NB.
@@ -92,7 +135,9 @@ __device__ __forceinline__ double2 shfl_xor_t(double2 v, int off)
 // are payloads that travel with the rows (indexed by local row, offset by
 // the caller): the original row index, the row's index at the start of the
 // block step, and (if rc is not null) at the start of this chunk. info[0] is
-// min-reduced with the first zero-pivot column.
+// min-reduced with the first zero-pivot column. The column loop is not
+// unrolled; every register access uses a compile-time index, with the
+// current column chosen by predicates.
 // ---------------------------------------------------------------------------
 template <typename T, int NB, int R>
 __global__ void __launch_bounds__(512)
@@ -123,17 +168,16 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
     }
 
     const int jl = m < w ? m : w;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        if (j >= jl) break;
+#pragma unroll 1
+    for (int j = 0; j < jl; ++j) {
         const int buf = j & 1;
-
         double bv = -2.0;
         int bi = INT_MAX;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int row = tid + r * nt;
-            if (row >= j && row < m) best_combine(bv, bi, pivot_key(v[r][j], row, j), row);
+            const T x = pick(v[r], j);
+            if (row >= j && row < m) best_combine(bv, bi, pivot_key(x, row, j), row);
         }
         wave_best(bv, bi);
         if (lane == 0) {
@@ -194,12 +238,15 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
                 p2[r] = pay[buf][5];
             }
             if (row > j && row < m) {
-                T l = v[r][j];
+                T l = pick(v[r], j);
                 if (!zero_pivot) l = (mag >= kSafeMin) ? s_mul(l, recip) : s_div(l, pv);
-                v[r][j] = l;
+                // j, w and the pivot row are uniform: these are scalar
+                // branches, not per-lane selects.
 #pragma unroll
-                for (int c = j + 1; c < NB; c++) {
-                    if (c < w) {
+                for (int c = 0; c < NB; c++) {
+                    if (c == j) {
+                        v[r][c] = l;
+                    } else if (c > j && c < w) {
                         const T u = prow[buf][c];
                         if (!s_iszero(u)) v[r][c] = s_add(v[r][c], s_mul(s_neg(u), l));
                     }
@@ -327,24 +374,28 @@ __global__ void compact_moves_kernel(int count, int base, int* __restrict__ rowi
 }
 
 // Apply a move list (rows relative to `a`) to columns [0, c0) and [c1, ncols).
-// Every source value is read before any destination is written.
-template <typename T, int MAXMOV>
-__global__ void __launch_bounds__(64)
+// A workgroup owns 256/QM columns and QM >= count move slots per column:
+// every thread reads its source element, one barrier, every thread writes
+// its destination, so all reads of a column precede its writes.
+template <typename T, int QM>
+__global__ void __launch_bounds__(256)
 apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const int* __restrict__ list)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    int left = c0, right = ncols - c1;
-    if (idx >= left + right) return;
-    int j = idx < left ? idx : c1 + (idx - left);
-    const int count = list[0] < MAXMOV ? list[0] : MAXMOV;
-    T* col = a + (size_t)j * lda;
-    T vals[MAXMOV];
-#pragma unroll
-    for (int q = 0; q < MAXMOV; q++)
-        if (q < count) vals[q] = col[list[2 + 2 * q]];
-#pragma unroll
-    for (int q = 0; q < MAXMOV; q++)
-        if (q < count) col[list[1 + 2 * q]] = vals[q];
+    constexpr int CPB = 256 / QM;
+    const int q = threadIdx.x % QM;
+    const int idx = blockIdx.x * CPB + threadIdx.x / QM;
+    const int left = c0, right = ncols - c1;
+    const int count = list[0] < QM ? list[0] : QM;
+    const bool active = idx < left + right && q < count;
+    T val{};
+    T* col = nullptr;
+    if (active) {
+        const int j = idx < left ? idx : c1 + (idx - left);
+        col = a + (size_t)j * lda;
+        val = col[list[2 + 2 * q]];
+    }
+    __syncthreads();
+    if (active) col[list[1 + 2 * q]] = val;
 }
 
 // Larger move lists (never produced by nb <= 64, kept for generality).
@@ -371,21 +422,26 @@ template <typename T, int W>
 __global__ void __launch_bounds__(64)
 trsm_llnu_kernel(int w, int ncols, const T* __restrict__ L, int ldl, T* __restrict__ B, int ldb)
 {
+    __shared__ T Ls[W][W];  // Ls[p][i] = L[i + p*ldl]: column p contiguous
+    for (int e = threadIdx.x; e < W * W; e += 64) {
+        const int i = e % W, p = e / W;
+        Ls[p][i] = (i < w && p < w) ? L[i + (size_t)p * ldl] : s_zero(T{});
+    }
+    __syncthreads();
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     T* col = B + (size_t)c * ldb;
     T x[W];
 #pragma unroll
     for (int i = 0; i < W; i++) x[i] = i < w ? col[i] : s_zero(T{});
-#pragma unroll
-    for (int p = 0; p < W; p++) {
-        if (p >= w) break;
-        const T xp = x[p];
+#pragma unroll 1
+    for (int p = 0; p < w; p++) {
+        const T xp = pick(x, p);
         if (s_iszero(xp)) continue;
         const T s = s_neg(xp);
 #pragma unroll
-        for (int i = p + 1; i < W; i++)
-            if (i < w) x[i] = s_add(x[i], s_mul(s, L[i + (size_t)p * ldl]));
+        for (int i = 1; i < W; i++)
+            if (i > p && i < w) x[i] = s_add(x[i], s_mul(s, Ls[p][i]));
     }
 #pragma unroll
     for (int i = 0; i < W; i++)
@@ -399,41 +455,41 @@ template <typename T, int W, bool LOWER, int TR, bool UNIT>
 __global__ void __launch_bounds__(64)
 trsm_diag_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
 {
+    __shared__ T Ds[W][W];  // Ds[k][i] = op(A)(i, k): column k of op(A) contiguous
+    for (int e = threadIdx.x; e < W * W; e += 64) {
+        const int i = e % W, k = e / W;
+        T v = s_zero(T{});
+        if (i < w && k < w) {
+            v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+            if (TR == 2) v = s_conj(v);
+        }
+        Ds[k][i] = v;
+    }
+    __syncthreads();
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     T* col = B + (size_t)c * ldb;
     T x[W];
 #pragma unroll
     for (int i = 0; i < W; i++) x[i] = i < w ? col[i] : s_zero(T{});
-    auto at = [&](int i, int k) -> T {  // element (i, k) of op(A)
-        T v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
-        return TR == 2 ? s_conj(v) : v;
-    };
     // op(A) lower <=> forward substitution.
     constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
-    if (FORWARD) {
-#pragma unroll
-        for (int p = 0; p < W; p++) {
-            if (p >= w) break;
-            if (!UNIT) x[p] = s_div(x[p], at(p, p));
-            const T xp = x[p];
-            if (s_iszero(xp)) continue;
-            const T s = s_neg(xp);
-#pragma unroll
-            for (int i = p + 1; i < W; i++)
-                if (i < w) x[i] = s_add(x[i], s_mul(s, at(i, p)));
-        }
-    } else {
-#pragma unroll
-        for (int p = W - 1; p >= 0; p--) {
-            if (p >= w) continue;
-            if (!UNIT) x[p] = s_div(x[p], at(p, p));
-            const T xp = x[p];
-            if (s_iszero(xp)) continue;
-            const T s = s_neg(xp);
+#pragma unroll 1
+    for (int q = 0; q < w; q++) {
+        const int p = FORWARD ? q : w - 1 - q;
+        T xp = pick(x, p);
+        if (!UNIT) {
+            xp = s_div(xp, Ds[p][p]);
 #pragma unroll
             for (int i = 0; i < W; i++)
-                if (i < p) x[i] = s_add(x[i], s_mul(s, at(i, p)));
+                if (i == p) x[i] = xp;
+        }
+        if (s_iszero(xp)) continue;
+        const T s = s_neg(xp);
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            const bool below = FORWARD ? (i > p && i < w) : (i < p);
+            if (below) x[i] = s_add(x[i], s_mul(s, Ds[p][i]));
         }
     }
 #pragma unroll
@@ -443,16 +499,18 @@ trsm_diag_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restri
 
 // Whole triangular solve for few right-hand sides: one workgroup of 256
 // threads per column of B. op(A) lower runs forward, upper backward, in
-// 64-row blocks; each block's diagonal part is solved by wavefront 0 with
-// shuffles, and the coupling to the other blocks is done by all threads:
-// right-looking (axpy order, the reference's) for op = none, left-looking
-// dot products for the transposed forms.
+// BS-row blocks. Each block's diagonal part is staged in LDS by all threads
+// and solved by wavefront 0 with shuffles; the coupling to the other blocks
+// is done by all threads, right-looking (in the reference's axpy order) for
+// op = none, left-looking dot products for the transposed forms.
 template <typename T, bool LOWER, int TR, bool UNIT>
 __global__ void __launch_bounds__(256)
 trsv_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
 {
-    __shared__ T xb[64];
-    __shared__ T part[4][64];
+    constexpr int BS = tri_block<T>();
+    __shared__ T D[BS][BS + 1];  // D[i][k] = op(A)(k0 + i, k0 + k)
+    __shared__ T xb[BS];
+    __shared__ T part[BS];
     T* x = B + (size_t)blockIdx.x * ldb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto at = [&](int i, int k) -> T {
@@ -460,41 +518,51 @@ trsv_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
         return TR == 2 ? s_conj(v) : v;
     };
     constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
-    const int nblk = (n + 63) / 64;
+    const int nblk = (n + BS - 1) / BS;
     for (int bq = 0; bq < nblk; bq++) {
         const int blk = FORWARD ? bq : nblk - 1 - bq;
-        const int k0 = blk * 64;
-        const int kb = n - k0 < 64 ? n - k0 : 64;
+        const int k0 = blk * BS;
+        const int kb = n - k0 < BS ? n - k0 : BS;
 
+        for (int e = tid; e < BS * BS; e += 256) {
+            const int i = e % BS, k = e / BS;
+            D[i][k] = (i < kb && k < kb) ? at(k0 + i, k0 + k) : s_zero(T{});
+        }
         if (TR != 0) {
             // Left-looking: subtract the already-solved part, a dot product
             // down column (k0 + i) of A per block row i.
             const int lo = FORWARD ? 0 : k0 + kb;
             const int hi = FORWARD ? k0 : n;
             for (int i = wave; i < kb; i += 4) {
-                T s = s_zero(T{});
-                for (int k = lo + lane; k < hi; k += 64) s = s_add(s, s_mul(at(k0 + i, k), x[k]));
+                T s0 = s_zero(T{}), s1 = s_zero(T{});
+                int k = lo + lane;
+                for (; k + 64 < hi; k += 128) {
+                    s0 = s_add(s0, s_mul(at(k0 + i, k), x[k]));
+                    s1 = s_add(s1, s_mul(at(k0 + i, k + 64), x[k + 64]));
+                }
+                if (k < hi) s0 = s_add(s0, s_mul(at(k0 + i, k), x[k]));
+                T sum = s_add(s0, s1);
 #pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) s = s_add(s, shfl_xor_t(s, off));
-                if (lane == 0) part[0][i] = s;
+                for (int off = 32; off >= 1; off >>= 1) sum = s_add(sum, shfl_xor_t(sum, off));
+                if (lane == 0) part[i] = sum;
             }
-            __syncthreads();
         }
+        __syncthreads();
 
-        if (wave == 0) {
+        if (wave == 0 && lane < BS) {
             T xi = lane < kb ? x[k0 + lane] : s_zero(T{});
-            if (TR != 0 && lane < kb) xi = s_sub(xi, part[0][lane]);
+            if (TR != 0 && lane < kb) xi = s_sub(xi, part[lane]);
             if (FORWARD) {
                 for (int p = 0; p < kb; p++) {
-                    if (!UNIT && lane == p) xi = s_div(xi, at(k0 + p, k0 + p));
+                    if (!UNIT && lane == p) xi = s_div(xi, D[p][p]);
                     const T xp = shfl_t(xi, p);
-                    if (lane > p && lane < kb && !s_iszero(xp)) xi = s_add(xi, s_mul(s_neg(xp), at(k0 + lane, k0 + p)));
+                    if (lane > p && lane < kb && !s_iszero(xp)) xi = s_add(xi, s_mul(s_neg(xp), D[lane][p]));
                 }
             } else {
                 for (int p = kb - 1; p >= 0; p--) {
-                    if (!UNIT && lane == p) xi = s_div(xi, at(k0 + p, k0 + p));
+                    if (!UNIT && lane == p) xi = s_div(xi, D[p][p]);
                     const T xp = shfl_t(xi, p);
-                    if (lane < p && !s_iszero(xp)) xi = s_add(xi, s_mul(s_neg(xp), at(k0 + lane, k0 + p)));
+                    if (lane < p && !s_iszero(xp)) xi = s_add(xi, s_mul(s_neg(xp), D[lane][p]));
                 }
             }
             if (lane < kb) {
@@ -505,21 +573,26 @@ trsv_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
         __syncthreads();
 
         if (TR == 0) {
-            // Right-looking: push this block into the rows still to come.
-            if (FORWARD) {
-                for (int i = k0 + kb + tid; i < n; i += 256) {
-                    T acc = x[i];
-                    for (int p = 0; p < kb; p++)
-                        if (!s_iszero(xb[p])) acc = s_add(acc, s_mul(s_neg(xb[p]), at(i, k0 + p)));
-                    x[i] = acc;
+            // Right-looking: push this block into the rows still to come,
+            // eight independent loads in flight per step.
+            const int r0 = FORWARD ? k0 + kb : 0;
+            const int r1 = FORWARD ? n : k0;
+            for (int i = r0 + tid; i < r1; i += 256) {
+                T acc = x[i];
+                for (int q = 0; q < kb; q += 8) {
+                    T av[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int p = FORWARD ? q + u : kb - 1 - (q + u);
+                        av[u] = (q + u < kb) ? at(i, k0 + p) : s_zero(T{});
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int p = FORWARD ? q + u : kb - 1 - (q + u);
+                        if (q + u < kb && !s_iszero(xb[p])) acc = s_add(acc, s_mul(s_neg(xb[p]), av[u]));
+                    }
                 }
-            } else {
-                for (int i = tid; i < k0; i += 256) {
-                    T acc = x[i];
-                    for (int p = kb - 1; p >= 0; p--)
-                        if (!s_iszero(xb[p])) acc = s_add(acc, s_mul(s_neg(xb[p]), at(i, k0 + p)));
-                    x[i] = acc;
-                }
+                x[i] = acc;
             }
             __syncthreads();
         }
@@ -596,14 +669,14 @@ struct Ops<double> {
     static constexpr int RMAX = 16;  // register panel: R rows x 64/R columns
 };
 template <>
-struct Ops<double2> {
-    static void gemm(Context* c, int ta, int m, int n, int k, double2 alpha, const double2* a, int lda,
-                     const double2* b, int ldb, double2 beta, double2* cc, int ldc)
+struct Ops<zd> {
+    static void gemm(Context* c, int ta, int m, int n, int k, zd alpha, const zd* a, int lda,
+                     const zd* b, int ldb, zd beta, zd* cc, int ldc)
     {
         zgemm_ex(c, ta, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
     }
-    static double2 one() { return make_double2(1.0, 0.0); }
-    static double2 neg_one() { return make_double2(-1.0, 0.0); }
+    static zd one() { return make_zd(1.0, 0.0); }
+    static zd neg_one() { return make_zd(-1.0, 0.0); }
     static constexpr int RMAX = 8;
 };
 
@@ -619,22 +692,23 @@ void launch_getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int o
 }
 
 // Register-resident panel shapes: R rows per thread (512 threads), NB
-// columns. Real: 64 doubles per thread at R=1, 96 above; complex half that.
+// columns, 64 doubles' worth per thread (128 VGPRs), which leaves the
+// compiler room for the rest without spilling.
 template <typename T>
 int reg_panel_width(int m)
 {
     if constexpr (sizeof(T) == 8) {
         if (m <= 512) return 64;
-        if (m <= 1024) return 48;
-        if (m <= 2048) return 24;
-        if (m <= 4096) return 12;
-        if (m <= 8192) return 6;
+        if (m <= 1024) return 32;
+        if (m <= 2048) return 16;
+        if (m <= 4096) return 8;
+        if (m <= 8192) return 4;
         return 0;
     } else {
-        if (m <= 512) return 48;
-        if (m <= 1024) return 24;
-        if (m <= 2048) return 12;
-        if (m <= 4096) return 6;
+        if (m <= 512) return 32;
+        if (m <= 1024) return 16;
+        if (m <= 2048) return 8;
+        if (m <= 4096) return 4;
         return 0;
     }
 }
@@ -644,16 +718,24 @@ void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int
 {
     if constexpr (sizeof(T) == 8) {
         if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 1024) launch_getf2_reg<T, 48, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 2048) launch_getf2_reg<T, 24, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 4096) launch_getf2_reg<T, 12, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else launch_getf2_reg<T, 6, 16>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 1024) launch_getf2_reg<T, 32, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 2048) launch_getf2_reg<T, 16, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 4096) launch_getf2_reg<T, 8, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else launch_getf2_reg<T, 4, 16>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
     } else {
-        if (m <= 512) launch_getf2_reg<T, 48, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 1024) launch_getf2_reg<T, 24, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 2048) launch_getf2_reg<T, 12, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else launch_getf2_reg<T, 6, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        if (m <= 512) launch_getf2_reg<T, 32, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 1024) launch_getf2_reg<T, 16, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else if (m <= 2048) launch_getf2_reg<T, 8, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        else launch_getf2_reg<T, 4, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
     }
+}
+
+template <typename T, int QM>
+void launch_apply_moves(Context* ctx, T* a, int lda, int c0, int c1, int ncols, const int* list, int count)
+{
+    constexpr int CPB = 256 / QM;
+    hipLaunchKernelGGL((apply_moves_kernel<T, QM>), dim3((count + CPB - 1) / CPB), dim3(256), 0, ctx->stream, a, lda,
+                       c0, c1, ncols, list);
 }
 
 template <typename T>
@@ -661,21 +743,14 @@ void apply_moves(Context* ctx, T* a, int lda, int c0, int c1, int ncols, const i
 {
     int count = c0 + (ncols - c1);
     if (count <= 0) return;
-    int blocks = (count + 63) / 64;
-    const int reg_cap = sizeof(T) == 8 ? 128 : 64;
-    if (max_moves <= 32)
-        hipLaunchKernelGGL((apply_moves_kernel<T, 32>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
-                           list);
-    else if (max_moves <= 64)
-        hipLaunchKernelGGL((apply_moves_kernel<T, 64>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
-                           list);
-    else if (max_moves <= reg_cap)
-        hipLaunchKernelGGL((apply_moves_kernel<T, (sizeof(T) == 8 ? 128 : 64)>), dim3(blocks), dim3(64), 0,
-                           ctx->stream, a, lda, c0, c1, ncols, list);
+    if (max_moves <= 32) launch_apply_moves<T, 32>(ctx, a, lda, c0, c1, ncols, list, count);
+    else if (max_moves <= 64) launch_apply_moves<T, 64>(ctx, a, lda, c0, c1, ncols, list, count);
+    else if (max_moves <= 128) launch_apply_moves<T, 128>(ctx, a, lda, c0, c1, ncols, list, count);
+    else if (max_moves <= 256) launch_apply_moves<T, 256>(ctx, a, lda, c0, c1, ncols, list, count);
     else {
         DevBuf<T> scratch(ctx, (size_t)count * max_moves);
-        hipLaunchKernelGGL((apply_moves_slow_kernel<T>), dim3(blocks), dim3(64), 0, ctx->stream, a, lda, c0, c1, ncols,
-                           list, scratch.get(), max_moves);
+        hipLaunchKernelGGL((apply_moves_slow_kernel<T>), dim3((count + 63) / 64), dim3(64), 0, ctx->stream, a, lda, c0,
+                           c1, ncols, list, scratch.get(), max_moves);
     }
 }
 
@@ -697,8 +772,8 @@ template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ldb)
 {
     int blocks = (ncols + 63) / 64;
-    hipLaunchKernelGGL((trsm_diag_kernel<T, 64, LOWER, TR, UNIT>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, A,
-                       lda, B, ldb);
+    hipLaunchKernelGGL((trsm_diag_kernel<T, tri_block<T>(), LOWER, TR, UNIT>), dim3(blocks), dim3(64), 0, ctx->stream,
+                       w, ncols, A, lda, B, ldb);
 }
 
 // op(A) X = B in place, A n x n triangular.
@@ -711,12 +786,13 @@ void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb
         return;
     }
     constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
-    const int nblk = (n + 63) / 64;
+    constexpr int BS = tri_block<T>();
+    const int nblk = (n + BS - 1) / BS;
     const int ta = TR;
     for (int bq = 0; bq < nblk; bq++) {
         const int blk = FORWARD ? bq : nblk - 1 - bq;
-        const int k0 = blk * 64;
-        const int kb = n - k0 < 64 ? n - k0 : 64;
+        const int k0 = blk * BS;
+        const int kb = n - k0 < BS ? n - k0 : BS;
         if (TR != 0) {
             // Left-looking: B_k -= op(A)[k, solved] X_solved.
             if (FORWARD && k0 > 0)
@@ -750,10 +826,15 @@ void trsm_llnu(Context* ctx, int w, int ncols, const T* L, int ldl, T* B, int ld
         hipLaunchKernelGGL((trsm_llnu_kernel<T, 16>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
     else if (w <= 32)
         hipLaunchKernelGGL((trsm_llnu_kernel<T, 32>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
-    else if (w <= 64)
-        hipLaunchKernelGGL((trsm_llnu_kernel<T, 64>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
-    else
+    else if constexpr (sizeof(T) == 8) {
+        if (w <= 64)
+            hipLaunchKernelGGL((trsm_llnu_kernel<T, 64>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B,
+                               ldb);
+        else
+            trsm_impl<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
+    } else {
         trsm_impl<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
+    }
 }
 
 template <typename T>
@@ -877,7 +958,7 @@ void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, 
 
 void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
 {
-    getrf_impl<double2>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+    getrf_impl<zd>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
 }
 
 void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb)
@@ -887,7 +968,7 @@ void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu
 
 void zgetrs(Context* ctx, int trans, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm, cplx* b, int ldb)
 {
-    getrs_impl<double2>(ctx, trans, n, nrhs, lu, ldlu, d_perm, b, ldb);
+    getrs_impl<zd>(ctx, trans, n, nrhs, lu, ldlu, d_perm, b, ldb);
 }
 
 void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb)
@@ -911,7 +992,7 @@ void dpivot_range(Context* ctx, int n, const double* a, int lda, double* d_out)
 
 void zpivot_range(Context* ctx, int n, const cplx* a, int lda, double* d_out)
 {
-    hipLaunchKernelGGL((pivot_range_kernel<double2>), dim3(1), dim3(256), 0, ctx->stream, n, a, lda, d_out);
+    hipLaunchKernelGGL((pivot_range_kernel<zd>), dim3(1), dim3(256), 0, ctx->stream, n, a, lda, d_out);
 }
 
 void dcopy2d(Context* ctx, int m, int n, const double* src, int lds, double* dst, int ldd)
@@ -921,7 +1002,7 @@ void dcopy2d(Context* ctx, int m, int n, const double* src, int lds, double* dst
 
 void zcopy2d(Context* ctx, int m, int n, const cplx* src, int lds, cplx* dst, int ldd)
 {
-    copy2d<double2>(ctx, m, n, src, lds, dst, ldd);
+    copy2d<zd>(ctx, m, n, src, lds, dst, ldd);
 }
 
 }  // namespace tns

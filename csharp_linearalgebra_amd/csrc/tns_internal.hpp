@@ -14,10 +14,11 @@
 #include <vector>
 
 #include "tensile_hip.h"
+#include "tns_scalar.hpp"
 
 namespace tns {
 
-using cplx = double2;  // (re, im), the layout of System.Numerics.Complex
+using cplx = zd;  // (re, im), the layout of System.Numerics.Complex
 
 // A thrown status: the C ABI catches it at the boundary and returns the code,
 // so no C++ exception ever crosses extern "C".

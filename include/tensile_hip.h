@@ -106,6 +106,11 @@ TNS_API int tns_context_stream(tns_context* ctx, void** stream);
 TNS_API int tns_synchronize(tns_context* ctx);
 /* Library build tag. */
 TNS_API const char* tns_version(void);
+/* Tile configuration of the GEMM kernels, for the library's autotuner:
+ * -1 restores the size heuristic; otherwise an index below
+ * tns_gemm_config_count(complex). Process-wide. */
+TNS_API int tns_gemm_set_config(int real_config, int complex_config);
+TNS_API int tns_gemm_config_count(int complex_);
 
 /* ---- GEMM: KernelEntry.Multiply (KernelEntry.cs:34-56) ------------------
  * C := beta*C + alpha*A*B; A m x k, B k x n, C m x n. beta == 0 overwrites

@@ -147,3 +147,58 @@ def test_large_square_gemm(ws, rng):
     c = ws.multiply(a, b)
     ref = O.gemm(a, b)
     assert np.all(np.abs(c - ref) <= 2 * bound(a, b, 0 * ref, 1, 0, n))
+
+
+CONFIG_SHAPES = [(17, 9, 5), (65, 63, 66), (256, 128, 64), (300, 257, 513), (512, 256, 48)]
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_every_tile_config_matches_oracle(ws, rng, complex_):
+    # Each tile shape the heuristic can pick, forced in turn, on ragged and
+    # on tile-aligned shapes (the latter take the paired 16-byte loads).
+    import csharp_linearalgebra_amd as T
+
+    lib = T.load()
+    try:
+        for cfg in range(lib.tns_gemm_config_count(int(complex_))):
+            lib.tns_gemm_set_config(-1 if complex_ else cfg, cfg if complex_ else -1)
+            for m, n, k in CONFIG_SHAPES:
+                a = rng.uniform(-0.5, 0.5, (m, k))
+                b = rng.uniform(-0.5, 0.5, (k, n))
+                if complex_:
+                    a = a + 1j * rng.uniform(-0.5, 0.5, (m, k))
+                    b = b + 1j * rng.uniform(-0.5, 0.5, (k, n))
+                c = ws.multiply(a, b)
+                ref = O.gemm(a, b)
+                assert np.all(np.abs(c - ref) <= 4 * bound(a, b, 0 * ref, 1, 0, k)), (cfg, m, n, k)
+    finally:
+        lib.tns_gemm_set_config(-1, -1)
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+def test_device_operands_at_odd_offsets(ws, rng, offset):
+    # Device operands that start 8 bytes past a 16-byte boundary, or have an
+    # odd leading dimension, must take the element-wise loads.
+    import torch
+
+    import csharp_linearalgebra_amd as T
+
+    lib = T.load()
+    m, n, k = 256, 128, 64
+    for ld_extra in (0, 1):
+        lda, ldb, ldc = m + ld_extra, k + ld_extra, m
+        a = rng.uniform(-0.5, 0.5, (m, k))
+        b = rng.uniform(-0.5, 0.5, (k, n))
+        abuf = torch.zeros(offset + lda * k, dtype=torch.float64, device="cuda")
+        bbuf = torch.zeros(offset + ldb * n, dtype=torch.float64, device="cuda")
+        cbuf = torch.zeros(ldc * n, dtype=torch.float64, device="cuda")
+        abuf[offset:].view(k, lda)[:, :m] = torch.from_numpy(a.T.copy()).cuda()
+        bbuf[offset:].view(n, ldb)[:, :k] = torch.from_numpy(b.T.copy()).cuda()
+        torch.cuda.synchronize()
+        rc = lib.tns_dgemm(ws.handle, m, n, k, 1.0, abuf.data_ptr() + 8 * offset, lda,
+                           bbuf.data_ptr() + 8 * offset, ldb, 0.0, cbuf.data_ptr(), ldc)
+        assert rc == 0
+        ws.synchronize()
+        c = cbuf.view(n, ldc).cpu().numpy().T
+        ref = O.gemm(a, b)
+        assert np.all(np.abs(c - ref) <= 2 * bound(a, b, 0 * ref, 1, 0, k)), (offset, ld_extra)
