@@ -19,15 +19,16 @@
 //    chunk, move its interchanges, solve and update the rest of the panel).
 //  * Row interchanges are carried with the rows as payloads: the row's
 //    original index (the permutation P, for the solves) and its index at the
-//    start of the block step (and of the chunk). After a block step the rows
-//    whose payload moved are compacted into a (destination, source) list, and
-//    one kernel applies that list to every column outside the panel: a
-//    gather of at most 2*nb rows per column instead of nb dependent swaps.
+//    start of the panel kernel. At its end the kernel lists the rows whose
+//    payload moved as (destination, source) pairs, and one kernel applies
+//    that list to every matrix column outside the kernel's columns: a gather
+//    of at most 2w rows per column instead of w dependent swaps.
 //  * Triangular solves with few right-hand sides run one workgroup per
 //    column: 64-row diagonal blocks solved by one wavefront with shuffles,
 //    the rest updated by the whole workgroup. Many right-hand sides (the
-//    Pade denominator's n) use blocked substitution: the diagonal block per
-//    column in registers, the off-diagonal update as a GEMM.
+//    Pade denominator's n) use blocked substitution: each diagonal block
+//    solved by workgroups of 32 columns with the block in LDS, the
+//    off-diagonal update as a GEMM.
 #include <climits>
 
 #include "tns_internal.hpp"
@@ -54,6 +55,49 @@ __device__ __forceinline__ void wave_best(double& bv, int& bi)
         int oi = __shfl_xor(bi, off, 64);
         best_combine(bv, bi, ov, oi);
     }
+}
+
+// One step of the argmax all-reduce across lanes paired by a DPP pattern
+// (register-to-register, no LDS round trip).
+template <int CTRL>
+__device__ __forceinline__ void dpp_best(double& bv, int& bi)
+{
+    const long long b = __double_as_longlong(bv);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(0, bi, CTRL, 0xF, 0xF, false);
+    best_combine(bv, bi, __longlong_as_double(((long long)hi << 32) | (unsigned)lo), oi);
+}
+
+// wave_best without LDS: the in-row all-reduce as DPP moves (quad_perm
+// [1,0,3,2] and [2,3,0,1], row_ror 4 and 8), then row_bcast 15 and 31 fold
+// rows 0..3 into lane 63, which every lane reads back. The combine (largest
+// key, lowest row on ties) is associative and commutative, so the pairing
+// order does not change the result.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_best_rows(double& bv, int& bi)
+{
+    const long long b = __double_as_longlong(bv);
+    const long long idv = __double_as_longlong(-2.0);
+    const int lo = __builtin_amdgcn_update_dpp((int)idv, (int)b, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(idv >> 32), (int)(b >> 32), CTRL, ROWS, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(INT_MAX, bi, CTRL, ROWS, 0xF, false);
+    best_combine(bv, bi, __longlong_as_double(((long long)hi << 32) | (unsigned)lo), oi);
+}
+
+__device__ __forceinline__ void wave_best_dpp(double& bv, int& bi)
+{
+    dpp_best<0xB1>(bv, bi);
+    dpp_best<0x4E>(bv, bi);
+    dpp_best<0x124>(bv, bi);
+    dpp_best<0x128>(bv, bi);
+    dpp_best_rows<0x142, 0xA>(bv, bi);  // row_bcast:15 into rows 1 and 3
+    dpp_best_rows<0x143, 0xC>(bv, bi);  // row_bcast:31 into rows 2 and 3
+    const long long b = __double_as_longlong(bv);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    bi = __builtin_amdgcn_readlane(bi, 63);
+    bv = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Pivot-search key of row `row` in column j: the pivot measure, with a NaN
@@ -131,41 +175,59 @@ __device__ __forceinline__ zd blend(zd x, zd y, bool take) { return zd{blend(x.x
 
 // ---------------------------------------------------------------------------
 // Panel factorization, register resident. m <= blockDim.x * R, w <= NB.
-// piv[j] receives the global pivot row (off + local row). perm, rs and rc
-// are payloads that travel with the rows (indexed by local row, offset by
-// the caller): the original row index, the row's index at the start of the
-// block step, and (if rc is not null) at the start of this chunk. info[0] is
-// min-reduced with the first zero-pivot column. The column loop is not
-// unrolled; every register access uses a compile-time index, with the
-// current column chosen by predicates.
+// piv[j] receives the global pivot row (off + local row); perm (indexed by
+// local row, offset by the caller) is the payload of original row indices
+// that travels with the rows. At the end the kernel writes the move list
+// (destination, source) of the rows that changed place to `list`, and
+// info[0] is min-reduced with the first zero-pivot column.
+//
+// Thread t holds rows t, t + nt, ... (its slots) across every column. Row
+// interchanges do not move data: each slot carries the row position it has
+// in the current order (pos), and step j marks the pivot's slot done (it is
+// row j of U from then on) and gives the slot that was at position j the
+// pivot's old position, which is the reference's swap of rows j and p. The
+// slots are stored to their final positions at the end.
+//
+// The column loop is not unrolled, yet every register access uses a
+// compile-time index: the loop over a row's columns is unrolled and the
+// runtime column j is met by uniform conditions. The next pivot column is
+// copied aside (nx) as it is updated, so the pivot search and the
+// multiplier need no runtime-indexed read. Updates with a zero entry of the
+// pivot row are skipped, as the reference's loop does (one ballot gives the
+// nonzero pattern); done slots are masked off.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int R>
-__global__ void __launch_bounds__(512)
+template <typename T, int NB, int R, int NT>
+__global__ void __launch_bounds__(NT)
 getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
-                 int* __restrict__ rs, int* __restrict__ rc, int* __restrict__ info)
+                 int* __restrict__ list, int cap, int* __restrict__ info)
 {
-    __shared__ double red_v[2][8];
-    __shared__ int red_i[2][8];
-    __shared__ T prow[2][NB];
-    __shared__ T jrow[2][NB];
-    __shared__ int pay[2][6];
+    static_assert(NB <= 64 && NB % 4 == 0, "the nonzero mask is one ballot; columns go in groups of 4");
+    constexpr int GW = NB < 8 ? NB : 8;  // columns per group of pivot-row reads
+    __shared__ double red_v[2][NT / 64];
+    __shared__ int red_i[2][NT / 64];
+    __shared__ T cand[2][NT / 64][NB];  // each wave's best row, before the vote
+    __shared__ int spiv[NB];
+    __shared__ int moves;
 
     const int nt = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-    const T zero = s_zero(T{});
-    const bool chunk = rc != nullptr;
+    int first_zero = -1;
 
     T v[R][NB];
-    int p0[R], p1[R], p2[R];
+    T nx[R];
+    int p0[R], pos[R];
+    unsigned done = 0;  // bit r: slot r is a pivot row already, or padding
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int row = tid + r * nt;
         const bool ok = row < m;
 #pragma unroll
-        for (int c = 0; c < NB; c++) v[r][c] = (ok && c < w) ? a[row + (size_t)c * lda] : zero;
+        for (int c = 0; c < NB; c++) v[r][c] = load_if(a + row + (size_t)c * lda, ok && c < w);
+        nx[r] = v[r][0];
         p0[r] = ok ? perm[row] : 0;
-        p1[r] = ok ? rs[row] : 0;
-        p2[r] = (ok && chunk) ? rc[row] : 0;
+        pos[r] = row;
+        if (!ok) done |= 1u << r;
     }
+    if (tid == 0) moves = 0;
 
     const int jl = m < w ? m : w;
 #pragma unroll 1
@@ -175,83 +237,130 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
         int bi = INT_MAX;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int row = tid + r * nt;
-            const T x = pick(v[r], j);
-            if (row >= j && row < m) best_combine(bv, bi, pivot_key(x, row, j), row);
+            const bool ok = !((done >> r) & 1u);
+            best_combine(bv, bi, ok ? pivot_key(nx[r], pos[r], j) : -2.0, ok ? pos[r] : INT_MAX);
         }
-        wave_best(bv, bi);
+        wave_best_dpp(bv, bi);
+        // Every wave publishes its best row with its vote, so one barrier
+        // serves the whole step: the winning wave's row is the pivot row.
         if (lane == 0) {
             red_v[buf][wave] = bv;
             red_i[buf][wave] = bi;
         }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const bool live = !((done >> r) & 1u);
+            if (live && pos[r] == bi) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) cand[buf][wave][c] = v[r][c];
+            }
+        }
         __syncthreads();
         bv = red_v[buf][0];
         bi = red_i[buf][0];
-        for (int q = 1; q < nw; ++q) best_combine(bv, bi, red_v[buf][q], red_i[buf][q]);
+        int wq = 0;
+        for (int q = 1; q < nw; ++q) {
+            const double qv = red_v[buf][q];
+            const int qi = red_i[buf][q];
+            if (qv > bv || (qv == bv && qi < bi)) {
+                bv = qv;
+                bi = qi;
+                wq = q;
+            }
+        }
         const int p = bi;
+        const T* prow = cand[buf][wq];
 
-        // Publish the pivot row and the diagonal row with their payloads.
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int row = tid + r * nt;
-            if (row == p) {
-#pragma unroll
-                for (int c = 0; c < NB; c++) prow[buf][c] = v[r][c];
-                pay[buf][0] = p0[r];
-                pay[buf][1] = p1[r];
-                pay[buf][2] = p2[r];
-            }
-            if (row == j) {
-#pragma unroll
-                for (int c = 0; c < NB; c++) jrow[buf][c] = v[r][c];
-                pay[buf][3] = p0[r];
-                pay[buf][4] = p1[r];
-                pay[buf][5] = p2[r];
-            }
-        }
-        __syncthreads();
-
-        const T pv = prow[buf][j];
+        const T pv = prow[j];
         const bool zero_pivot = s_iszero(pv);
-        if (tid == 0) {
-            piv[j] = off + p;
-            if (zero_pivot) atomicMin(info, off + j);
-        }
-        const bool swap = !zero_pivot && p != j;
-        const double mag = s_abs(pv);
-        const T recip = s_div(s_one(T{}), pv);
-
+        // Pivots and the first zero pivot stay on chip until the end: a
+        // global store here would make the next barrier wait for it.
+        if (tid == 0) spiv[j] = off + p;
+        if (zero_pivot && first_zero < 0) first_zero = off + j;
+        // The pivot's slot retires as row j; the slot at position j takes
+        // the pivot's place. A zero pivot leaves the rows where they are.
+        const int pe = zero_pivot ? j : p;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int row = tid + r * nt;
-            if (swap && row == j) {
-#pragma unroll
-                for (int c = 0; c < NB; c++) v[r][c] = prow[buf][c];
-                p0[r] = pay[buf][0];
-                p1[r] = pay[buf][1];
-                p2[r] = pay[buf][2];
-            } else if (swap && row == p) {
-#pragma unroll
-                for (int c = 0; c < NB; c++) v[r][c] = jrow[buf][c];
-                p0[r] = pay[buf][3];
-                p1[r] = pay[buf][4];
-                p2[r] = pay[buf][5];
+            const bool live = !((done >> r) & 1u);
+            if (live && pos[r] == pe) {
+                done |= 1u << r;
+                pos[r] = j;
+            } else if (live && pos[r] == j) {
+                pos[r] = pe;
             }
-            if (row > j && row < m) {
-                T l = pick(v[r], j);
-                if (!zero_pivot) l = (mag >= kSafeMin) ? s_mul(l, recip) : s_div(l, pv);
-                // j, w and the pivot row are uniform: these are scalar
-                // branches, not per-lane selects.
+        }
+        // Multipliers, in nx: x * (1/pivot) when |pivot| >= the safe
+        // minimum, x / pivot below it, x itself for a zero pivot. |pivot|
+        // >= max(|Re|, |Im|), so the modulus is needed only when both parts
+        // are below the safe minimum.
+        if (!zero_pivot) {
+            bool safe = s_pivmax(pv) >= kSafeMin;
+            if (!safe) safe = s_abs(pv) >= kSafeMin;
+            if (safe) {
+                const T recip = s_div(s_one(T{}), pv);
 #pragma unroll
-                for (int c = 0; c < NB; c++) {
-                    if (c == j) {
-                        v[r][c] = l;
-                    } else if (c > j && c < w) {
-                        const T u = prow[buf][c];
-                        if (!s_iszero(u)) v[r][c] = s_add(v[r][c], s_mul(s_neg(u), l));
+                for (int r = 0; r < R; r++) nx[r] = s_mul(nx[r], recip);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) nx[r] = s_div(nx[r], pv);
+            }
+        }
+        // Columns to update: right of j, inside w, nonzero in the pivot row
+        // (one ballot: lane c looks at column c).
+        const bool nz = lane < NB && lane < w && lane > j && !s_iszero(prow[lane < NB ? lane : 0]);
+        const unsigned long long mask = __ballot(nz);
+
+        // Groups of GW columns: the group's pivot-row entries are read from
+        // LDS together, then every live slot of the thread is updated;
+        // groups left of j are skipped by a uniform branch. A group whose
+        // columns are all to be updated (right of j, nonzero pivot-row
+        // entries: the usual case) takes plain arithmetic; otherwise each
+        // column is chosen by a uniform select. The multiplier (nx) is
+        // column j's new value; column j receives it, and nx moves on to
+        // column j + 1, after every group.
+#pragma unroll
+        for (int g = 0; g < NB; g += GW) {
+            if (g + GW <= j) continue;
+            T u[GW];
+#pragma unroll
+            for (int q = 0; q < GW; q++) u[q] = prow[g + q];
+            const unsigned long long gbits = (mask >> g) & ((1ull << GW) - 1);
+            const bool full = gbits == (1ull << GW) - 1;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if ((done >> r) & 1u) continue;
+                const T l = nx[r];
+                if (full) {
+#pragma unroll
+                    for (int q = 0; q < GW; q++) v[r][g + q] = s_add(v[r][g + q], s_mul(s_neg(u[q]), l));
+                } else {
+#pragma unroll
+                    for (int q = 0; q < GW; q++) {
+                        const T x = v[r][g + q];
+                        v[r][g + q] = s_sel(((gbits >> q) & 1ull) != 0, s_add(x, s_mul(s_neg(u[q]), l)), x);
                     }
                 }
             }
+        }
+        // Column j takes the multipliers; nx moves on to column j + 1.
+#pragma unroll
+        for (int g = 0; g < NB; g += GW) {
+            if (j < g || j >= g + GW) continue;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if ((done >> r) & 1u) continue;
+#pragma unroll
+                for (int q = 0; q < GW; q++) v[r][g + q] = s_sel(g + q == j, nx[r], v[r][g + q]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < NB; g += GW) {
+            if (j + 1 < g || j + 1 >= g + GW) continue;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int q = 0; q < GW; q++) nx[r] = s_sel(g + q == j + 1, v[r][g + q], nx[r]);
         }
     }
 
@@ -259,13 +368,25 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
     for (int r = 0; r < R; r++) {
         const int row = tid + r * nt;
         if (row < m) {
+            const int dst = pos[r];
 #pragma unroll
             for (int c = 0; c < NB; c++)
-                if (c < w) a[row + (size_t)c * lda] = v[r][c];
-            perm[row] = p0[r];
-            rs[row] = p1[r];
-            if (chunk) rc[row] = p2[r];
+                if (c < w) a[dst + (size_t)c * lda] = v[r][c];
+            perm[dst] = p0[r];
+            if (dst != row) {
+                const int at = atomicAdd(&moves, 1);
+                if (at < cap) {
+                    list[1 + 2 * at] = dst;
+                    list[2 + 2 * at] = row;
+                }
+            }
         }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        list[0] = moves;
+        for (int j = 0; j < jl; ++j) piv[j] = spiv[j];
+        if (first_zero >= 0) atomicMin(info, first_zero);
     }
 }
 
@@ -274,7 +395,7 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
 template <typename T>
 __global__ void __launch_bounds__(1024)
 getf2_global_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
-                    int* __restrict__ rs, int* __restrict__ rc, int* __restrict__ info)
+                    int* __restrict__ rc, int* __restrict__ info)
 {
     __shared__ double red_v[16];
     __shared__ int red_i[16];
@@ -315,8 +436,7 @@ getf2_global_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
             }
             if (tid == 0) {
                 int t = perm[j]; perm[j] = perm[p]; perm[p] = t;
-                t = rs[j]; rs[j] = rs[p]; rs[p] = t;
-                if (rc) { t = rc[j]; rc[j] = rc[p]; rc[p] = t; }
+                t = rc[j]; rc[j] = rc[p]; rc[p] = t;
             }
         }
         __syncthreads();
@@ -356,21 +476,29 @@ __global__ void finish_info_kernel(int* info)
 
 // Rows of a slice [0, count) whose payload (a global row index; the slice
 // starts at global row `base`) says they moved: append (row, source) in
-// slice-local indices and reset the payload for the next step. list[0]
-// holds the count; at most `cap` moves are stored.
-__global__ void compact_moves_kernel(int count, int base, int* __restrict__ rowid, int* __restrict__ list, int cap)
+// slice-local indices and reset the payload for the next step. One
+// workgroup: the counter starts at zero in LDS and list[0] receives it at
+// the end; at most `cap` moves are stored. The list order is irrelevant to
+// apply_moves_kernel, which gathers every source before any write.
+__global__ void __launch_bounds__(1024)
+compact_moves_kernel(int count, int base, int* __restrict__ rowid, int* __restrict__ list, int cap)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    int src = rowid[i] - base;
-    if (src != i) {
-        int at = atomicAdd(list, 1);
-        if (at < cap) {
-            list[1 + 2 * at] = i;
-            list[2 + 2 * at] = src;
+    __shared__ int moves;
+    if (threadIdx.x == 0) moves = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const int src = rowid[i] - base;
+        if (src != i) {
+            const int at = atomicAdd(&moves, 1);
+            if (at < cap) {
+                list[1 + 2 * at] = i;
+                list[2 + 2 * at] = src;
+            }
+            rowid[i] = i + base;
         }
-        rowid[i] = i + base;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) list[0] = moves;
 }
 
 // Apply a move list (rows relative to `a`) to columns [0, c0) and [c1, ncols).
@@ -414,87 +542,81 @@ __global__ void apply_moves_slow_kernel(T* __restrict__ a, int lda, int c0, int 
     for (int q = 0; q < count; q++) col[list[1 + 2 * q]] = s[q];
 }
 
-// U12 := L11^-1 A12 for the panel step, one thread per column of A12, the
-// column in registers, L11 read at uniform addresses (scalar loads).
-// The update order is the reference's column-oriented substitution
-// (Triangular.cs:45-57): x_i += (-x_p) * l_ip for p ascending.
-template <typename T, int W>
-__global__ void __launch_bounds__(64)
-trsm_llnu_kernel(int w, int ncols, const T* __restrict__ L, int ldl, T* __restrict__ B, int ldb)
+// Triangular solve op(A) X = B for one diagonal block (order w <= BS) and
+// many right-hand sides, in place; TR: 0 none, 1 transpose, 2 conjugate
+// transpose. A workgroup of 256 threads owns CB columns of B: thread (g, c)
+// holds rows g, g + G, g + 2G, ... of column c in registers (G = 256 / CB
+// row groups), op(A) sits in LDS. Step p finalizes x_p (divided by the
+// diagonal unless UNIT), passes it through LDS, and every thread subtracts
+// x_p times column p of op(A) from its rows still to come: the reference's
+// column-oriented substitution (Triangular.cs:45-57, 126-140), in the same
+// order for every element. The step loop is unrolled, so the owner's
+// register index p / G is a constant.
+template <typename T, int BS, int CB, bool LOWER, int TR, bool UNIT>
+__global__ void __launch_bounds__(256)
+trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
 {
-    __shared__ T Ls[W][W];  // Ls[p][i] = L[i + p*ldl]: column p contiguous
-    for (int e = threadIdx.x; e < W * W; e += 64) {
-        const int i = e % W, p = e / W;
-        Ls[p][i] = (i < w && p < w) ? L[i + (size_t)p * ldl] : s_zero(T{});
-    }
-    __syncthreads();
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    T* col = B + (size_t)c * ldb;
-    T x[W];
-#pragma unroll
-    for (int i = 0; i < W; i++) x[i] = i < w ? col[i] : s_zero(T{});
-#pragma unroll 1
-    for (int p = 0; p < w; p++) {
-        const T xp = pick(x, p);
-        if (s_iszero(xp)) continue;
-        const T s = s_neg(xp);
-#pragma unroll
-        for (int i = 1; i < W; i++)
-            if (i > p && i < w) x[i] = s_add(x[i], s_mul(s, Ls[p][i]));
-    }
-#pragma unroll
-    for (int i = 0; i < W; i++)
-        if (i < w) col[i] = x[i];
-}
+    constexpr int G = 256 / CB;
+    constexpr int Q = BS / G;
+    static_assert(256 % CB == 0 && BS % G == 0, "block must split over the row groups");
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    __shared__ T D[BS][BS + 1];  // D[i][k] = op(A)(i, k)
+    __shared__ T X[BS][CB + 1];
+    const int tid = threadIdx.x;
+    const int col0 = blockIdx.x * CB;
+    const T zero = s_zero(T{});
 
-// Diagonal-block solve with the block in registers, one thread per
-// right-hand side column; all eight triangle/transpose/diagonal cases.
-// TR: 0 none, 1 transpose, 2 conjugate transpose.
-template <typename T, int W, bool LOWER, int TR, bool UNIT>
-__global__ void __launch_bounds__(64)
-trsm_diag_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
-{
-    __shared__ T Ds[W][W];  // Ds[k][i] = op(A)(i, k): column k of op(A) contiguous
-    for (int e = threadIdx.x; e < W * W; e += 64) {
-        const int i = e % W, k = e / W;
-        T v = s_zero(T{});
+    for (int e = tid; e < BS * BS; e += 256) {
+        const int i = e % BS, k = e / BS;
+        T v = zero;
         if (i < w && k < w) {
             v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
             if (TR == 2) v = s_conj(v);
         }
-        Ds[k][i] = v;
+        D[i][k] = v;
+    }
+    for (int e = tid; e < BS * CB; e += 256) {
+        const int i = e % BS, c = e / BS;
+        X[i][c] = load_if(B + i + (size_t)(col0 + c) * ldb, i < w && col0 + c < ncols);
     }
     __syncthreads();
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    T* col = B + (size_t)c * ldb;
-    T x[W];
+
+    const int c = tid % CB, g = tid / CB;
+    T x[Q];
 #pragma unroll
-    for (int i = 0; i < W; i++) x[i] = i < w ? col[i] : s_zero(T{});
-    // op(A) lower <=> forward substitution.
-    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
-#pragma unroll 1
-    for (int q = 0; q < w; q++) {
-        const int p = FORWARD ? q : w - 1 - q;
-        T xp = pick(x, p);
-        if (!UNIT) {
-            xp = s_div(xp, Ds[p][p]);
+    for (int q = 0; q < Q; q++) x[q] = X[g + q * G][c];
+
 #pragma unroll
-            for (int i = 0; i < W; i++)
-                if (i == p) x[i] = xp;
-        }
-        if (s_iszero(xp)) continue;
-        const T s = s_neg(xp);
+    for (int s = 0; s < BS; s++) {
+        const int p = FORWARD ? s : BS - 1 - s;
+        if (p < w) {
+            if (g == p % G) {
+                T xp = x[p / G];
+                if (!UNIT) xp = s_div(xp, D[p][p]);
+                x[p / G] = xp;
+                X[p][c] = xp;
+            }
+            __syncthreads();
+            const T xp = X[p][c];
+            if (!s_iszero(xp)) {
+                const T nxp = s_neg(xp);
 #pragma unroll
-        for (int i = 0; i < W; i++) {
-            const bool below = FORWARD ? (i > p && i < w) : (i < p);
-            if (below) x[i] = s_add(x[i], s_mul(s, Ds[p][i]));
+                for (int q = 0; q < Q; q++) {
+                    const int i = g + q * G;
+                    const bool ahead = FORWARD ? (i > p && i < w) : (i < p);
+                    if (ahead) x[q] = s_add(x[q], s_mul(nxp, D[i][p]));
+                }
+            }
         }
     }
+    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < W; i++)
-        if (i < w) col[i] = x[i];
+    for (int q = 0; q < Q; q++) X[g + q * G][c] = x[q];
+    __syncthreads();
+    for (int e = tid; e < BS * CB; e += 256) {
+        const int i = e % BS, cc = e / BS;
+        if (i < w && col0 + cc < ncols) B[i + (size_t)(col0 + cc) * ldb] = X[i][cc];
+    }
 }
 
 // Whole triangular solve for few right-hand sides: one workgroup of 256
@@ -550,7 +672,7 @@ trsv_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
         __syncthreads();
 
         if (wave == 0 && lane < BS) {
-            T xi = lane < kb ? x[k0 + lane] : s_zero(T{});
+            T xi = load_if(x + k0 + lane, lane < kb);
             if (TR != 0 && lane < kb) xi = s_sub(xi, part[lane]);
             if (FORWARD) {
                 for (int p = 0; p < kb; p++) {
@@ -666,7 +788,6 @@ struct Ops<double> {
     }
     static double one() { return 1.0; }
     static double neg_one() { return -1.0; }
-    static constexpr int RMAX = 16;  // register panel: R rows x 64/R columns
 };
 template <>
 struct Ops<zd> {
@@ -677,18 +798,16 @@ struct Ops<zd> {
     }
     static zd one() { return make_zd(1.0, 0.0); }
     static zd neg_one() { return make_zd(-1.0, 0.0); }
-    static constexpr int RMAX = 8;
 };
 
-template <typename T, int NB, int R>
-void launch_getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rs, int* rc,
-                      int* info)
+template <typename T, int NB, int R, int NT = 512>
+void launch_getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* list, int* info)
 {
     int need = (m + R - 1) / R;
     int nt = ((need + 63) / 64) * 64;
     if (nt < 64) nt = 64;
-    hipLaunchKernelGGL((getf2_reg_kernel<T, NB, R>), dim3(1), dim3(nt), 0, ctx->stream, m, w, a, lda, piv, off, perm,
-                       rs, rc, info);
+    hipLaunchKernelGGL((getf2_reg_kernel<T, NB, R, NT>), dim3(1), dim3(nt), 0, ctx->stream, m, w, a, lda, piv, off,
+                       perm, list, 2 * w, info);
 }
 
 // Register-resident panel shapes: R rows per thread (512 threads), NB
@@ -713,20 +832,21 @@ int reg_panel_width(int m)
     }
 }
 
+// Factors m x w at a and leaves its move list, at most 2w moves, in `list`.
 template <typename T>
-void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rs, int* rc, int* info)
+void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* list, int* info)
 {
     if constexpr (sizeof(T) == 8) {
-        if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 1024) launch_getf2_reg<T, 32, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 2048) launch_getf2_reg<T, 16, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 4096) launch_getf2_reg<T, 8, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else launch_getf2_reg<T, 4, 16>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 1024) launch_getf2_reg<T, 32, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 2048) launch_getf2_reg<T, 16, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 4096) launch_getf2_reg<T, 8, 8>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else launch_getf2_reg<T, 4, 16>(ctx, m, w, a, lda, piv, off, perm, list, info);
     } else {
-        if (m <= 512) launch_getf2_reg<T, 32, 1>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 1024) launch_getf2_reg<T, 16, 2>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else if (m <= 2048) launch_getf2_reg<T, 8, 4>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
-        else launch_getf2_reg<T, 4, 8>(ctx, m, w, a, lda, piv, off, perm, rs, rc, info);
+        if (m <= 512) launch_getf2_reg<T, 32, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 1024) launch_getf2_reg<T, 16, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 2048) launch_getf2_reg<T, 8, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else launch_getf2_reg<T, 4, 8>(ctx, m, w, a, lda, piv, off, perm, list, info);
     }
 }
 
@@ -762,18 +882,17 @@ template <typename T>
 void sync_columns(Context* ctx, int m, int base, T* a, int lda, int c0, int c1, int ncols, int* rowid, int* list,
                   int max_moves)
 {
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, ctx->stream, list, 0);
-    hipLaunchKernelGGL(compact_moves_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, base, rowid, list,
-                       max_moves);
+    hipLaunchKernelGGL(compact_moves_kernel, dim3(1), dim3(1024), 0, ctx->stream, m, base, rowid, list, max_moves);
     apply_moves(ctx, a, lda, c0, c1, ncols, list, max_moves);
 }
 
 template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ldb)
 {
-    int blocks = (ncols + 63) / 64;
-    hipLaunchKernelGGL((trsm_diag_kernel<T, tri_block<T>(), LOWER, TR, UNIT>), dim3(blocks), dim3(64), 0, ctx->stream,
-                       w, ncols, A, lda, B, ldb);
+    constexpr int CB = 32;
+    const int blocks = (ncols + CB - 1) / CB;
+    hipLaunchKernelGGL((trsm_block_kernel<T, tri_block<T>(), CB, LOWER, TR, UNIT>), dim3(blocks), dim3(256), 0,
+                       ctx->stream, w, ncols, A, lda, B, ldb);
 }
 
 // op(A) X = B in place, A n x n triangular.
@@ -815,26 +934,14 @@ void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb
     }
 }
 
-// U12 := L11^-1 A12 for a block step: one thread per column for w <= 64,
-// blocked substitution beyond.
+// U12 := L11^-1 A12 for a block step: one diagonal-block solve for
+// w <= 64 real / 32 complex, blocked substitution beyond.
 template <typename T>
 void trsm_llnu(Context* ctx, int w, int ncols, const T* L, int ldl, T* B, int ldb)
 {
     if (w <= 0 || ncols <= 0) return;
-    int blocks = (ncols + 63) / 64;
-    if (w <= 16)
-        hipLaunchKernelGGL((trsm_llnu_kernel<T, 16>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
-    else if (w <= 32)
-        hipLaunchKernelGGL((trsm_llnu_kernel<T, 32>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B, ldb);
-    else if constexpr (sizeof(T) == 8) {
-        if (w <= 64)
-            hipLaunchKernelGGL((trsm_llnu_kernel<T, 64>), dim3(blocks), dim3(64), 0, ctx->stream, w, ncols, L, ldl, B,
-                               ldb);
-        else
-            trsm_impl<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
-    } else {
-        trsm_impl<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
-    }
+    if (w <= tri_block<T>()) trsm_diag<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
+    else trsm_impl<T, true, 0, true>(ctx, w, ncols, L, ldl, B, ldb);
 }
 
 template <typename T>
@@ -856,32 +963,34 @@ void copy2d(Context* ctx, int m, int n, const T* src, int lds, T* dst, int ldd)
 }
 
 // Factor the m x w panel at a (global top row/column `off`): min(m, w)
-// pivot steps, with every one of the w columns receiving the interchanges
-// and the updates. Pivots and payloads are indexed from the panel top. A
-// panel wider than the register kernel holds is factored in column chunks,
-// right-looking: factor a chunk, move its interchanges to the panel's other
-// columns, solve and update the panel columns to its right.
+// pivot steps. The panel's rows span `left` matrix columns to its left and
+// `right` to its right; every interchange reaches all of them (the
+// reference applies a block step's interchanges either side of the panel,
+// Lu.cs:270-278; applying each chunk's as it is made is the same row
+// permutation). Pivots and payloads are indexed from the panel top. A panel
+// wider than the register kernel holds is factored in column chunks,
+// right-looking: factor a chunk, move its interchanges to every other
+// column, solve and update the panel columns to its right.
 template <typename T>
-void panel(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rs, int* rc, int* info,
-           int* list)
+void panel(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rc, int* info, int* list,
+           int left, int right)
 {
     if (m <= 0 || w <= 0) return;
     const int jl = m < w ? m : w;
+    const int ncols = left + w + right;
+    T* rows = a - (size_t)left * lda;  // column 0 of the matrix, panel's top row
     const int cw = reg_panel_width<T>(m);
     if (cw == 0) {
         hipLaunchKernelGGL((getf2_global_kernel<T>), dim3(1), dim3(1024), 0, ctx->stream, m, w, a, lda, piv, off, perm,
-                           rs, (int*)nullptr, info);
-        return;
-    }
-    if (w <= cw) {
-        getf2_reg<T>(ctx, m, w, a, lda, piv, off, perm, rs, nullptr, info);
+                           rc, info);
+        sync_columns<T>(ctx, m, off, rows, lda, left, left + w, ncols, rc, list, 2 * jl);
         return;
     }
     for (int c0 = 0; c0 < jl; c0 += cw) {
         const int cwid = cw < jl - c0 ? cw : jl - c0;
         T* sub = a + c0 + (size_t)c0 * lda;
-        getf2_reg<T>(ctx, m - c0, cwid, sub, lda, piv + c0, off + c0, perm + c0, rs + c0, rc + c0, info);
-        sync_columns<T>(ctx, m - c0, off + c0, a + c0, lda, c0, c0 + cwid, w, rc + c0, list, 2 * cwid);
+        getf2_reg<T>(ctx, m - c0, cwid, sub, lda, piv + c0, off + c0, perm + c0, list, info);
+        apply_moves<T>(ctx, rows + c0, lda, left + c0, left + c0 + cwid, ncols, list, 2 * cwid);
         if (c0 + cwid >= w) continue;
         trsm_llnu<T>(ctx, cwid, w - c0 - cwid, sub, lda, sub + (size_t)cwid * lda, lda);
         if (m > c0 + cwid)
@@ -898,22 +1007,18 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
     if (limit > 0) {
         if (nb < 1) nb = limit < 1024 ? 32 : 64;
         const int maxw = nb < limit ? nb : n;
-        DevBuf<int> rs(ctx, (size_t)m), rc(ctx, (size_t)m);
+        DevBuf<int> rc(ctx, (size_t)m);
         DevBuf<int> list(ctx, 4 * (size_t)maxw + 4);
-        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, d_perm, rs.get());
-        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, rc.get(), (int*)nullptr);
+        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, d_perm, rc.get());
         if (nb >= limit) {
             // Small problems are all panel (Lu.cs:244-249).
-            panel<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, rs, rc, d_info, list);
+            panel<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, rc, d_info, list, 0, 0);
         } else {
             for (int jb = 0; jb < limit; jb += nb) {
                 const int w = nb < limit - jb ? nb : limit - jb;
                 T* pa = a + jb + (size_t)jb * lda;
-                panel<T>(ctx, m - jb, w, pa, lda, d_piv + jb, jb, d_perm + jb, rs.get() + jb, rc.get() + jb, d_info,
-                         list);
-                // This step's interchanges, applied to the columns either
-                // side of the panel (Lu.cs:270-278).
-                sync_columns<T>(ctx, m - jb, jb, a + jb, lda, jb, jb + w, n, rs.get() + jb, list, 2 * w);
+                panel<T>(ctx, m - jb, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb,
+                         n - jb - w);
                 if (jb + w >= n) continue;
                 trsm_llnu<T>(ctx, w, n - jb - w, pa, lda, pa + (size_t)w * lda, lda);
                 if (jb + w >= m) continue;

@@ -74,5 +74,26 @@ __host__ __device__ inline double s_abs(zd z)
 // The pivot measure: |x|, or |Re| + |Im| as LAPACK's izamax (cabs1).
 __host__ __device__ inline double s_pivmag(double a) { return fabs(a); }
 __host__ __device__ inline double s_pivmag(zd a) { return fabs(a.x) + fabs(a.y); }
+// max(|Re|, |Im|), a lower bound of the modulus; NaN if either part is.
+__host__ __device__ inline double s_pivmax(double a) { return fabs(a); }
+__host__ __device__ inline double s_pivmax(zd a)
+{
+    return (a.x != a.x || a.y != a.y) ? a.x + a.y : fmax(fabs(a.x), fabs(a.y));
+}
+
+// c ? a : b, component by component (a conditional expression over
+// struct lvalues would select an address; see load_if).
+__host__ __device__ inline double s_sel(bool c, double a, double b) { return c ? a : b; }
+__host__ __device__ inline zd s_sel(bool c, zd a, zd b) { return make_zd(c ? a.x : b.x, c ? a.y : b.y); }
+
+// *p if ok, else zero. A conditional expression between two lvalues of a
+// struct type selects an address, and an address of a local puts the local
+// in scratch memory; returning by value keeps everything in registers.
+template <typename T>
+__device__ __forceinline__ T load_if(const T* p, bool ok)
+{
+    if (ok) return *p;
+    return s_zero(T{});
+}
 
 }  // namespace tns
