@@ -125,6 +125,57 @@ struct View {
     DevBuf<T> buf;
 };
 
+constexpr size_t kStageMax = size_t(64) << 20;
+
+// The staging buffer, grown to hold min(bytes, kStageMax).
+char* stage(Context* ctx, size_t bytes)
+{
+    const size_t want = bytes < kStageMax ? bytes : kStageMax;
+    if (ctx->h_stage_bytes < want) {
+        if (ctx->h_stage) TNS_HIP(hipHostFree(ctx->h_stage));
+        ctx->h_stage = nullptr;
+        ctx->h_stage_bytes = 0;
+        void* h = nullptr;
+        TNS_HIP(hipHostMalloc(&h, want, hipHostMallocDefault));
+        ctx->h_stage = static_cast<char*>(h);
+        ctx->h_stage_bytes = want;
+    }
+    return ctx->h_stage;
+}
+
+// Host (ld) <-> device (packed, ld = rows) through the staging buffer, in
+// chunks of whole columns. Each chunk waits for its copy before the buffer
+// is reused, so the host pointer may be freed as soon as the call returns.
+template <typename T>
+void host_to_device(Context* ctx, T* d, const T* h, int rows, int cols, int ld)
+{
+    const size_t col_bytes = sizeof(T) * (size_t)rows;
+    char* buf = stage(ctx, col_bytes * cols);
+    const int per = (int)std::max<size_t>(1, ctx->h_stage_bytes / col_bytes);
+    for (int c0 = 0; c0 < cols; c0 += per) {
+        const int nc = std::min(per, cols - c0);
+        if (ctx->h_stage_bytes < col_bytes * nc) buf = stage(ctx, col_bytes * nc);
+        for (int c = 0; c < nc; c++) std::memcpy(buf + col_bytes * c, h + (size_t)(c0 + c) * ld, col_bytes);
+        TNS_HIP(hipMemcpyAsync(d + (size_t)c0 * rows, buf, col_bytes * nc, hipMemcpyHostToDevice, ctx->stream));
+        TNS_HIP(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+template <typename T>
+void device_to_host(Context* ctx, T* h, const T* d, int rows, int cols, int ld)
+{
+    const size_t col_bytes = sizeof(T) * (size_t)rows;
+    char* buf = stage(ctx, col_bytes * cols);
+    const int per = (int)std::max<size_t>(1, ctx->h_stage_bytes / col_bytes);
+    for (int c0 = 0; c0 < cols; c0 += per) {
+        const int nc = std::min(per, cols - c0);
+        if (ctx->h_stage_bytes < col_bytes * nc) buf = stage(ctx, col_bytes * nc);
+        TNS_HIP(hipMemcpyAsync(buf, d + (size_t)c0 * rows, col_bytes * nc, hipMemcpyDeviceToHost, ctx->stream));
+        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < nc; c++) std::memcpy(h + (size_t)(c0 + c) * ld, buf + col_bytes * c, col_bytes);
+    }
+}
+
 template <typename T>
 View<T> bring(Context* ctx, const T* p, int rows, int cols, int ld, bool copy_in)
 {
@@ -139,18 +190,17 @@ View<T> bring(Context* ctx, const T* p, int rows, int cols, int ld, bool copy_in
     v.d = v.buf.get();
     v.ld = rows;
     v.staged = true;
-    if (copy_in)
-        TNS_HIP(hipMemcpy2DAsync(v.d, sizeof(T) * (size_t)v.ld, p, sizeof(T) * (size_t)ld, sizeof(T) * (size_t)rows,
-                                 cols, hipMemcpyHostToDevice, ctx->stream));
+    if (copy_in) host_to_device(ctx, v.d, p, rows, cols, ld);
     return v;
 }
 
+// Only the rows x cols part of the caller's matrix is written (padding
+// rows between columns are left alone).
 template <typename T>
 void send_back(Context* ctx, const View<T>& v, T* p, int rows, int cols, int ld)
 {
     if (!v.staged || rows <= 0 || cols <= 0) return;
-    TNS_HIP(hipMemcpy2DAsync(p, sizeof(T) * (size_t)ld, v.d, sizeof(T) * (size_t)v.ld, sizeof(T) * (size_t)rows, cols,
-                             hipMemcpyDeviceToHost, ctx->stream));
+    device_to_host(ctx, p, v.d, rows, cols, ld);
 }
 
 // Operand.cs:80-96: non-negative shape, stride >= rows, extent in 64 bits.
@@ -509,6 +559,9 @@ TNS_API int tns_context_destroy(tns_context* ctx)
     (void)hipSetDevice(c.device);
     (void)hipStreamSynchronize(c.stream);
     c.pool.trim();
+    if (c.h_stage) (void)hipHostFree(c.h_stage);
+    c.h_stage = nullptr;
+    c.h_stage_bytes = 0;
     (void)hipFree(c.d_scal);
     (void)hipFree(c.d_int);
     (void)hipStreamDestroy(c.stream);

@@ -88,11 +88,13 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     Pool pool{size_t(64) << 30};
-    // Small fixed device and pinned-host areas for reductions and flags.
+    // Small fixed device areas for reductions and flags.
     double* d_scal = nullptr;   // 64 doubles
     int* d_int = nullptr;       // 64 ints
-    double* h_scal = nullptr;   // pinned mirror, 64 doubles
-    int* h_int = nullptr;       // pinned mirror, 64 ints
+    // Page-locked staging for host matrices (grow-only, at most
+    // kStageMax bytes; larger matrices pass through it in column chunks).
+    char* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
     std::string last_error;
     std::mutex gate;            // the Workspace lock (Workspace.cs:136-139)
     bool timing = false;
