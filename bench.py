@@ -14,6 +14,13 @@ CPU baseline: the oracle (a C restatement of Tensile's algorithms, OpenMP
 GEMM) running one exponential of the same workload on the host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py --roofline-only   # the dominant kernel alone (20 launches)
+                                    # and a 1 GiB device copy, for rocprofv3
+
+`roofline.traffic` is read from profiles/zgemm_n1000_traffic.json when that
+file exists: HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and
+WRITE_SIZE runs of --roofline-only, scaled by what the same counters read
+for the 1 GiB copy.
 """
 from __future__ import annotations
 
@@ -69,16 +76,55 @@ def event_time(ws, fn, reps):
     return start.elapsed_time(stop) / reps / 1e3
 
 
+def roofline_only():
+    """The dominant kernel alone: 20 launches of the 4M ZGEMM at n = 1000 on
+    the benchmark's operator, then 20 device copies of 1 GiB (the counter
+    calibration: 1 GiB read and 1 GiB written per launch)."""
+    import torch
+
+    import csharp_linearalgebra_amd as T
+    from csharp_linearalgebra_amd import workloads as W
+
+    ws = T.Workspace(0)
+    n = 1000
+    a, _ = W.expm_operator(n, 50.0, 8, seed=0)
+    dA = to_device(a)
+    dC = empty_device(n, n, True)
+    torch.cuda.synchronize()
+    s = event_time(ws, lambda: ws.multiply(dA, dA, dC), 20)
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(20):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"zgemm_n1000_us": round(s * 1e6, 2), "copy_bytes": 2 * src.numel() * 8}))
+    ws.dispose()
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--roofline-only", action="store_true", help="run only the dominant kernel (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this backend has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
+    if args.roofline_only:
+        roofline_only()
+        return
 
     import torch
 
@@ -169,7 +215,7 @@ def main():
                    "n": n, "degree": diag.degree, "squarings": diag.squarings},
         "roofline": {"bound": "mfma", "kernel": "zgemm_kernel (4M, n=1000)", "achieved": round(zg_tflops, 2),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(zg_tflops / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "avg_launch_us": round(g_s * 1e6, 1)},
+                     "traffic": load_traffic(), "avg_launch_us": round(g_s * 1e6, 1)},
         "expmv": {"workload": "complex128 exp(tA)B, n=1000, B 1000x8, t=0.5", "ms": round(expmv_s * 1e3, 3),
                   "degree": vd.degree, "scaling": vd.scaling, "applications": vd.applications,
                   "gflops": round(expmv_flops / expmv_s / 1e9, 2)},
@@ -183,14 +229,21 @@ def main():
     if not args.no_cpu:
         import oracle as O
 
-        t0 = time.perf_counter()
-        _, cdiag = O.expm(a)
-        cpu_s = time.perf_counter() - t0
-        cflops = W.expm_flops(n, cdiag[1], True)
+        # A bounded sample: whole exponentials of the same operator until
+        # about 10 s of host time (at most 30 s) have passed.
+        reps, cflops, t0 = 0, 0.0, time.perf_counter()
+        while True:
+            _, cdiag = O.expm(a)
+            reps += 1
+            cflops += W.expm_flops(n, cdiag[1], True)
+            cpu_s = time.perf_counter() - t0
+            if cpu_s >= 10.0 or cpu_s * (reps + 1) / reps > 30.0:
+                break
         result["cpu_baseline"] = {"value": round(cflops / cpu_s / 1e9, 2), "unit": "GFLOP/s",
                                   "cores": O.threads(), "kind": "port",
-                                  "sample": "one complex128 Expm of the same n=1000 operator "
-                                            "(C restatement of Tensile, OpenMP AVX2 GEMM), %.1f s" % cpu_s}
+                                  "sample": "%d complex128 Expm of the same n=1000 operator (degree %d, s=%d; "
+                                            "C restatement of Tensile, OpenMP AVX2 GEMM), %.1f s"
+                                            % (reps, cdiag[0], cdiag[1], cpu_s)}
     print(json.dumps(result))
     ws.dispose()
 
