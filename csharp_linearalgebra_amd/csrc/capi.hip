@@ -531,6 +531,13 @@ TNS_API int tns_context_create(int device, tns_context** out)
     if (e == hipSuccess) e = hipMalloc(&c.d_scal, 64 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c.d_int, 64 * sizeof(int));
     if (e == hipSuccess) {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+            c.max_lds = lds;
+        else
+            (void)hipGetLastError();
+    }
+    if (e == hipSuccess) {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0) c.pool.set_limit(total_b / 4 * 3);
         else (void)hipGetLastError();

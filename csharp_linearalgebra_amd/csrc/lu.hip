@@ -214,7 +214,7 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
 
     T v[R][NB];
     T nx[R];
-    int p0[R], pos[R];
+    int pos[R];
     unsigned done = 0;  // bit r: slot r is a pivot row already, or padding
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -223,7 +223,6 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
 #pragma unroll
         for (int c = 0; c < NB; c++) v[r][c] = load_if(a + row + (size_t)c * lda, ok && c < w);
         nx[r] = v[r][0];
-        p0[r] = ok ? perm[row] : 0;
         pos[r] = row;
         if (!ok) done |= 1u << r;
     }
@@ -364,6 +363,15 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
         }
     }
 
+    // The perm payloads move with the rows: every slot reads its own before
+    // any is overwritten.
+    int p0[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int row = tid + r * nt;
+        p0[r] = row < m ? perm[row] : 0;
+    }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int row = tid + r * nt;
@@ -721,6 +729,117 @@ trsv_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
     }
 }
 
+// trsv_kernel with the right-hand side held in LDS for the whole solve
+// (dynamic shared memory, n elements) and each diagonal block solved from
+// registers: lane i of wavefront 0 holds row i of the block of op(A) and
+// its x_i; step p divides on lane p (unless UNIT), reads x_p back with
+// readlane and updates the lanes ahead. Same operation order as trsv_kernel.
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane);
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <>
+__device__ __forceinline__ zd readlane_t<zd>(zd v, int lane)
+{
+    return make_zd(readlane_t(v.x, lane), readlane_t(v.y, lane));
+}
+
+template <typename T, bool LOWER, int TR, bool UNIT>
+__global__ void __launch_bounds__(256)
+trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
+{
+    constexpr int BS = tri_block<T>();
+    extern __shared__ unsigned char trsv_smem[];
+    T* xs = reinterpret_cast<T*>(trsv_smem);
+    __shared__ T part[BS];
+    T* xg = B + (size_t)blockIdx.x * ldb;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto at = [&](int i, int k) -> T {
+        T v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+        return TR == 2 ? s_conj(v) : v;
+    };
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    for (int i = tid; i < n; i += 256) xs[i] = xg[i];
+    __syncthreads();
+
+    const int nblk = (n + BS - 1) / BS;
+    for (int bq = 0; bq < nblk; bq++) {
+        const int blk = FORWARD ? bq : nblk - 1 - bq;
+        const int k0 = blk * BS;
+        const int kb = n - k0 < BS ? n - k0 : BS;
+        if (TR != 0) {
+            // Left-looking: the solved part, a dot product down column
+            // (k0 + i) of A per block row i.
+            const int lo = FORWARD ? 0 : k0 + kb;
+            const int hi = FORWARD ? k0 : n;
+            for (int i = wave; i < kb; i += 4) {
+                T s0 = s_zero(T{}), s1 = s_zero(T{});
+                int k = lo + lane;
+                for (; k + 64 < hi; k += 128) {
+                    s0 = s_add(s0, s_mul(at(k0 + i, k), xs[k]));
+                    s1 = s_add(s1, s_mul(at(k0 + i, k + 64), xs[k + 64]));
+                }
+                if (k < hi) s0 = s_add(s0, s_mul(at(k0 + i, k), xs[k]));
+                T sum = s_add(s0, s1);
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) sum = s_add(sum, shfl_xor_t(sum, off));
+                if (lane == 0) part[i] = sum;
+            }
+            __syncthreads();
+        }
+        if (wave == 0) {
+            const bool mine = lane < kb;
+            T d[BS];
+#pragma unroll
+            for (int k = 0; k < BS; k++) d[k] = (mine && k < kb) ? at(k0 + lane, k0 + k) : s_zero(T{});
+            T xi = load_if(xs + k0 + lane, mine);
+            if (TR != 0 && mine) xi = s_sub(xi, part[lane]);
+#pragma unroll
+            for (int s = 0; s < BS; s++) {
+                const int p = FORWARD ? s : BS - 1 - s;
+                if (p >= kb) continue;
+                if (!UNIT && lane == p) xi = s_div(xi, d[p]);
+                const T xp = readlane_t(xi, p);
+                if (s_iszero(xp)) continue;
+                const bool ahead = FORWARD ? (lane > p && lane < kb) : (lane < p);
+                if (ahead) xi = s_add(xi, s_mul(s_neg(xp), d[p]));
+            }
+            if (mine) xs[k0 + lane] = xi;
+        }
+        __syncthreads();
+        if (TR == 0) {
+            // Right-looking: push this block into the rows still to come, in
+            // the reference's axpy order, two rows at a time.
+            const int r0 = FORWARD ? k0 + kb : 0;
+            const int r1 = FORWARD ? n : k0;
+            for (int i = r0 + tid; i < r1; i += 512) {
+                const int i2 = i + 256;
+                const bool two = i2 < r1;
+                T acc = xs[i];
+                T acc2 = two ? xs[i2] : s_zero(T{});
+                for (int q = 0; q < kb; q++) {
+                    const int p = FORWARD ? q : kb - 1 - q;
+                    const T xp = xs[k0 + p];
+                    if (s_iszero(xp)) continue;
+                    const T nx = s_neg(xp);
+                    acc = s_add(acc, s_mul(nx, at(i, k0 + p)));
+                    if (two) acc2 = s_add(acc2, s_mul(nx, at(i2, k0 + p)));
+                }
+                xs[i] = acc;
+                if (two) xs[i2] = acc2;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < n; i += 256) xg[i] = xs[i];
+}
+
 // dst[i] = src[perm[i]] (gather) or dst[perm[i]] = src[i] (scatter).
 template <typename T>
 __global__ void permute_rows_kernel(int n, int ncols, const int* __restrict__ perm, const T* __restrict__ src, int lds,
@@ -901,7 +1020,14 @@ void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb
 {
     if (n <= 0 || nrhs <= 0) return;
     if (nrhs <= 32 || n <= 64) {
-        hipLaunchKernelGGL((trsv_kernel<T, LOWER, TR, UNIT>), dim3(nrhs), dim3(256), 0, ctx->stream, n, A, lda, B, ldb);
+        const size_t xbytes = sizeof(T) * (size_t)n;
+        if (xbytes + sizeof(T) * tri_block<T>() <= (size_t)ctx->max_lds) {
+            hipLaunchKernelGGL((trsv_lds_kernel<T, LOWER, TR, UNIT>), dim3(nrhs), dim3(256), xbytes, ctx->stream, n, A,
+                               lda, B, ldb);
+        } else {
+            hipLaunchKernelGGL((trsv_kernel<T, LOWER, TR, UNIT>), dim3(nrhs), dim3(256), 0, ctx->stream, n, A, lda, B,
+                               ldb);
+        }
         return;
     }
     constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);

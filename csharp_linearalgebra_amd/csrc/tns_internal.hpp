@@ -86,6 +86,7 @@ private:
 
 struct Context {
     int device = 0;
+    int max_lds = 65536;  // shared memory per workgroup on this device, bytes
     hipStream_t stream = nullptr;
     Pool pool{size_t(64) << 30};
     // Small fixed device areas for reductions and flags.
