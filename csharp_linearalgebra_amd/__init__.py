@@ -19,6 +19,7 @@ from .tensile import (  # noqa: F401
     ILinearOperator,
     InvalidOperationError,
     LuDecomposition,
+    MixedInfo,
     NormEstimateResult,
     ObjectDisposedError,
     OperatorError,

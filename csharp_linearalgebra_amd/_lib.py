@@ -37,6 +37,12 @@ class ExpmDiag(C.Structure):
     _fields_ = [("degree", C.c_int), ("squarings", C.c_int)]
 
 
+class MixedInfo(C.Structure):
+    _fields_ = [("precision", C.c_int), ("iterations", C.c_int), ("fell_back", C.c_int),
+                ("backward_error", C.c_double), ("factor_ms", C.c_double), ("refine_ms", C.c_double),
+                ("fallback_ms", C.c_double), ("first_error", C.c_double)]
+
+
 class ExpmvDiag(C.Structure):
     _fields_ = [("degree", C.c_int), ("scaling", C.c_int), ("applications", C.c_int), ("early_exits", C.c_int)]
 
@@ -72,6 +78,10 @@ SIGNATURES = {
     "tns_znormest_power": (_I, [_P, _I, _P, _I, _I, _I, _I, _I, C.POINTER(NormEst)]),
     "tns_dgecon": (_I, [_P, _I, _P, _I, _P, _I, _D, _I, C.POINTER(_D)]),
     "tns_zgecon": (_I, [_P, _I, _P, _I, _P, _I, _D, _I, C.POINTER(_D)]),
+    "tns_dgesv_mixed": (_I, [_P, _I, _I, _P, _I, _P, _I, _P, _I, _I, _D, _I, C.POINTER(MixedInfo)]),
+    "tns_sgemm": (_I, [_P, _I, _I, _I, _I, C.c_float, _P, _I, _P, _I, C.c_float, _P, _I]),
+    "tns_sgetrf": (_I, [_P, _I, _I, _P, _I, _P, C.POINTER(_I)]),
+    "tns_sgetrs": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _I]),
     "tns_dexpm": (_I, [_P, _I, _P, _I, _P, _I, C.POINTER(ExpmDiag)]),
     "tns_zexpm": (_I, [_P, _I, _P, _I, _P, _I, C.POINTER(ExpmDiag)]),
     "tns_dexpmv": (_I, [_P, _I, _I, _P, _I, _P, _I, _D, _P, _I, C.POINTER(ExpmvDiag)]),

@@ -9,6 +9,7 @@
 
 #include "algos.hpp"
 #include "blas1.hpp"
+#include "mixed.hpp"
 
 struct tns_context {
     tns::Context impl;
@@ -357,6 +358,30 @@ void gecon_body(Context* ctx, int n, const T* lu, int ldlu, const int* pivots, i
     } else {
         *rcond = tns::gecon<T>(ctx, n, vlu.d, vlu.ld, perm.get(), singular, anorm, columns);
     }
+}
+
+void mixed_body(Context* ctx, int n, int nrhs, const double* a, int lda, const double* b, int ldb, double* x, int ldx,
+                int precision, double tol, int max_iter, tns_mixed_info* info)
+{
+    if (n < 0 || nrhs < 0) fail(TNS_EINVAL, "negative dimension");
+    if (precision != 32 && precision != 16) fail(TNS_EINVAL, "precision must be 32 or 16");
+    if (!(tol > 0.0) || max_iter < 1) fail(TNS_ERANGE, "tolerance must be positive and max_iter at least 1");
+    check_operand(a, n, n, lda, "A");
+    check_operand(b, n, nrhs, ldb, "B");
+    check_operand(x, n, nrhs, ldx, "X");
+    tns::MixedInfo mi;
+    mi.precision = precision;
+    if (n > 0 && nrhs > 0) {
+        View<double> va = bring(ctx, a, n, n, lda, true);
+        View<double> vb = bring(ctx, b, n, nrhs, ldb, true);
+        View<double> vx = bring(ctx, x, n, nrhs, ldx, false);
+        tns::dgesv_mixed(ctx, n, nrhs, va.d, va.ld, vb.d, vb.ld, vx.d, vx.ld, precision, tol, max_iter, &mi);
+        send_back(ctx, vx, x, n, nrhs, ldx);
+        tns::sync(ctx);
+    }
+    if (info)
+        *info = tns_mixed_info{mi.precision, mi.iterations, mi.fell_back, mi.backward_error, mi.factor_ms,
+                               mi.refine_ms, mi.fallback_ms, mi.first_error};
 }
 
 template <typename T>
@@ -748,6 +773,79 @@ TNS_API int tns_zgecon(tns_context* ctx, int n, const tns_complex* lu, int ldlu,
 {
     return guarded(ctx, [&](Context* c) {
         gecon_body<zd>(c, n, (const zd*)lu, ldlu, pivots, singular_column, anorm, columns, rcond);
+    });
+}
+
+TNS_API int tns_dgesv_mixed(tns_context* ctx, int n, int nrhs, const double* a, int lda, const double* b, int ldb,
+                            double* x, int ldx, int precision, double tol, int max_iter, tns_mixed_info* info)
+{
+    return guarded(ctx, [&](Context* c) { mixed_body(c, n, nrhs, a, lda, b, ldb, x, ldx, precision, tol, max_iter, info); });
+}
+
+TNS_API int tns_sgemm(tns_context* ctx, int precision, int m, int n, int k, float alpha, const float* a, int lda,
+                      const float* b, int ldb, float beta, float* c, int ldc)
+{
+    return guarded(ctx, [&](Context* cx) {
+        if (precision != 32 && precision != 16) fail(TNS_EINVAL, "precision must be 32 or 16");
+        if (m < 0 || n < 0 || k < 0) fail(TNS_EINVAL, "negative dimension");
+        check_operand(c, m, n, ldc, "C");
+        if (m == 0 || n == 0) return;
+        check_operand(a, m, k, lda, "A");
+        check_operand(b, k, n, ldb, "B");
+        View<float> va = bring(cx, a, m, k, lda, true);
+        View<float> vb = bring(cx, b, k, n, ldb, true);
+        View<float> vc = bring(cx, c, m, n, ldc, beta != 0.f);
+        tns::sgemm_lp(cx, precision == 16 ? tns::LP_BF16 : tns::LP_FP32, m, n, k, alpha, va.d, va.ld, vb.d, vb.ld,
+                      beta, vc.d, vc.ld);
+        send_back(cx, vc, c, m, n, ldc);
+        tns::sync(cx);
+    });
+}
+
+TNS_API int tns_sgetrf(tns_context* ctx, int precision, int n, float* a, int lda, int* pivots, int* singular_column)
+{
+    return guarded(ctx, [&](Context* cx) {
+        if (precision != 32 && precision != 16) fail(TNS_EINVAL, "precision must be 32 or 16");
+        if (n < 0) fail(TNS_EINVAL, "negative order");
+        check_operand(a, n, n, lda, "A");
+        if (n == 0) {
+            if (singular_column) *singular_column = -1;
+            return;
+        }
+        View<float> va = bring(cx, a, n, n, lda, true);
+        DevBuf<int> piv(cx, n), perm(cx, n), info(cx, 1);
+        const int saved = cx->lp_mode;
+        cx->lp_mode = precision == 16 ? tns::LP_BF16 : tns::LP_FP32;
+        tns::sgetrf(cx, n, n, va.d, va.ld, 0, piv.get(), perm.get(), info.get());
+        cx->lp_mode = saved;
+        send_back(cx, va, a, n, n, lda);
+        int h = -1;
+        TNS_HIP(hipMemcpyAsync(pivots, piv.get(), sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, cx->stream));
+        TNS_HIP(hipMemcpyAsync(&h, info.get(), sizeof(int), hipMemcpyDeviceToHost, cx->stream));
+        tns::sync(cx);
+        if (singular_column) *singular_column = h;
+    });
+}
+
+TNS_API int tns_sgetrs(tns_context* ctx, int precision, int n, int nrhs, const float* lu, int ldlu, const int* pivots,
+                       float* b, int ldb)
+{
+    return guarded(ctx, [&](Context* cx) {
+        if (precision != 32 && precision != 16) fail(TNS_EINVAL, "precision must be 32 or 16");
+        if (n < 0 || nrhs < 0) fail(TNS_EINVAL, "negative dimension");
+        check_operand(lu, n, n, ldlu, "LU");
+        check_operand(b, n, nrhs, ldb, "B");
+        if (n == 0 || nrhs == 0) return;
+        std::vector<int> perm = permutation_of(cx, pivots, n, n);
+        DevBuf<int> dperm = upload_ints(cx, perm);
+        View<float> vlu = bring(cx, lu, n, n, ldlu, true);
+        View<float> vb = bring(cx, b, n, nrhs, ldb, true);
+        const int saved = cx->lp_mode;
+        cx->lp_mode = precision == 16 ? tns::LP_BF16 : tns::LP_FP32;
+        tns::sgetrs(cx, n, nrhs, vlu.d, vlu.ld, dperm.get(), vb.d, vb.ld);
+        cx->lp_mode = saved;
+        send_back(cx, vb, b, n, nrhs, ldb);
+        tns::sync(cx);
     });
 }
 

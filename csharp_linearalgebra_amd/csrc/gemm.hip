@@ -19,6 +19,7 @@
 // remapped so each XCD receives a contiguous band of tiles, grouped 8 tile
 // rows at a time, and the A row-panels and B column-panels a band shares
 // stay in that XCD's L2.
+#include "tile_order.hpp"
 #include "tns_internal.hpp"
 
 namespace tns {
@@ -29,22 +30,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// blockIdx -> (tile_m, tile_n): XCD-contiguous bands, then grouped order.
-__device__ __forceinline__ void tile_of(int bid, int grid, int tiles_m, int tiles_n, int& tm, int& tn)
-{
-    int per = grid >> 3;
-    int id = bid;
-    if (bid < (per << 3)) id = (bid & 7) * per + (bid >> 3);
-    constexpr int GROUP = 8;
-    int group_size = GROUP * tiles_n;
-    int group = id / group_size;
-    int first = group * GROUP;
-    int gm = tiles_m - first < GROUP ? tiles_m - first : GROUP;
-    int in = id - group * group_size;
-    tm = first + in % gm;
-    tn = in / gm;
 }
 
 template <int BM, int BN, int BK, int WM, int WN>

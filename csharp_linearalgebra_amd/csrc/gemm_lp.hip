@@ -1,0 +1,207 @@
+// Single-precision products for the mixed-precision factorisation: the
+// trailing updates C := beta*C + alpha*A*B of an FP32 LU (and of its
+// solves), on the matrix cores with FP32 accumulation. Two input modes:
+//   LP_FP32: v_mfma_f32_16x16x4f32, operands as stored;
+//   LP_BF16: v_mfma_f32_16x16x32_bf16, operands rounded to bfloat16 (to
+//            nearest, ties to even) as they are staged to LDS.
+// C and the accumulators stay FP32 in both. A is never transposed here.
+//
+// Tiles of 64 x 64 x 32 per workgroup of four waves (2 x 2, 32 x 32 each,
+// four 16 x 16 MFMA blocks per wave); the tile order is the XCD-banded,
+// grouped order of the FP64 kernels. Loads are clamped into the matrix and
+// zeroed at the edges, never branched around.
+#include "tile_order.hpp"
+#include "tns_internal.hpp"
+
+namespace tns {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int LBM = 64, LBN = 64, LBK = 32, LNT = 256;
+
+// Round to nearest, ties to even; NaN stays NaN (quiet).
+__device__ __forceinline__ unsigned short to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <int MODE>
+struct LpPlanes;
+
+// FP32: k-contiguous rows padded to 33 floats (the 16 rows of a fragment
+// read hit distinct banks).
+template <>
+struct LpPlanes<LP_FP32> {
+    static constexpr int KP = LBK + 1;
+    static constexpr int WORDS = LBM * KP;  // floats per plane
+};
+
+// BF16: k-contiguous rows of 40 bf16 (80 bytes: 16-byte aligned, and a
+// lane's eight consecutive k are one 16-byte read).
+template <>
+struct LpPlanes<LP_BF16> {
+    static constexpr int KP = LBK + 8;
+    static constexpr int WORDS = LBM * KP / 2;  // 32-bit words per plane
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(LNT, 2)
+lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int lda, const float* __restrict__ B,
+              int ldb, float beta, float* __restrict__ C, int ldc, int tiles_m, int tiles_n)
+{
+    using P = LpPlanes<MODE>;
+    __shared__ __align__(16) unsigned As[P::WORDS];
+    __shared__ __align__(16) unsigned Bs[P::WORDS];
+
+    int tmi, tni;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
+    const int m0 = tmi * LBM, n0 = tni * LBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    v4f acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    constexpr int LA = LBM * LBK / LNT;  // 8 elements of A and of B per thread
+    float ra[LA], rb[LA];
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < LA; r++) {
+            const int e = tid + r * LNT;
+            // A: i fastest (coalesced down a column).
+            const int i = e % LBM, kk = e / LBM;
+            const int gi = m0 + i, gk = k0 + kk;
+            const int ci = gi < m ? gi : m - 1, ck = gk < k ? gk : k - 1;
+            const float va = A[ci + (size_t)ck * lda];
+            ra[r] = (gi < m && gk < k) ? va : 0.f;
+            // B: k fastest.
+            const int kb = e % LBK, j = e / LBK;
+            const int gkb = k0 + kb, gj = n0 + j;
+            const int ckb = gkb < k ? gkb : k - 1, cj = gj < n ? gj : n - 1;
+            const float vb = B[ckb + (size_t)cj * ldb];
+            rb[r] = (gkb < k && gj < n) ? vb : 0.f;
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int r = 0; r < LA; r++) {
+            const int e = tid + r * LNT;
+            const int i = e % LBM, kk = e / LBM;
+            const int kb = e % LBK, j = e / LBK;
+            if (MODE == LP_FP32) {
+                As[i * P::KP + kk] = __float_as_uint(ra[r]);
+                Bs[j * P::KP + kb] = __float_as_uint(rb[r]);
+            } else {
+                reinterpret_cast<unsigned short*>(As)[i * P::KP + kk] = to_bf16(ra[r]);
+                reinterpret_cast<unsigned short*>(Bs)[j * P::KP + kb] = to_bf16(rb[r]);
+            }
+        }
+    };
+
+    const int ktiles = (k + LBK - 1) / LBK;
+    gload(0);
+    sstore();
+    __syncthreads();
+
+    const int frow = lane & 15, kq = lane >> 4;
+    for (int t = 0; t < ktiles; ++t) {
+        if (t + 1 < ktiles) gload((t + 1) * LBK);
+        if (MODE == LP_FP32) {
+#pragma unroll
+            for (int kk = 0; kk < LBK; kk += 4) {
+                float af[2], bf[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) af[i] = __uint_as_float(As[(wm * 32 + i * 16 + frow) * P::KP + kk + kq]);
+#pragma unroll
+                for (int j = 0; j < 2; j++) bf[j] = __uint_as_float(Bs[(wn * 32 + j * 16 + frow) * P::KP + kk + kq]);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+            }
+        } else {
+            const unsigned short* as = reinterpret_cast<const unsigned short*>(As);
+            const unsigned short* bs = reinterpret_cast<const unsigned short*>(Bs);
+            bf16x8 af[2], bfr[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(as + (wm * 32 + i * 16 + frow) * P::KP + 8 * kq));
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                bfr[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(bs + (wn * 32 + j * 16 + frow) * P::KP + 8 * kq));
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (t + 1 < ktiles) {
+            sstore();
+            __syncthreads();
+        }
+    }
+
+    // D layout of both 16x16 FP32-accumulating forms: lane l, register r
+    // holds row 4*(l/16) + r, column l%16 of the block.
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+            if (col >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm * 32 + i * 16 + 4 * kq + r;
+                if (row >= m) continue;
+                float* p = C + row + (size_t)col * ldc;
+                const float v = alpha * acc[i][j][r];
+                *p = beta == 0.f ? v : beta * *p + v;
+            }
+        }
+}
+
+__global__ void lp_scale_kernel(int m, int n, float beta, float* __restrict__ c, int ldc)
+{
+    const size_t total = (size_t)m * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % m), j = (int)(e / m);
+        float* p = c + i + (size_t)j * ldc;
+        *p = beta == 0.f ? 0.f : beta * *p;
+    }
+}
+
+}  // namespace
+
+void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const float* a, int lda, const float* b,
+              int ldb, float beta, float* c, int ldc)
+{
+    if (m <= 0 || n <= 0) return;
+    if (k == 0 || alpha == 0.f) {
+        const size_t total = (size_t)m * n;
+        const int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+        hipLaunchKernelGGL(lp_scale_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, beta, c, ldc);
+        TNS_HIP(hipGetLastError());
+        return;
+    }
+    const int tm = (m + LBM - 1) / LBM, tn = (n + LBN - 1) / LBN;
+    if (mode == LP_BF16)
+        hipLaunchKernelGGL(lpgemm_kernel<LP_BF16>, dim3(tm * tn), dim3(LNT), 0, ctx->stream, m, n, k, alpha, a, lda, b,
+                           ldb, beta, c, ldc, tm, tn);
+    else
+        hipLaunchKernelGGL(lpgemm_kernel<LP_FP32>, dim3(tm * tn), dim3(LNT), 0, ctx->stream, m, n, k, alpha, a, lda, b,
+                           ldb, beta, c, ldc, tm, tn);
+    TNS_HIP(hipGetLastError());
+}
+
+}  // namespace tns

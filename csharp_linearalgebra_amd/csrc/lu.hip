@@ -37,7 +37,14 @@
 namespace tns {
 namespace {
 
-constexpr double kSafeMin = 2.2250738585072014e-308;
+// Smallest normal number of the working precision: below it the reference
+// divides by the pivot instead of multiplying by its reciprocal
+// (Lu.cs:407-412). The float form serves the mixed-precision factorisation.
+template <typename T>
+constexpr double safe_min()
+{
+    return sizeof(T) == 4 ? 1.1754943508222875e-38 : 2.2250738585072014e-308;
+}
 
 __device__ __forceinline__ void best_combine(double& bv, int& bi, double v, int i)
 {
@@ -124,6 +131,13 @@ __device__ __forceinline__ zd shfl_t<zd>(zd v, int src)
     return make_zd(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
+template <>
+__device__ __forceinline__ float shfl_t<float>(float v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+__device__ __forceinline__ float shfl_xor_t(float v, int off) { return __shfl_xor(v, off, 64); }
 __device__ __forceinline__ double shfl_xor_t(double v, int off) { return __shfl_xor(v, off, 64); }
 __device__ __forceinline__ zd shfl_xor_t(zd v, int off)
 {
@@ -163,7 +177,7 @@ __device__ __forceinline__ zd pick(const zd (&v)[N], int c)
 // 64-element complex column does not fit the register file, nor a 64 x 64
 // complex block in one workgroup's LDS).
 template <typename T>
-constexpr int tri_block() { return sizeof(T) == 8 ? 64 : 32; }
+constexpr int tri_block() { return sizeof(T) == 16 ? 32 : 64; }
 
 // y if take, else x, by masks.
 __device__ __forceinline__ double blend(double x, double y, bool take)
@@ -294,8 +308,8 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
         // >= max(|Re|, |Im|), so the modulus is needed only when both parts
         // are below the safe minimum.
         if (!zero_pivot) {
-            bool safe = s_pivmax(pv) >= kSafeMin;
-            if (!safe) safe = s_abs(pv) >= kSafeMin;
+            bool safe = s_pivmax(pv) >= safe_min<T>();
+            if (!safe) safe = s_abs(pv) >= safe_min<T>();
             if (safe) {
                 const T recip = s_div(s_one(T{}), pv);
 #pragma unroll
@@ -453,7 +467,7 @@ getf2_global_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
         const T recip = s_div(s_one(T{}), pv);
         if (!zero_pivot)
             for (int row = j + 1 + tid; row < m; row += nt)
-                col[row] = (mag >= kSafeMin) ? s_mul(col[row], recip) : s_div(col[row], pv);
+                col[row] = (mag >= safe_min<T>()) ? s_mul(col[row], recip) : s_div(col[row], pv);
         __syncthreads();
         for (int c = j + 1; c < w; ++c) {
             T* tc = a + (size_t)c * lda;
@@ -745,6 +759,11 @@ __device__ __forceinline__ double readlane_t<double>(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
 __device__ __forceinline__ zd readlane_t<zd>(zd v, int lane)
 {
     return make_zd(readlane_t(v.x, lane), readlane_t(v.y, lane));
@@ -909,6 +928,17 @@ struct Ops<double> {
     static double neg_one() { return -1.0; }
 };
 template <>
+struct Ops<float> {
+    static void gemm(Context* c, int ta, int m, int n, int k, float alpha, const float* a, int lda, const float* b,
+                     int ldb, float beta, float* cc, int ldc)
+    {
+        if (ta != 0) fail(TNS_EINVAL, "transposed single-precision products are not provided");
+        sgemm_lp(c, c->lp_mode, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
+    }
+    static float one() { return 1.0f; }
+    static float neg_one() { return -1.0f; }
+};
+template <>
 struct Ops<zd> {
     static void gemm(Context* c, int ta, int m, int n, int k, zd alpha, const zd* a, int lda,
                      const zd* b, int ldb, zd beta, zd* cc, int ldc)
@@ -935,7 +965,14 @@ void launch_getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int o
 template <typename T>
 int reg_panel_width(int m)
 {
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 4) {
+        // float: 128 values per thread fit where double fits 64.
+        if (m <= 1024) return 64;
+        if (m <= 2048) return 32;
+        if (m <= 4096) return 16;
+        if (m <= 8192) return 8;
+        return 0;
+    } else if constexpr (sizeof(T) == 8) {
         if (m <= 512) return 64;
         if (m <= 1024) return 32;
         if (m <= 2048) return 16;
@@ -955,7 +992,13 @@ int reg_panel_width(int m)
 template <typename T>
 void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* list, int* info)
 {
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 4) {
+        if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 1024) launch_getf2_reg<T, 64, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 2048) launch_getf2_reg<T, 32, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 4096) launch_getf2_reg<T, 16, 8>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else launch_getf2_reg<T, 8, 16>(ctx, m, w, a, lda, piv, off, perm, list, info);
+    } else if constexpr (sizeof(T) == 8) {
         if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 1024) launch_getf2_reg<T, 32, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 2048) launch_getf2_reg<T, 16, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);
@@ -1190,6 +1233,16 @@ void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, 
 void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
 {
     getrf_impl<zd>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+}
+
+void sgetrf(Context* ctx, int m, int n, float* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+{
+    getrf_impl<float>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+}
+
+void sgetrs(Context* ctx, int n, int nrhs, const float* lu, int ldlu, const int* d_perm, float* b, int ldb)
+{
+    getrs_impl<float>(ctx, 0, n, nrhs, lu, ldlu, d_perm, b, ldb);
 }
 
 void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb)

@@ -87,6 +87,7 @@ private:
 struct Context {
     int device = 0;
     int max_lds = 65536;  // shared memory per workgroup on this device, bytes
+    int lp_mode = 0;      // LP_FP32 or LP_BF16: products inside sgetrf/sgetrs
     hipStream_t stream = nullptr;
     Pool pool{size_t(64) << 30};
     // Small fixed device areas for reductions and flags.
@@ -158,6 +159,18 @@ void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, doubl
 // min/max |u_ii| into d_out[0], d_out[1] (and a NaN flag).
 void dpivot_range(Context* ctx, int n, const double* a, int lda, double* d_out);
 void zpivot_range(Context* ctx, int n, const cplx* a, int lda, double* d_out);
+
+// Mixed precision (gemm_lp.hip, lu.hip). Single-precision products on the
+// matrix cores with FP32 accumulation; mode LP_FP32 takes the operands as
+// they are, LP_BF16 rounds them to bfloat16 first. A untransposed.
+enum { LP_FP32 = 0, LP_BF16 = 1 };
+void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const float* a, int lda, const float* b,
+              int ldb, float beta, float* c, int ldc);
+// FP32 LU with partial pivoting; its trailing updates use sgemm_lp in
+// ctx->lp_mode. Same outputs as dgetrf.
+void sgetrf(Context* ctx, int m, int n, float* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info);
+// A X = B against FP32 factors (no transpose).
+void sgetrs(Context* ctx, int n, int nrhs, const float* lu, int ldlu, const int* d_perm, float* b, int ldb);
 
 // Level-1/2 helpers (blas1.hip).
 void dnorm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* d_out);

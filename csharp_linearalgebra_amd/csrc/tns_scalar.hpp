@@ -18,6 +18,21 @@ struct alignas(16) zd {
 };
 __host__ __device__ inline zd make_zd(double x, double y) { return zd{x, y}; }
 
+// float: the working precision of the mixed-precision factorisation.
+__host__ __device__ inline float s_zero(float) { return 0.0f; }
+__host__ __device__ inline float s_one(float) { return 1.0f; }
+__host__ __device__ inline float s_add(float a, float b) { return a + b; }
+__host__ __device__ inline float s_sub(float a, float b) { return a - b; }
+__host__ __device__ inline float s_mul(float a, float b) { return a * b; }
+__host__ __device__ inline float s_neg(float a) { return -a; }
+__host__ __device__ inline float s_conj(float a) { return a; }
+__host__ __device__ inline bool s_iszero(float a) { return a == 0.0f; }
+__host__ __device__ inline float s_div(float a, float b) { return a / b; }
+__host__ __device__ inline double s_abs(float a) { return fabsf(a); }
+__host__ __device__ inline double s_pivmag(float a) { return fabsf(a); }
+__host__ __device__ inline double s_pivmax(float a) { return fabsf(a); }
+__host__ __device__ inline float s_sel(bool c, float a, float b) { return c ? a : b; }
+
 __host__ __device__ inline double s_zero(double) { return 0.0; }
 __host__ __device__ inline zd s_zero(zd) { return make_zd(0.0, 0.0); }
 __host__ __device__ inline double s_one(double) { return 1.0; }

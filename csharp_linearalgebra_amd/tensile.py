@@ -273,6 +273,27 @@ class Workspace:
         return NormEstimateResult(out.value, out.iterations, out.products)
 
     # ---- exponentials ---------------------------------------------------------------------
+    def solve_mixed(self, a, b, precision: int = 32, tol: float = 1e-15, max_iterations: int = 30, out=None):
+        """X with A X = B by a low-precision LU (FP32, or BF16 products with
+        equilibration when precision=16) and FP64 iterative refinement to a
+        backward error <= tol, falling back to the FP64 LU when refinement
+        stalls. Returns (X, MixedInfo). Real matrices only."""
+        if _is_complex(a, b):
+            raise ArgumentError("The mixed-precision solve takes real matrices.")
+        A = _Operand(a, False)
+        B = _Operand(b, False)
+        if A.rows != A.cols or B.rows != A.rows:
+            raise ArgumentError(f"Shapes disagree: A is {A.rows}x{A.cols}, B is {B.rows}x{B.cols}.")
+        n, t = A.rows, B.cols
+        X = out if out is not None else _host_out(n, t, False)
+        Xo = _Operand(X, False) if isinstance(X, DeviceMatrix) else None
+        xptr, xld = (Xo.ptr, Xo.ld) if Xo else (X.ctypes.data if X.size else None, max(1, n))
+        info = L.MixedInfo()
+        self.check(self._lib.tns_dgesv_mixed(self._h, n, t, A.ptr, A.ld, B.ptr, B.ld, xptr, xld, int(precision),
+                                             float(tol), int(max_iterations), C.byref(info)))
+        return X, MixedInfo(info.precision, info.iterations, bool(info.fell_back), info.backward_error,
+                            info.factor_ms, info.refine_ms, info.fallback_ms, info.first_error)
+
     def expm(self, a, out=None):
         """exp(A) and its ExpmDiagnostics (MatrixExponential.cs:143-270)."""
         cplx = _is_complex(a)
@@ -342,6 +363,18 @@ class NormEstimateResult:
     value: float
     iterations: int
     products: int
+
+
+@dataclass(frozen=True)
+class MixedInfo:
+    precision: int
+    iterations: int
+    fell_back: bool
+    backward_error: float
+    factor_ms: float
+    refine_ms: float
+    fallback_ms: float
+    first_error: float
 
 
 @dataclass(frozen=True)

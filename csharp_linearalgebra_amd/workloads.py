@@ -73,6 +73,34 @@ def conditioned(n: int, kappa: float, nrhs: int = 32, seed: int = 0):
     return np.asfortranarray(a), np.asfortranarray(b)
 
 
+def conditioned_device(n: int, kappa: float, nrhs: int = 32, seed: int = 0):
+    """conditioned() built on the device with torch: the same seeded draws,
+    the reflectors applied to diag(sigma) as rank-one updates (O(n^2) each),
+    so n = 8192 takes milliseconds. Returns (A, B) as torch float64 matrices
+    on cuda:0 (logical row/column order; transpose for column-major storage)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    def draws():
+        out = []
+        for _ in range(64):
+            v = rng.standard_normal(n)
+            out.append(v / np.linalg.norm(v))
+        return torch.from_numpy(np.stack(out)).cuda()
+    hu = draws()
+    hv = draws()
+    b = torch.from_numpy(rng.standard_normal((n, nrhs))).cuda()
+    sigma = torch.from_numpy(np.logspace(0.0, -np.log10(kappa), n)).cuda()
+    a = torch.diag(sigma)
+    for k in range(64):  # A <- H_k A, k = 1..64: U = H_64 ... H_1
+        v = hu[k]
+        a -= 2.0 * torch.outer(v, v @ a)
+    for k in range(64):  # A <- A G_k: A = U S G_1 ... G_64 = U S V^T
+        g = hv[k]
+        a -= 2.0 * torch.outer(a @ g, g)
+    return a, b
+
+
 def frequency_sweep(count: int = 512, nmin: int = 64, nmax: int = 1024, seed: int = 0):
     rng = np.random.default_rng(seed)
     orders = rng.integers(nmin, nmax + 1, count)

@@ -220,6 +220,40 @@ TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* us
                           const tns_complex* b, int ldb, double t, double one_norm_bound, tns_complex* f, int ldf,
                           tns_expmv_diag* diag);
 
+/* ---- mixed precision: A X = B by an FP32 LU and FP64 iterative refinement.
+ * No counterpart in the reference: it stands beside the FP64 solve
+ * KernelEntry.FactorLu + SolveLu (KernelEntry.cs:59-95, Lu.cs:189-378),
+ * which it falls back to. precision 32: FP32 factorisation on the FP32
+ * matrix cores; 16: the same with its products on the BF16 matrix cores
+ * (FP32 accumulation) after power-of-two row and column equilibration.
+ * Refinement (residual and update in FP64) runs until the backward error
+ * max_j ||b_j - A x_j||inf / (||A||inf ||x_j||inf + ||b_j||inf) <= tol, at
+ * most max_iter steps; a step that fails to halve it, or an exactly zero
+ * low-precision pivot, switches to the FP64 LU (fell_back = 1). A and B
+ * are not modified. TNS_ESTATE if the FP64 LU also meets a zero pivot. */
+typedef struct {
+    int precision;          /* 32 or 16, as requested */
+    int iterations;         /* refinement steps taken */
+    int fell_back;          /* 1: the result comes from the FP64 LU */
+    double backward_error;  /* of the returned X */
+    double factor_ms;       /* low-precision factorisation, host wall time */
+    double refine_ms;       /* refinement */
+    double fallback_ms;     /* FP64 LU + solve, when taken */
+    double first_error;     /* backward error after the first (plain low-precision) solve */
+} tns_mixed_info;
+TNS_API int tns_dgesv_mixed(tns_context* ctx, int n, int nrhs, const double* a, int lda, const double* b, int ldb,
+                            double* x, int ldx, int precision, double tol, int max_iter, tns_mixed_info* info);
+/* Its low-precision parts, exported so they can be checked on their own
+ * (no counterparts in the reference). precision as above: C := beta*C +
+ * alpha*A*B in FP32 with the products on the FP32 or BF16 matrix cores;
+ * the FP32 LU with partial pivoting (pivots and diagnostics as tns_dgetrf);
+ * and the solve against its factors. */
+TNS_API int tns_sgemm(tns_context* ctx, int precision, int m, int n, int k, float alpha, const float* a, int lda,
+                      const float* b, int ldb, float beta, float* c, int ldc);
+TNS_API int tns_sgetrf(tns_context* ctx, int precision, int n, float* a, int lda, int* pivots, int* singular_column);
+TNS_API int tns_sgetrs(tns_context* ctx, int precision, int n, int nrhs, const float* lu, int ldlu, const int* pivots,
+                       float* b, int ldb);
+
 #ifdef __cplusplus
 }
 #endif
