@@ -103,6 +103,70 @@ def roofline_only():
     ws.dispose()
 
 
+def mixed_leg(ws, n, kappas, nrhs=32):
+    """The mixed-precision workload: A = U diag(sigma) V^T (64 Householder
+    reflectors each side, sigma log-spaced to condition kappa), B n x 32
+    Gaussian, on the device. Times the FP64 LU + solve (getrf + getrs on
+    device copies) and the mixed solve in FP32 and BF16 (factorisation +
+    FP64 refinement to backward error 1e-15, or the FP64 fallback)."""
+    import ctypes as C
+
+    import torch
+
+    from csharp_linearalgebra_amd import DeviceMatrix
+    from csharp_linearalgebra_amd import _lib as L
+    from csharp_linearalgebra_amd import workloads as W
+
+    lib = ws._lib
+    out = []
+    for kappa in kappas:
+        a, b = W.conditioned_device(n, kappa, nrhs, seed=0)
+        at = a.T.contiguous()  # column-major storage
+        bt = b.T.contiguous()
+        dA, dB = DeviceMatrix.from_torch(at), DeviceMatrix.from_torch(bt)
+        x64 = torch.empty_like(bt)
+        f = torch.empty_like(at)
+        piv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        info = L.LuInfo()
+
+        def fp64():
+            f.copy_(at)
+            x64.copy_(bt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ws.check(lib.tns_dgetrf(ws.handle, n, n, f.data_ptr(), n, 0, piv.data_ptr(), C.byref(info)))
+            ws.check(lib.tns_dgetrs(ws.handle, 0, n, nrhs, f.data_ptr(), n, piv.data_ptr(), info.singular_column,
+                                    x64.data_ptr(), n))
+            ws.synchronize()
+            return time.perf_counter() - t0
+
+        fp64()
+        t64 = min(fp64() for _ in range(2))
+        r = bt.T - a @ x64.T
+        den = a.abs().sum(1).max() * x64.T.abs().max(0).values + bt.T.abs().max(0).values
+        berr64 = float((r.abs().max(0).values / den).max())
+        row = {"kappa": kappa, "fp64_ms": round(t64 * 1e3, 2), "fp64_backward_error": berr64}
+        for prec in (32, 16):
+            dX = DeviceMatrix.from_torch(torch.empty_like(bt))
+            ws.solve_mixed(dA, dB, precision=prec, out=dX)
+            best, inf = None, None
+            for _ in range(2):
+                ws.synchronize()
+                t0 = time.perf_counter()
+                _, inf = ws.solve_mixed(dA, dB, precision=prec, out=dX)
+                ws.synchronize()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            key = "fp32" if prec == 32 else "bf16"
+            row[key] = {"ms": round(best * 1e3, 2), "speedup": round(t64 / best, 3),
+                        "backward_error": inf.backward_error, "iterations": inf.iterations,
+                        "fell_back": inf.fell_back, "first_error": inf.first_error}
+        out.append(row)
+        del a, b, at, bt, f, x64
+        torch.cuda.empty_cache()
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
     if not os.path.exists(path):

@@ -12,6 +12,7 @@ from .tensile import (  # noqa: F401
     AllocationLimitError,
     ArgumentError,
     ArgumentOutOfRangeError,
+    BatchedSystems,
     DeviceError,
     DeviceMatrix,
     ExpmDiagnostics,

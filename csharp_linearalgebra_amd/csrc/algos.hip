@@ -34,53 +34,6 @@ void read_doubles(Context* ctx, const double* d, double* h, int count)
 namespace {
 
 // ---------------------------------------------------------------------------
-// System.Random(seed): Knuth's subtractive generator (TAOCP vol. 2, 3.6), in
-// the form .NET keeps for seeded instances; Next(k) = (int)(Sample() * k).
-// ---------------------------------------------------------------------------
-class SeededRandom {
-public:
-    explicit SeededRandom(int seed)
-    {
-        const int mbig = INT_MAX;
-        int subtraction = (seed == INT_MIN) ? INT_MAX : std::abs(seed);
-        int mj = 161803398 - subtraction;
-        s_[55] = mj;
-        int mk = 1, ii = 0;
-        for (int i = 1; i < 55; i++) {
-            if ((ii += 21) >= 55) ii -= 55;
-            s_[ii] = mk;
-            mk = mj - mk;
-            if (mk < 0) mk += mbig;
-            mj = s_[ii];
-        }
-        for (int k = 1; k < 5; k++)
-            for (int i = 1; i < 56; i++) {
-                int n = i + 30;
-                if (n >= 55) n -= 55;
-                s_[i] -= s_[1 + n];
-                if (s_[i] < 0) s_[i] += mbig;
-            }
-        a_ = 0;
-        b_ = 21;
-    }
-    double sample()
-    {
-        if (++a_ >= 56) a_ = 1;
-        if (++b_ >= 56) b_ = 1;
-        int v = s_[a_] - s_[b_];
-        if (v == INT_MAX) v--;
-        if (v < 0) v += INT_MAX;
-        s_[a_] = v;
-        return v * (1.0 / INT_MAX);
-    }
-    int next(int max_value) { return (int)(sample() * max_value); }
-
-private:
-    int s_[56] = {0};
-    int a_ = 0, b_ = 0;
-};
-
-// ---------------------------------------------------------------------------
 // Array.Sort(double[] keys, int[] items): NaNs first, then introspective sort
 // (median-of-three quicksort, insertion sort at <= 16, heapsort past depth
 // 2*(floor(log2 n)+1)). Equal keys keep the order this procedure gives them,
@@ -334,126 +287,175 @@ int ceil_int(double x)
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// NormEstimate.Estimate (NormEstimate.cs:104-274).
+// NormEstimate.Estimate (NormEstimate.cs:104-274), as a resumable run: the
+// host logic advances until it needs a product with the operator, hands
+// out the panel, and resumes with the result. normest() drives one run;
+// a batch of runs can be driven in lockstep (their products are in step:
+// every iteration is one product with A then one with A^H).
 // ---------------------------------------------------------------------------
 template <typename T>
-NormEst normest(Context* ctx, const DevOp<T>& op, int columns, int max_iterations, int seed)
+NormEstRun<T>::NormEstRun(int n, int columns, int max_iterations, int seed) : n_(n), rng_(seed)
 {
-    NormEst res;
-    const int n = op.n;
-    if (n <= 0) return res;
-    const int t = columns < 1 ? 1 : (columns > n ? n : columns);
-    const int itmax = max_iterations > 2 ? max_iterations : 2;
-    if ((long long)n * t > INT_MAX) fail(TNS_ERANGE, "probe panel exceeds a buffer's extent");
-    const size_t extent = (size_t)n * t;
-
-    std::vector<T> x(extent), y(extent), s(extent), s_old(extent), z(extent);
-    std::vector<double> h(n), keys(n);
-    std::vector<int> order(n), scratch(n), current(t);
-    std::vector<char> used(n, 0);
-    DevBuf<T> dx(ctx, extent), dy(ctx, extent);
-
-    SeededRandom rng(seed);
+    if (n <= 0) {
+        want_ = 0;
+        return;
+    }
+    t_ = columns < 1 ? 1 : (columns > n ? n : columns);
+    itmax_ = max_iterations > 2 ? max_iterations : 2;
+    if ((long long)n * t_ > INT_MAX) fail(TNS_ERANGE, "probe panel exceeds a buffer's extent");
+    const size_t extent = (size_t)n * t_;
+    x_.resize(extent);
+    s_.resize(extent);
+    s_old_.resize(extent);
+    out_.resize(extent);
+    h_.resize(n);
+    keys_.resize(n);
+    order_.resize(n);
+    scratch_.resize(n);
+    current_.resize(t_);
+    used_.assign(n, 0);
     const T one = from_h(1.0, T{});
-    for (int i = 0; i < n; i++) x[i] = one;
-    for (int j = 1; j < t; j++)
+    for (int i = 0; i < n; i++) x_[i] = one;
+    for (int j = 1; j < t_; j++)
         for (int attempt = 0; attempt < 8; attempt++) {
-            fill_signs(rng, x.data() + (size_t)j * n, n);
-            if (!parallel_to_any(n, x.data() + (size_t)j * n, x.data(), j)) break;
+            fill_signs(rng_, x_.data() + (size_t)j * n, n);
+            if (!parallel_to_any(n, x_.data() + (size_t)j * n, x_.data(), j)) break;
         }
     const T inv = from_h(1.0 / n, T{});
-    for (size_t i = 0; i < extent; i++) x[i] = s_mul(x[i], inv);
+    for (size_t i = 0; i < extent; i++) x_[i] = s_mul(x_[i], inv);
+    k_ = 1;
+    iterations_ = 1;
+    want_ = 1;
+}
 
-    double est = 0.0, est_old = 0.0;
-    int best_index = -1, iterations = 0, products = 0;
-    for (int k = 1;; k++) {
-        iterations = k;
-        upload(ctx, dx.get(), x);
-        op.apply(t, dx.get(), dy.get());
-        download(ctx, y, dy.get());
-        products += t;
-
+template <typename T>
+void NormEstRun<T>::resume()
+{
+    const int n = n_, t = t_;
+    const size_t extent = (size_t)n * t;
+    if (want_ == 1) {
+        // out_ = A x.
+        const std::vector<T>& y = out_;
+        products_ += t;
         int argmax = 0;
-        est = column_one_norm(y.data(), n);
+        est_ = column_one_norm(y.data(), n);
         for (int j = 1; j < t; j++) {
             double v = column_one_norm(y.data() + (size_t)j * n, n);
-            if (v > est) {
-                est = v;
+            if (v > est_) {
+                est_ = v;
                 argmax = j;
             }
         }
-        if (k >= 2 && (est > est_old || k == 2)) best_index = current[argmax];
-        if (k >= 2 && est <= est_old) {
-            est = est_old;
-            break;
+        if (k_ >= 2 && (est_ > est_old_ || k_ == 2)) best_index_ = current_[argmax];
+        if (k_ >= 2 && est_ <= est_old_) {
+            est_ = est_old_;
+            want_ = 0;
+            return;
         }
-        est_old = est;
-        if (k > itmax) break;
-
-        std::swap(s, s_old);
-        for (size_t i = 0; i < extent; i++) s[i] = sign_h(y[i]);
-
+        est_old_ = est_;
+        if (k_ > itmax_) {
+            want_ = 0;
+            return;
+        }
+        std::swap(s_, s_old_);
+        for (size_t i = 0; i < extent; i++) s_[i] = sign_h(y[i]);
         if (discrete_signs<T>() && t > 1) {
             bool all = true;
             for (int j = 0; j < t && all; j++)
-                if (!parallel_to_any(n, s.data() + (size_t)j * n, s_old.data(), t)) all = false;
-            if (all) break;
+                if (!parallel_to_any(n, s_.data() + (size_t)j * n, s_old_.data(), t)) all = false;
+            if (all) {
+                want_ = 0;
+                return;
+            }
             for (int j = 0; j < t; j++)
                 for (int attempt = 0; attempt < 8; attempt++) {
-                    if (!parallel_to_any(n, s.data() + (size_t)j * n, s.data(), j) &&
-                        !parallel_to_any(n, s.data() + (size_t)j * n, s_old.data(), t))
+                    if (!parallel_to_any(n, s_.data() + (size_t)j * n, s_.data(), j) &&
+                        !parallel_to_any(n, s_.data() + (size_t)j * n, s_old_.data(), t))
                         break;
-                    fill_signs(rng, s.data() + (size_t)j * n, n);
+                    fill_signs(rng_, s_.data() + (size_t)j * n, n);
                 }
         }
-
-        upload(ctx, dx.get(), s);
-        op.adjoint(t, dx.get(), dy.get());
-        download(ctx, z, dy.get());
-        products += t;
-
+        want_ = 2;
+        return;
+    }
+    if (want_ == 2) {
+        // out_ = A^H s.
+        const std::vector<T>& z = out_;
+        products_ += t;
         for (int i = 0; i < n; i++) {
             double best = 0.0;
             for (int j = 0; j < t; j++) best = std::fmax(best, mag_h(z[(size_t)j * n + i]));
-            h[i] = best;
+            h_[i] = best;
         }
-        if (k >= 2 && best_index >= 0) {
+        if (k_ >= 2 && best_index_ >= 0) {
             double largest = 0.0;
-            for (int i = 0; i < n; i++) largest = std::fmax(largest, h[i]);
-            if (largest == h[best_index]) break;
+            for (int i = 0; i < n; i++) largest = std::fmax(largest, h_[i]);
+            if (largest == h_[best_index_]) {
+                want_ = 0;
+                return;
+            }
         }
         for (int i = 0; i < n; i++) {
-            keys[i] = -h[i];
-            order[i] = i;
+            keys_[i] = -h_[i];
+            order_[i] = i;
         }
-        sort_keys_items(n, keys.data(), order.data());
+        sort_keys_items(n, keys_.data(), order_.data());
         if (t > 1) {
             bool all_used = true;
             for (int j = 0; j < t; j++)
-                if (!used[order[j]]) {
+                if (!used_[order_[j]]) {
                     all_used = false;
                     break;
                 }
-            if (all_used) break;
+            if (all_used) {
+                want_ = 0;
+                return;
+            }
             int at = 0;
             for (int i = 0; i < n; i++)
-                if (!used[order[i]]) scratch[at++] = order[i];
+                if (!used_[order_[i]]) scratch_[at++] = order_[i];
             for (int i = 0; i < n; i++)
-                if (used[order[i]]) scratch[at++] = order[i];
-            order.swap(scratch);
+                if (used_[order_[i]]) scratch_[at++] = order_[i];
+            order_.swap(scratch_);
         }
-        for (size_t i = 0; i < extent; i++) x[i] = from_h(0.0, T{});
+        const T one = from_h(1.0, T{});
+        for (size_t i = 0; i < extent; i++) x_[i] = from_h(0.0, T{});
         for (int j = 0; j < t; j++) {
-            int idx = order[j];
-            current[j] = idx;
-            used[idx] = 1;
-            x[(size_t)j * n + idx] = one;
+            int idx = order_[j];
+            current_[j] = idx;
+            used_[idx] = 1;
+            x_[(size_t)j * n + idx] = one;
         }
+        k_++;
+        iterations_ = k_;
+        want_ = 1;
     }
-    res.value = est;
-    res.iterations = iterations;
-    res.products = products;
+}
+
+template <typename T>
+NormEst NormEstRun<T>::result() const
+{
+    NormEst res;
+    res.value = est_;
+    res.iterations = iterations_;
+    res.products = products_;
     return res;
+}
+
+template <typename T>
+NormEst normest(Context* ctx, const DevOp<T>& op, int columns, int max_iterations, int seed)
+{
+    NormEstRun<T> run(op.n, columns, max_iterations, seed);
+    if (run.want() == 0) return NormEst{};
+    DevBuf<T> dx(ctx, run.input().size()), dy(ctx, run.input().size());
+    while (const int w = run.want()) {
+        upload(ctx, dx.get(), run.input());
+        if (w == 1) op.apply(run.columns(), dx.get(), dy.get());
+        else op.adjoint(run.columns(), dx.get(), dy.get());
+        download(ctx, run.output(), dy.get());
+        run.resume();
+    }
+    return run.result();
 }
 
 template <typename T>
@@ -924,6 +926,8 @@ void expmv_op(Context* ctx, const DevOp<T>& op, int nb, const T* b, int ldb, dou
     run<T>(ctx, op, nullptr, 0, nb, b, ldb, t, from_h(0.0, T{}), one_norm_bound, nullptr, f, ldf, diag);
 }
 
+template class NormEstRun<double>;
+template class NormEstRun<zd>;
 template NormEst normest<double>(Context*, const DevOp<double>&, int, int, int);
 template NormEst normest<zd>(Context*, const DevOp<zd>&, int, int, int);
 template DevOp<double> power_operator<double>(Context*, int, const double*, int, int);

@@ -5,6 +5,8 @@
 // runs on the device.
 #pragma once
 
+#include <climits>
+#include <cstdlib>
 #include <functional>
 #include <vector>
 
@@ -26,7 +28,79 @@ struct NormEst {
     int products = 0;
 };
 
-// NormEstimate.Estimate (NormEstimate.cs:104-274).
+// ---------------------------------------------------------------------------
+// System.Random(seed): Knuth's subtractive generator (TAOCP vol. 2, 3.6), in
+// the form .NET keeps for seeded instances; Next(k) = (int)(Sample() * k).
+// ---------------------------------------------------------------------------
+class SeededRandom {
+public:
+    explicit SeededRandom(int seed)
+    {
+        const int mbig = INT_MAX;
+        int subtraction = (seed == INT_MIN) ? INT_MAX : std::abs(seed);
+        int mj = 161803398 - subtraction;
+        s_[55] = mj;
+        int mk = 1, ii = 0;
+        for (int i = 1; i < 55; i++) {
+            if ((ii += 21) >= 55) ii -= 55;
+            s_[ii] = mk;
+            mk = mj - mk;
+            if (mk < 0) mk += mbig;
+            mj = s_[ii];
+        }
+        for (int k = 1; k < 5; k++)
+            for (int i = 1; i < 56; i++) {
+                int n = i + 30;
+                if (n >= 55) n -= 55;
+                s_[i] -= s_[1 + n];
+                if (s_[i] < 0) s_[i] += mbig;
+            }
+        a_ = 0;
+        b_ = 21;
+    }
+    double sample()
+    {
+        if (++a_ >= 56) a_ = 1;
+        if (++b_ >= 56) b_ = 1;
+        int v = s_[a_] - s_[b_];
+        if (v == INT_MAX) v--;
+        if (v < 0) v += INT_MAX;
+        s_[a_] = v;
+        return v * (1.0 / INT_MAX);
+    }
+    int next(int max_value) { return (int)(sample() * max_value); }
+
+private:
+    int s_[56] = {0};
+    int a_ = 0, b_ = 0;
+};
+
+// NormEstimate.Estimate (NormEstimate.cs:104-274) as a resumable run.
+// want(): 0 finished, 1 needs out = A * input, 2 needs out = A^H * input
+// (n x columns panels, column-major, ld n); resume() consumes output().
+template <typename T>
+class NormEstRun {
+public:
+    NormEstRun(int n, int columns, int max_iterations, int seed);
+    int want() const { return want_; }
+    int columns() const { return t_; }
+    const std::vector<T>& input() const { return want_ == 2 ? s_ : x_; }
+    std::vector<T>& output() { return out_; }
+    void resume();
+    NormEst result() const;
+
+private:
+    int n_ = 0, t_ = 1, itmax_ = 2, want_ = 0;
+    int k_ = 0, iterations_ = 0, products_ = 0, best_index_ = -1;
+    double est_ = 0.0, est_old_ = 0.0;
+    std::vector<T> x_, s_, s_old_, out_;
+    std::vector<double> h_, keys_;
+    std::vector<int> order_, scratch_, current_;
+    std::vector<char> used_;
+    SeededRandom rng_;
+};
+
+// NormEstimate.Estimate driven to completion over a device operator.
 template <typename T>
 NormEst normest(Context* ctx, const DevOp<T>& op, int columns, int max_iterations, int seed);
 

@@ -3,17 +3,24 @@
 // host operands through the device, the context and its device pool.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <numeric>
 #include <set>
 
 #include "algos.hpp"
+#include "batch.hpp"
 #include "blas1.hpp"
 #include "mixed.hpp"
 
 struct tns_context {
     tns::Context impl;
     bool disposed = false;
+};
+
+struct tns_batch {
+    tns_context* ctx = nullptr;
+    std::unique_ptr<tns::BatchPlan> plan;
 };
 
 namespace tns {
@@ -847,6 +854,75 @@ TNS_API int tns_sgetrs(tns_context* ctx, int precision, int n, int nrhs, const f
         send_back(cx, vb, b, n, nrhs, ldb);
         tns::sync(cx);
     });
+}
+
+TNS_API int tns_zbatch_create(tns_context* ctx, int count, const int* n, tns_complex* const* a, const int* lda,
+                              tns_complex* const* b, const int* ldb, int nrhs, int tune, tns_batch** out)
+{
+    if (!out) return TNS_EINVAL;
+    *out = nullptr;
+    return guarded(ctx, [&](Context* c) {
+        if (count < 0 || nrhs < 0) fail(TNS_EINVAL, "negative count or right-hand side count");
+        if (tune < 0 || tune > 2) fail(TNS_EINVAL, "tune must be 0, 1 or 2");
+        if (count > 0 && (!n || !a || !lda || !b || !ldb)) fail(TNS_EINVAL, "null batch arrays");
+        for (int i = 0; i < count; i++) {
+            check_shape(n[i], n[i], lda[i], "A_i");
+            check_shape(n[i], nrhs, ldb[i], "B_i");
+            if (n[i] > 0 && (!tns::is_device_pointer(a[i]) || (nrhs > 0 && !tns::is_device_pointer(b[i]))))
+                fail(TNS_EINVAL, "batched systems must be in device memory");
+        }
+        auto batch = std::make_unique<tns_batch>();
+        batch->ctx = ctx;
+        batch->plan = tns::make_batch(c, count, n, reinterpret_cast<zd* const*>(a), lda,
+                                      reinterpret_cast<zd* const*>(b), ldb, nrhs, tune);
+        *out = batch.release();
+    });
+}
+
+TNS_API int tns_zbatch_solve(tns_batch* batch, int* singular_columns)
+{
+    if (!batch) return TNS_EINVAL;
+    return guarded(batch->ctx, [&](Context*) { batch->plan->solve(singular_columns); });
+}
+
+TNS_API int tns_zbatch_rcond(tns_batch* batch, int columns, double* rcond)
+{
+    if (!batch || !rcond) return TNS_EINVAL;
+    return guarded(batch->ctx, [&](Context*) {
+        if (columns < 1) fail(TNS_EINVAL, "columns must be at least 1");
+        batch->plan->rcond(rcond, columns);
+    });
+}
+
+TNS_API int tns_zbatch_pivots(tns_batch* batch, int index, int* pivots)
+{
+    if (!batch || !pivots) return TNS_EINVAL;
+    return guarded(batch->ctx, [&](Context*) { batch->plan->pivots(index, pivots); });
+}
+
+TNS_API int tns_zbatch_info(tns_batch* batch, int* streams, int* gemm_config, double* tuned_ms)
+{
+    if (!batch) return TNS_EINVAL;
+    if (streams) *streams = batch->plan->streams;
+    if (gemm_config) *gemm_config = batch->plan->gemm_cfg;
+    if (tuned_ms) *tuned_ms = batch->plan->tuned_ms;
+    return TNS_OK;
+}
+
+TNS_API int tns_zbatch_destroy(tns_batch* batch)
+{
+    if (!batch) return TNS_EINVAL;
+    if (batch->ctx && !batch->ctx->disposed) {
+        std::lock_guard<std::mutex> lock(batch->ctx->impl.gate);
+        (void)hipSetDevice(batch->ctx->impl.device);
+        (void)hipStreamSynchronize(batch->ctx->impl.stream);
+        batch->plan.reset();
+    } else if (batch->plan) {
+        batch->plan->ctx = nullptr;  // the context's pool is gone: drop, do not return
+        batch->plan.reset();
+    }
+    delete batch;
+    return TNS_OK;
 }
 
 TNS_API int tns_dexpm(tns_context* ctx, int n, const double* a, int lda, double* e, int lde, tns_expm_diag* diag)

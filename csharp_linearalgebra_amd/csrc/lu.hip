@@ -965,14 +965,9 @@ void launch_getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int o
 template <typename T>
 int reg_panel_width(int m)
 {
-    if constexpr (sizeof(T) == 4) {
-        // float: 128 values per thread fit where double fits 64.
-        if (m <= 1024) return 64;
-        if (m <= 2048) return 32;
-        if (m <= 4096) return 16;
-        if (m <= 8192) return 8;
-        return 0;
-    } else if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) != 16) {
+        // The panel's cost per column grows with the values each thread
+        // holds, so float keeps double's shapes (half the registers).
         if (m <= 512) return 64;
         if (m <= 1024) return 32;
         if (m <= 2048) return 16;
@@ -992,13 +987,7 @@ int reg_panel_width(int m)
 template <typename T>
 void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* list, int* info)
 {
-    if constexpr (sizeof(T) == 4) {
-        if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
-        else if (m <= 1024) launch_getf2_reg<T, 64, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
-        else if (m <= 2048) launch_getf2_reg<T, 32, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);
-        else if (m <= 4096) launch_getf2_reg<T, 16, 8>(ctx, m, w, a, lda, piv, off, perm, list, info);
-        else launch_getf2_reg<T, 8, 16>(ctx, m, w, a, lda, piv, off, perm, list, info);
-    } else if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) != 16) {
         if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 1024) launch_getf2_reg<T, 32, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 2048) launch_getf2_reg<T, 16, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);

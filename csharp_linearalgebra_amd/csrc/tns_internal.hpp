@@ -88,6 +88,11 @@ struct Context {
     int device = 0;
     int max_lds = 65536;  // shared memory per workgroup on this device, bytes
     int lp_mode = 0;      // LP_FP32 or LP_BF16: products inside sgetrf/sgetrs
+    // While a graph is being captured, released pool buffers are kept here
+    // (the graph's nodes still use them; the graph's owner returns them).
+    std::vector<std::pair<void*, size_t>>* hold = nullptr;
+    // The batch autotuner's choices: batch shape -> (streams, 4M tile).
+    std::map<std::string, std::pair<int, int>> batch_tuning;
     hipStream_t stream = nullptr;
     Pool pool{size_t(64) << 30};
     // Small fixed device areas for reductions and flags.
@@ -113,7 +118,10 @@ DevBuf<T>::DevBuf(Context* ctx, size_t count) : ctx_(ctx), bytes_(sizeof(T) * (c
 template <typename T>
 void DevBuf<T>::release()
 {
-    if (ctx_ && p_) ctx_->pool.give(p_, bytes_);
+    if (ctx_ && p_) {
+        if (ctx_->hold) ctx_->hold->emplace_back(p_, bytes_);
+        else ctx_->pool.give(p_, bytes_);
+    }
     ctx_ = nullptr;
     p_ = nullptr;
     bytes_ = 0;

@@ -273,6 +273,10 @@ class Workspace:
         return NormEstimateResult(out.value, out.iterations, out.products)
 
     # ---- exponentials ---------------------------------------------------------------------
+    def batched_systems(self, matrices, rhs, tune: int = 1) -> "BatchedSystems":
+        """A BatchedSystems plan over device matrices (see that class)."""
+        return BatchedSystems(self, matrices, rhs, tune)
+
     def solve_mixed(self, a, b, precision: int = 32, tol: float = 1e-15, max_iterations: int = 30, out=None):
         """X with A X = B by a low-precision LU (FP32, or BF16 products with
         equilibration when precision=16) and FP64 iterative refinement to a
@@ -389,6 +393,75 @@ class ExpmvDiagnostics:
     scaling: int
     applications: int
     early_exits: int
+
+
+class BatchedSystems:
+    """Many complex systems Z_i X_i = B_i of uneven order, factored and
+    solved in one launch of a captured hipGraph (the frequency sweep). The
+    matrices must be device memory (DeviceMatrix); solve() overwrites each
+    Z_i with its LU factors and each B_i with X_i. tune: 0 defaults, 1 the
+    workspace's remembered choice for this batch shape (tuning on scratch
+    copies if there is none), 2 tune again."""
+
+    def __init__(self, ws: "Workspace", matrices, rhs, tune: int = 1):
+        if len(matrices) != len(rhs):
+            raise ArgumentError("one right-hand side block per matrix")
+        self._ws = ws
+        self._h = None
+        self.count = len(matrices)
+        for m, r in zip(matrices, rhs):
+            if not isinstance(m, DeviceMatrix) or not isinstance(r, DeviceMatrix):
+                raise ArgumentError("batched systems must be DeviceMatrix operands")
+        nrhs = rhs[0].cols if rhs else 0
+        n = np.array([m.rows for m in matrices], dtype=np.int32)
+        lda = np.array([m.ld for m in matrices], dtype=np.int32)
+        ldb = np.array([r.ld for r in rhs], dtype=np.int32)
+        for m, r in zip(matrices, rhs):
+            if m.rows != m.cols or r.rows != m.rows or r.cols != nrhs:
+                raise ArgumentError("each matrix square, each right-hand side n_i x nrhs")
+        a = (C.c_void_p * max(1, self.count))(*[m.ptr for m in matrices])
+        b = (C.c_void_p * max(1, self.count))(*[r.ptr for r in rhs])
+        self._n = n
+        self._keep = (n, lda, ldb, a, b, matrices, rhs)
+        h = C.c_void_p()
+        ws.check(ws._lib.tns_zbatch_create(ws.handle, self.count, n.ctypes.data, C.cast(a, C.c_void_p),
+                                           lda.ctypes.data, C.cast(b, C.c_void_p), ldb.ctypes.data, nrhs,
+                                           int(tune), C.byref(h)))
+        self._h = h
+
+    def solve(self) -> np.ndarray:
+        """One graph launch; returns each system's first zero pivot or -1."""
+        out = np.zeros(self.count, dtype=np.int32)
+        self._ws.check(self._ws._lib.tns_zbatch_solve(self._h, out.ctypes.data if self.count else None))
+        return out
+
+    def reciprocal_condition(self, columns: int = 2) -> np.ndarray:
+        out = np.zeros(self.count, dtype=np.float64)
+        self._ws.check(self._ws._lib.tns_zbatch_rcond(self._h, int(columns), out.ctypes.data if self.count else
+                                                       C.cast(C.pointer(C.c_double()), C.c_void_p)))
+        return out
+
+    def pivots(self, index: int) -> np.ndarray:
+        out = np.zeros(max(1, int(self._n[index])), dtype=np.int32)
+        self._ws.check(self._ws._lib.tns_zbatch_pivots(self._h, int(index), out.ctypes.data))
+        return out[: int(self._n[index])]
+
+    @property
+    def config(self):
+        s, g, t = C.c_int(), C.c_int(), C.c_double()
+        self._ws.check(self._ws._lib.tns_zbatch_info(self._h, C.byref(s), C.byref(g), C.byref(t)))
+        return s.value, g.value, t.value
+
+    def close(self):
+        if self._h:
+            self._ws._lib.tns_zbatch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LuDecomposition:

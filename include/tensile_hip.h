@@ -254,6 +254,33 @@ TNS_API int tns_sgetrf(tns_context* ctx, int precision, int n, float* a, int lda
 TNS_API int tns_sgetrs(tns_context* ctx, int precision, int n, int nrhs, const float* lu, int ldlu, const int* pivots,
                        float* b, int ldb);
 
+/* ---- batched frequency sweep: many complex systems Z_i X_i = B_i of uneven
+ * order, factored and solved in one launch of a captured hipGraph. No single
+ * counterpart in the reference: per system it is BlockedLu.Factor + Solve
+ * (BlockedLu.cs:45-206) with ||Z_i||_1 taken first (LuDecomposition's
+ * OneNormOfA, LuDecomposition.cs:37-80), and Condition.ReciprocalOne
+ * (NormEstimate.cs:432-466) for rcond. All matrices must be device memory;
+ * the batch records the pointers, so they must stay valid until
+ * tns_zbatch_destroy. tune: 0 defaults, 1 the context's remembered choice for
+ * this batch shape (tuning on scratch copies if there is none), 2 tune
+ * again. tns_zbatch_solve overwrites every A_i with its LU factors and every
+ * B_i with X_i; refill them before solving again. singular_columns (host,
+ * count entries, may be null) receives each system's first zero pivot or
+ * -1. tns_zbatch_rcond (after a solve) writes count reciprocal condition
+ * numbers (host), estimating with `columns` probe columns (2 in the
+ * reference). */
+typedef struct tns_batch tns_batch;
+TNS_API int tns_zbatch_create(tns_context* ctx, int count, const int* n, tns_complex* const* a, const int* lda,
+                              tns_complex* const* b, const int* ldb, int nrhs, int tune, tns_batch** out);
+TNS_API int tns_zbatch_solve(tns_batch* batch, int* singular_columns);
+TNS_API int tns_zbatch_rcond(tns_batch* batch, int columns, double* rcond);
+TNS_API int tns_zbatch_pivots(tns_batch* batch, int index, int* pivots);
+/* What the autotuner chose: streams the capture forks over, the 4M GEMM tile
+ * (-1 the heuristic), and the best graph time it measured (0 if it did not
+ * run). */
+TNS_API int tns_zbatch_info(tns_batch* batch, int* streams, int* gemm_config, double* tuned_ms);
+TNS_API int tns_zbatch_destroy(tns_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
