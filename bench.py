@@ -167,6 +167,60 @@ def mixed_leg(ws, n, kappas, nrhs=32):
     return out
 
 
+def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
+    """The frequency sweep: `count` complex systems Z(w) = R + i w L of order
+    n ~ U{nmin..nmax} with 4 right-hand sides, factored and solved in one
+    graph launch (autotuned), then rcond of every system."""
+    import torch
+
+    from csharp_linearalgebra_amd import DeviceMatrix
+    from csharp_linearalgebra_amd import workloads as W
+
+    mats, rhs = W.frequency_sweep_device(count, nmin, nmax, nrhs, seed=0)
+    work_a = [m.clone() for m in mats]
+    work_b = [b.clone() for b in rhs]
+    dm = [DeviceMatrix.from_torch(t) for t in work_a]
+    db = [DeviceMatrix.from_torch(t) for t in work_b]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = ws.batched_systems(dm, db, tune=1)
+    tune_s = time.perf_counter() - t0
+    best = None
+    for _ in range(3):
+        for w, m in zip(work_a, mats):
+            w.copy_(m)
+        for w, b in zip(work_b, rhs):
+            w.copy_(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sing = plan.solve()
+        ws.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    t0 = time.perf_counter()
+    rc = plan.reciprocal_condition()
+    rc_s = time.perf_counter() - t0
+    streams, cfg, tuned_ms = plan.config
+    orders = [m.shape[0] for m in mats]
+    # Residual check of every system (host): max backward error.
+    worst = 0.0
+    for m, b, x in zip(mats, rhs, work_b):
+        a = m.T
+        xx = x.T
+        r = b.T - a @ xx
+        den = a.abs().sum(1).max() * xx.abs().max() + b.abs().max()
+        worst = max(worst, float(r.abs().max() / den))
+    plan.close()
+    return {"workload": "%d complex128 systems, n ~ U{%d..%d}, %d RHS, one graph launch" % (count, nmin, nmax, nrhs),
+            "solve_ms": round(best * 1e3, 2), "solves_per_s": round(count / best, 1),
+            "gflops": round(W.sweep_flops(orders, nrhs) / best / 1e9, 1),
+            "rcond_ms": round(rc_s * 1e3, 2), "solves_plus_rcond_per_s": round(count / (best + rc_s), 1),
+            "tuned": {"streams": streams, "zgemm_config": cfg, "graph_ms": round(tuned_ms, 2),
+                      "tuning_s": round(tune_s, 2)},
+            "singular": int((sing >= 0).sum()), "max_backward_error": worst,
+            "rcond_range": [float(rc.min()), float(rc.max())]}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
     if not os.path.exists(path):
@@ -289,6 +343,12 @@ def main():
                      "identity_pivots": bool(np.array_equal(lu.pivots, np.arange(1024)))},
         "dgemm_n8192": {"tflops": round(dg_tflops, 2), "frac": round(dg_tflops / FP64_PEAK_TFLOPS, 4)},
     }
+
+    # ---- mixed-precision LU + FP64 refinement vs FP64 LU, n = 4096 and 8192 -------
+    result["mixed_lu"] = {str(nm): mixed_leg(ws, nm, [1e2, 1e6, 1e10]) for nm in (4096, 8192)}
+
+    # ---- batched frequency sweep ------------------------------------------------
+    result["batched_sweep"] = sweep_leg(ws)
 
     if not args.no_cpu:
         import oracle as O

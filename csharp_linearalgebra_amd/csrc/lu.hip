@@ -46,12 +46,13 @@ constexpr double safe_min()
     return sizeof(T) == 4 ? 1.1754943508222875e-38 : 2.2250738585072014e-308;
 }
 
+// Larger key wins; equal keys go to the lower row. Written with bitwise
+// operators so it compiles to compares and selects, not branches.
 __device__ __forceinline__ void best_combine(double& bv, int& bi, double v, int i)
 {
-    if (v > bv || (v == bv && i < bi)) {
-        bv = v;
-        bi = i;
-    }
+    const bool take = (v > bv) | ((v == bv) & (i < bi));
+    bv = take ? v : bv;
+    bi = take ? i : bi;
 }
 
 __device__ __forceinline__ void wave_best(double& bv, int& bi)
@@ -251,7 +252,8 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const bool ok = !((done >> r) & 1u);
-            best_combine(bv, bi, ok ? pivot_key(nx[r], pos[r], j) : -2.0, ok ? pos[r] : INT_MAX);
+            const double key = pivot_key(nx[r], pos[r], j);
+            best_combine(bv, bi, ok ? key : -2.0, ok ? pos[r] : INT_MAX);
         }
         wave_best_dpp(bv, bi);
         // Every wave publishes its best row with its vote, so one barrier
@@ -275,11 +277,10 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
         for (int q = 1; q < nw; ++q) {
             const double qv = red_v[buf][q];
             const int qi = red_i[buf][q];
-            if (qv > bv || (qv == bv && qi < bi)) {
-                bv = qv;
-                bi = qi;
-                wq = q;
-            }
+            const bool take = (qv > bv) | ((qv == bv) & (qi < bi));
+            bv = take ? qv : bv;
+            bi = take ? qi : bi;
+            wq = take ? q : wq;
         }
         const int p = bi;
         const T* prow = cand[buf][wq];
@@ -296,12 +297,10 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const bool live = !((done >> r) & 1u);
-            if (live && pos[r] == pe) {
-                done |= 1u << r;
-                pos[r] = j;
-            } else if (live && pos[r] == j) {
-                pos[r] = pe;
-            }
+            const bool is_p = live & (pos[r] == pe);
+            const bool is_j = live & (pos[r] == j) & !is_p;
+            done |= is_p ? (1u << r) : 0u;
+            pos[r] = is_p ? j : (is_j ? pe : pos[r]);
         }
         // Multipliers, in nx: x * (1/pivot) when |pivot| >= the safe
         // minimum, x / pivot below it, x itself for a zero pivot. |pivot|

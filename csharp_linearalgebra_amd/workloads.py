@@ -114,6 +114,34 @@ def frequency_sweep(count: int = 512, nmin: int = 64, nmax: int = 1024, seed: in
     return mats
 
 
+def frequency_sweep_device(count: int = 512, nmin: int = 64, nmax: int = 1024, nrhs: int = 4, seed: int = 0):
+    """frequency_sweep() built on the device for the benchmark: the orders,
+    frequencies and R diagonals from numpy's seeded generator, G and the
+    right-hand sides (complex uniform(-1,1)) from a seeded torch generator
+    on cuda:0. Returns (mats, rhs): lists of complex128 torch tensors in
+    column-major storage (each tensor holds the transpose, shape (cols,
+    rows))."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    orders = rng.integers(nmin, nmax + 1, count)
+    omegas = np.logspace(1.0, 6.0, count)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    mats, rhs = [], []
+    for n, w in zip(orders, omegas):
+        n = int(n)
+        r = torch.from_numpy(rng.uniform(0.01, 1.0, n)).cuda()
+        g = torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64)
+        l = g @ g.T / n + torch.eye(n, device="cuda", dtype=torch.float64)
+        z = torch.complex(torch.diag(r), w * l)  # symmetric: its transpose is itself
+        mats.append(z.contiguous())
+        re = torch.rand((nrhs, n), generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        im = torch.rand((nrhs, n), generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        rhs.append(torch.complex(re, im).contiguous())
+    return mats, rhs
+
+
 def adjoint_problem(n: int, complex_: bool, target: float = 20.0, seed: int = 0):
     rng = np.random.default_rng(seed)
     if complex_:
@@ -137,6 +165,11 @@ def expm_flops(n: int, squarings: int, complex_: bool = True) -> float:
     lu = (8.0 / 3.0 if complex_ else 2.0 / 3.0) * n ** 3
     solve = (8.0 if complex_ else 2.0) * n ** 3
     return (6 + squarings) * mul * n ** 3 + lu + solve
+
+
+def sweep_flops(orders, nrhs: int = 4) -> float:
+    """Complex LU (8/3 n^3) and the solve (8 n^2 nrhs) of every system."""
+    return float(sum((8.0 / 3.0) * n ** 3 + 8.0 * n * n * nrhs for n in orders))
 
 
 def getrf_flops(n: int) -> float:
