@@ -17,6 +17,7 @@ from .tensile import (  # noqa: F401
     DeviceMatrix,
     ExpmDiagnostics,
     ExpmvDiagnostics,
+    ExpmvFunction,
     ILinearOperator,
     InvalidOperationError,
     LuDecomposition,

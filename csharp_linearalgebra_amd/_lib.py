@@ -82,6 +82,8 @@ SIGNATURES = {
     "tns_sgemm": (_I, [_P, _I, _I, _I, _I, C.c_float, _P, _I, _P, _I, C.c_float, _P, _I]),
     "tns_sgetrf": (_I, [_P, _I, _I, _P, _I, _P, C.POINTER(_I)]),
     "tns_sgetrs": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _I]),
+    "tns_dexpmv_grad": (_I, [_P, _I, _I, _P, _I, _P, _I, _D, _P, _I, _P, _I, _P, _I, C.POINTER(ExpmvDiag)]),
+    "tns_zexpmv_grad": (_I, [_P, _I, _I, _P, _I, _P, _I, _D, _P, _I, _P, _I, _P, _I, C.POINTER(ExpmvDiag)]),
     "tns_zbatch_create": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _I, C.POINTER(C.c_void_p)]),
     "tns_zbatch_solve": (_I, [_P, _P]),
     "tns_zbatch_rcond": (_I, [_P, _I, _P]),

@@ -926,6 +926,244 @@ void expmv_op(Context* ctx, const DevOp<T>& op, int nb, const T* b, int ldb, dou
     run<T>(ctx, op, nullptr, 0, nb, b, ldb, t, from_h(0.0, T{}), one_norm_bound, nullptr, f, ldf, diag);
 }
 
+// ---------------------------------------------------------------------------
+// Gradient of a loss through F = exp(tA) B computed as above: given
+// G = dL/dF, the gradients dL/dA and dL/dB of the computed F (the adjoint of
+// the algorithm, so FD of expmv agrees to rounding). The forward pass keeps
+// only each stage's input (s checkpoints) and how many Taylor terms the
+// stage used; the backward pass recomputes a stage's terms from its
+// checkpoint and runs the recurrences backwards:
+//   stage i: Z_0 = F_{i-1}, Z_{j+1} = c_j At Z_j (c_j = t/(s(j+1))),
+//            F_i = eta * sum_j Z_j, At = A - mu I, eta = exp(mu t / s);
+//   backward: Pbar = conj(eta) Gbar, W_J = Pbar,
+//            W_j = Pbar + c_j At^H W_{j+1},  dL/dAt += c_j W_{j+1} Z_j^H,
+//            dL/dmu += (t/s) conj(eta <Gbar, P>), Gbar <- W_0;
+//   finally dL/dB = Gbar, dL/dA = dL/dAt + (dL/dmu - tr(dL/dAt)) I / n
+//   (mu = tr(A)/n). Inner products are <X, Y> = sum conj(x) y; for complex
+//   data these are the gradients g with dL = Re <g, dA>.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+__global__ void axpy_panel_kernel(size_t count, const T* __restrict__ x, double c, const T* __restrict__ y,
+                                  T* __restrict__ out)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        out[e] = s_add(x[e], s_scale(y[e], c));
+}
+
+template <typename T>
+__global__ void scaled_copy_kernel(size_t count, const T* __restrict__ x, double c, T* __restrict__ out)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        out[e] = s_scale(x[e], c);
+}
+
+template <typename T>
+__global__ void add_panel_kernel(size_t count, const T* __restrict__ x, T* __restrict__ acc)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        acc[e] = s_add(acc[e], x[e]);
+}
+
+// dst (cols x rows, ld ldd) = src^H (src rows x cols, ld lds).
+template <typename T>
+__global__ void adjoint_copy_kernel(int rows, int cols, const T* __restrict__ src, int lds, T* __restrict__ dst,
+                                    int ldd)
+{
+    const size_t total = (size_t)rows * cols;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % rows), j = (int)(e / rows);
+        dst[j + (size_t)i * ldd] = s_conj(src[i + (size_t)j * lds]);
+    }
+}
+
+// out[0] = sum conj(x) y over count elements (one workgroup, fixed order).
+template <typename T>
+__global__ void __launch_bounds__(256) dot_kernel(size_t count, const T* __restrict__ x, const T* __restrict__ y,
+                                                  T* __restrict__ out)
+{
+    __shared__ T part[256];
+    T acc = s_zero(T{});
+    for (size_t e = threadIdx.x; e < count; e += 256) acc = s_add(acc, s_mul(s_conj(x[e]), y[e]));
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T sum = s_zero(T{});
+        for (int q = 0; q < 256; q++) sum = s_add(sum, part[q]);
+        out[0] = sum;
+    }
+}
+
+int grid_of(size_t count)
+{
+    const size_t b = (count + 255) / 256;
+    return (int)(b < 4096 ? b : 4096);
+}
+
+template <typename T>
+T read_scalar(Context* ctx, const T* d)
+{
+    T h{};
+    TNS_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+}
+
+inline double conj_h(double x) { return x; }
+inline zd conj_h(zd x) { return make_zd(x.x, -x.y); }
+
+template <typename T>
+void dscale_or_zero(Context* ctx, int n, T* p)
+{
+    if constexpr (sizeof(T) == 8) dscale_c(ctx, n, n, 0.0, (double*)p, n);
+    else zscale_c(ctx, n, n, make_zd(0.0, 0.0), (zd*)p, n);
+}
+
+}  // namespace
+
+template <typename T>
+void expmv_grad(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb, double t, const T* g, int ldg,
+                T* ga, int ldga, T* gb, int ldgb, ExpmvDiag* diag)
+{
+    if (diag) *diag = ExpmvDiag{};
+    if (n == 0) return;
+    const size_t extent = (size_t)n * nb;
+    // ---- the forward set-up of expmv: shift, parameters ----------------------------
+    Mat<T> shifted(ctx, n);
+    copy_t<T>(ctx, n, n, a, lda, shifted.p(), n);
+    trace<T>(ctx, n, a, lda, (T*)ctx->d_scal);
+    const T tr = read_scalar<T>(ctx, (const T*)ctx->d_scal);
+    T mu;
+    if constexpr (sizeof(T) == 8) mu = tr / n;
+    else mu = make_zd(tr.x / n, tr.y / n);
+    shift_diag<T>(ctx, n, shifted.p(), n, neg_h(mu));
+    const double anorm1 = one_norm<T>(ctx, n, shifted.p());
+    const T* sp = shifted.p();
+    double dcache[kMaxPower + 2];
+    for (double& v : dcache) v = NAN;
+    auto dpow = [&](int p) {
+        if (std::isnan(dcache[p]))
+            dcache[p] = std::pow(normest<T>(ctx, power_operator<T>(ctx, n, sp, n, p), kEll, 5, 20260915).value,
+                                 1.0 / p);
+        return dcache[p];
+    };
+    std::function<double(int)> alpha = [&](int p) { return std::fmax(dpow(p), dpow(p + 1)); };
+    DevOp<T> op = power_operator<T>(ctx, n, sp, n, 1);
+    const double scale = std::fabs(t);
+    // With A - mu I = 0 (or t = 0) expmv takes no Taylor term; the
+    // gradient keeps the first one, which adds nothing to F there but
+    // carries the derivative (t conj(eta) G B^H) the zero-term form drops.
+    int m = 1, s = 1;
+    if (nb > 0 && scale * anorm1 != 0.0) select_parameters(scale * anorm1, &alpha, scale, nb, m, s);
+    if (diag) {
+        diag->degree = m;
+        diag->scaling = s;
+    }
+    T eta;
+    if constexpr (sizeof(T) == 8) eta = exp_scalar<T>((mu * t) / (double)s);
+    else eta = exp_scalar<T>(s_div(s_scale(mu, t), make_zd((double)s, 0.0)));
+    const bool unit_eta = eq_h(eta, from_h(1.0, T{}));
+    const T zero = from_h(0.0, T{});
+
+    // dL/dAt accumulates in a packed n x n buffer.
+    Mat<T> gat(ctx, n);
+    dscale_or_zero<T>(ctx, n, gat.p());
+    if (nb == 0) {
+        copy_t<T>(ctx, n, n, gat.p(), n, ga, ldga);
+        return;
+    }
+
+    // ---- forward with checkpoints (expmv's loop, unfused) -------------------------
+    DevBuf<T> ckpt(ctx, extent * (size_t)s), f(ctx, extent), work(ctx, extent), next(ctx, extent);
+    std::vector<int> terms(s, 0);
+    copy_t<T>(ctx, n, nb, b, ldb, f.get(), n);
+    copy_t<T>(ctx, n, nb, b, ldb, work.get(), n);
+    double* d_norms = ctx->d_scal + 8;
+    double hn[2];
+    for (int i = 0; i < s; i++) {
+        copy_t<T>(ctx, n, nb, f.get(), n, ckpt.get() + extent * i, n);
+        zero_scalars(ctx, d_norms + 2, 1);
+        panel_infnorm<T>(ctx, n, nb, work.get(), d_norms + 2);
+        double c1 = read_double(ctx, d_norms + 2);
+        for (int j = 0; j < m; j++) {
+            const double coef = t / (s * (double)(j + 1));
+            zero_scalars(ctx, d_norms, 2);
+            op.apply(nb, work.get(), next.get());
+            taylor_step<T>(ctx, n, nb, 1, next.get(), coef, work.get(), f.get(), d_norms);
+            read_doubles(ctx, d_norms, hn, 2);
+            terms[i] = j + 1;
+            if (diag) diag->applications++;
+            if (c1 + hn[0] <= kUnitRoundoff * hn[1]) {
+                if (diag) diag->early_exits++;
+                break;
+            }
+            c1 = hn[0];
+        }
+        if (!unit_eta) mul_scalar<T>(ctx, extent, f.get(), eta);
+        copy_t<T>(ctx, n, nb, f.get(), n, work.get(), n);
+    }
+
+    // ---- backward -------------------------------------------------------------------
+    DevBuf<T> gbar(ctx, extent), pbar(ctx, extent), p(ctx, extent), tmp(ctx, extent);
+    DevBuf<T> zs(ctx, extent * (size_t)(m + 1)), ws(ctx, extent * (size_t)(m > 0 ? m : 1));
+    DevBuf<T> zh(ctx, extent * (size_t)(m > 0 ? m : 1));
+    DevBuf<T> d_dot(ctx, 1);
+    copy_t<T>(ctx, n, nb, g, ldg, gbar.get(), n);
+    T mu_g = zero;
+    const int gx = grid_of(extent);
+    for (int i = s - 1; i >= 0; i--) {
+        const int J = terms[i];
+        // Recompute Z_0..Z_J and P = sum Z_j (in the forward's order).
+        copy_t<T>(ctx, n, nb, ckpt.get() + extent * i, n, zs.get(), n);
+        copy_t<T>(ctx, n, nb, zs.get(), n, p.get(), n);
+        for (int j = 0; j < J; j++) {
+            op.apply(nb, zs.get() + extent * j, next.get());
+            hipLaunchKernelGGL((scaled_copy_kernel<T>), dim3(gx), dim3(256), 0, ctx->stream, extent, next.get(),
+                               t / (s * (double)(j + 1)), zs.get() + extent * (j + 1));
+            hipLaunchKernelGGL((add_panel_kernel<T>), dim3(gx), dim3(256), 0, ctx->stream, extent,
+                               zs.get() + extent * (j + 1), p.get());
+        }
+        // dL/dmu from eta: (t/s) conj(eta <Gbar, P>).
+        hipLaunchKernelGGL((dot_kernel<T>), dim3(1), dim3(256), 0, ctx->stream, extent, gbar.get(), p.get(), d_dot.get());
+        const T dp = read_scalar<T>(ctx, d_dot.get());
+        mu_g = s_add(mu_g, s_scale(conj_h(s_mul(eta, dp)), t / s));
+        // Pbar = conj(eta) Gbar.
+        copy_t<T>(ctx, n, nb, gbar.get(), n, pbar.get(), n);
+        if (!unit_eta) mul_scalar<T>(ctx, extent, pbar.get(), conj_h(eta));
+        // W_J = Pbar; W_j = Pbar + c_j At^H W_{j+1}; stack c_j W_{j+1}.
+        copy_t<T>(ctx, n, nb, pbar.get(), n, tmp.get(), n);  // tmp = W_{j+1}
+        for (int j = J - 1; j >= 0; j--) {
+            const double cj = t / (s * (double)(j + 1));
+            hipLaunchKernelGGL((scaled_copy_kernel<T>), dim3(gx), dim3(256), 0, ctx->stream, extent, tmp.get(), cj,
+                               ws.get() + extent * j);
+            op.adjoint(nb, tmp.get(), next.get());
+            hipLaunchKernelGGL((axpy_panel_kernel<T>), dim3(gx), dim3(256), 0, ctx->stream, extent, pbar.get(), cj,
+                               next.get(), tmp.get());
+        }
+        // dL/dAt += [c_0 W_1 .. c_{J-1} W_J] [Z_0 .. Z_{J-1}]^H.
+        if (J > 0) {
+            const int k = J * nb;
+            hipLaunchKernelGGL((adjoint_copy_kernel<T>), dim3(grid_of((size_t)n * k)), dim3(256), 0, ctx->stream, n, k,
+                               zs.get(), n, zh.get(), k);
+            gemm_t<T>(ctx, n, n, k, 1.0, ws.get(), n, zh.get(), k, 1.0, gat.p(), n);
+        }
+        copy_t<T>(ctx, n, nb, tmp.get(), n, gbar.get(), n);  // Gbar <- W_0
+    }
+    TNS_HIP(hipGetLastError());
+    copy_t<T>(ctx, n, nb, gbar.get(), n, gb, ldgb);
+    // mu = tr(A)/n: dL/dA = dL/dAt + (dL/dmu - tr(dL/dAt)) I / n.
+    trace<T>(ctx, n, gat.p(), n, (T*)ctx->d_scal);
+    const T trg = read_scalar<T>(ctx, (const T*)ctx->d_scal);
+    const T shift = s_scale(s_sub(mu_g, trg), 1.0 / n);
+    shift_diag<T>(ctx, n, gat.p(), n, shift);
+    copy_t<T>(ctx, n, n, gat.p(), n, ga, ldga);
+}
+
+template void expmv_grad<double>(Context*, int, int, const double*, int, const double*, int, double, const double*,
+                                 int, double*, int, double*, int, ExpmvDiag*);
+template void expmv_grad<zd>(Context*, int, int, const zd*, int, const zd*, int, double, const zd*, int, zd*, int, zd*,
+                             int, ExpmvDiag*);
 template class NormEstRun<double>;
 template class NormEstRun<zd>;
 template NormEst normest<double>(Context*, const DevOp<double>&, int, int, int);

@@ -148,6 +148,13 @@ template <typename T>
 void expmv_op(Context* ctx, const DevOp<T>& op, int nb, const T* b, int ldb, double t, double one_norm_bound, T* f,
               int ldf, ExpmvDiag* diag);
 
+// Gradients through F = exp(tA) B (computed as expmv computes it) for an
+// upstream gradient G = dL/dF (n x nb): dL/dA into ga (n x n), dL/dB into gb
+// (n x nb). All device pointers.
+template <typename T>
+void expmv_grad(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb, double t, const T* g, int ldg,
+                T* ga, int ldga, T* gb, int ldgb, ExpmvDiag* diag);
+
 // Read device scalars back (waits for the stream).
 double read_double(Context* ctx, const double* d);
 void read_doubles(Context* ctx, const double* d, double* h, int count);

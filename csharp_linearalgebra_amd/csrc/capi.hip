@@ -2,6 +2,7 @@
 // Tensile's seam (Operand.cs:80-96, KernelEntry.cs:246-269), staging of
 // host operands through the device, the context and its device pool.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -389,6 +390,32 @@ void mixed_body(Context* ctx, int n, int nrhs, const double* a, int lda, const d
     if (info)
         *info = tns_mixed_info{mi.precision, mi.iterations, mi.fell_back, mi.backward_error, mi.factor_ms,
                                mi.refine_ms, mi.fallback_ms, mi.first_error};
+}
+
+template <typename T>
+void expmv_grad_body(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb, double t, const T* g,
+                     int ldg, T* ga, int ldga, T* gb, int ldgb, tns_expmv_diag* diag)
+{
+    if (n < 0 || nb < 0) fail(TNS_EINVAL, "negative dimension");
+    if (!std::isfinite(t)) fail(TNS_ERANGE, "t must be finite");
+    check_operand(a, n, n, lda, "A");
+    check_operand(b, n, nb, ldb, "B");
+    check_operand(g, n, nb, ldg, "G");
+    check_operand(ga, n, n, ldga, "dA");
+    check_operand(gb, n, nb, ldgb, "dB");
+    tns::ExpmvDiag d;
+    if (n > 0) {
+        View<T> va = bring(ctx, a, n, n, lda, true);
+        View<T> vb = bring(ctx, b, n, nb, ldb, true);
+        View<T> vg = bring(ctx, g, n, nb, ldg, true);
+        View<T> vga = bring(ctx, ga, n, n, ldga, false);
+        View<T> vgb = bring(ctx, gb, n, nb, ldgb, false);
+        tns::expmv_grad<T>(ctx, n, nb, va.d, va.ld, vb.d, vb.ld, t, vg.d, vg.ld, vga.d, vga.ld, vgb.d, vgb.ld, &d);
+        send_back(ctx, vga, ga, n, n, ldga);
+        send_back(ctx, vgb, gb, n, nb, ldgb);
+        tns::sync(ctx);
+    }
+    if (diag) *diag = tns_expmv_diag{d.degree, d.scaling, d.applications, d.early_exits};
 }
 
 template <typename T>
@@ -923,6 +950,25 @@ TNS_API int tns_zbatch_destroy(tns_batch* batch)
     }
     delete batch;
     return TNS_OK;
+}
+
+TNS_API int tns_dexpmv_grad(tns_context* ctx, int n, int nb, const double* a, int lda, const double* b, int ldb,
+                            double t, const double* g, int ldg, double* ga, int ldga, double* gb, int ldgb,
+                            tns_expmv_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) {
+        expmv_grad_body<double>(c, n, nb, a, lda, b, ldb, t, g, ldg, ga, ldga, gb, ldgb, diag);
+    });
+}
+
+TNS_API int tns_zexpmv_grad(tns_context* ctx, int n, int nb, const tns_complex* a, int lda, const tns_complex* b,
+                            int ldb, double t, const tns_complex* g, int ldg, tns_complex* ga, int ldga,
+                            tns_complex* gb, int ldgb, tns_expmv_diag* diag)
+{
+    return guarded(ctx, [&](Context* c) {
+        expmv_grad_body<zd>(c, n, nb, (const zd*)a, lda, (const zd*)b, ldb, t, (const zd*)g, ldg, (zd*)ga, ldga,
+                            (zd*)gb, ldgb, diag);
+    });
 }
 
 TNS_API int tns_dexpm(tns_context* ctx, int n, const double* a, int lda, double* e, int lde, tns_expm_diag* diag)

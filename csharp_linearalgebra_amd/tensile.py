@@ -329,6 +329,29 @@ class Workspace:
         self.check(fn(self._h, A.rows, B.cols, A.ptr, A.ld, B.ptr, B.ld, float(t), fptr, fld, C.byref(d)))
         return F, ExpmvDiagnostics(d.degree, d.scaling, d.applications, d.early_exits)
 
+    def expmv_backward(self, a, b, t, grad_f, out_a=None, out_b=None):
+        """The gradients (dL/dA, dL/dB) through F = exp(tA) B for an
+        upstream gradient grad_f = dL/dF, by the adjoint of the Taylor
+        algorithm with recomputed stages (see ExpmvFunction)."""
+        cplx = _is_complex(a, b, grad_f)
+        A, B, G = _Operand(a, cplx), _Operand(b, cplx), _Operand(grad_f, cplx)
+        n, nb = A.rows, B.cols
+        if A.rows != A.cols or B.rows != n or G.rows != n or G.cols != nb:
+            raise ArgumentError("Shapes disagree: A n x n, B and dL/dF n x nb.")
+        GA = out_a if out_a is not None else _host_out(n, n, cplx)
+        GB = out_b if out_b is not None else _host_out(n, nb, cplx)
+        def ptr(x, rows):
+            if isinstance(x, DeviceMatrix):
+                o = _Operand(x, cplx)
+                return o.ptr, o.ld
+            return (x.ctypes.data if x.size else None), max(1, rows)
+        pa, la = ptr(GA, n)
+        pb, lb = ptr(GB, n)
+        d = L.ExpmvDiag()
+        fn = self._lib.tns_zexpmv_grad if cplx else self._lib.tns_dexpmv_grad
+        self.check(fn(self._h, n, nb, A.ptr, A.ld, B.ptr, B.ld, float(t), G.ptr, G.ld, pa, la, pb, lb, C.byref(d)))
+        return GA, GB
+
     def expmv_operator(self, op: "ILinearOperator", b, t: float, one_norm_bound: float, complex_=None):
         """The matrix-free action over a Python ILinearOperator
         (MatrixExponentialAction.cs:203-230)."""
@@ -393,6 +416,29 @@ class ExpmvDiagnostics:
     scaling: int
     applications: int
     early_exits: int
+
+
+class ExpmvFunction:
+    """exp(tA) B as a differentiable function in the style of an autograd
+    Function: forward(a, b, t) returns F and keeps its inputs; backward(grad_f)
+    returns (dL/dA, dL/dB) for grad_f = dL/dF. For complex data the gradients
+    g satisfy dL = Re tr(g^H dA) (the real-loss convention)."""
+
+    def __init__(self, workspace: "Workspace" = None):
+        self._ws = workspace or Workspace.shared()
+        self._saved = None
+
+    def forward(self, a, b, t: float = 1.0):
+        f, diag = self._ws.expmv(a, b, t)
+        self._saved = (a, b, float(t))
+        self.diagnostics = diag
+        return f
+
+    def backward(self, grad_f):
+        if self._saved is None:
+            raise InvalidOperationError("backward() before forward()")
+        a, b, t = self._saved
+        return self._ws.expmv_backward(a, b, t, grad_f)
 
 
 class BatchedSystems:

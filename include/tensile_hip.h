@@ -254,6 +254,23 @@ TNS_API int tns_sgetrf(tns_context* ctx, int precision, int n, float* a, int lda
 TNS_API int tns_sgetrs(tns_context* ctx, int precision, int n, int nrhs, const float* lu, int ldlu, const int* pivots,
                        float* b, int ldb);
 
+/* ---- gradients through exp(tA)B: the adjoint of tns_{d,z}expmv. No
+ * counterpart in the reference; it differentiates MatrixExponentialAction's
+ * dense path (MatrixExponentialAction.cs:102-178, the complex trace shift of
+ * MatrixExponentialAction.Complex.cs:53-63). Given G = dL/dF for F = exp(tA)B
+ * (n x nb), writes dL/dA (n x n) to ga and dL/dB (n x nb) to gb: the
+ * gradients of the computed F, with the same (m, s) and Taylor truncation
+ * (for complex data, dL = Re tr(ga^H dA) + Re tr(gb^H dB)). The Taylor
+ * stages are recomputed from per-stage checkpoints, not stored.
+ * Mathematically dL/dB = exp(tA^H) G and dL/dA = t L(tA^H, G B^H), L the
+ * Frechet derivative of exp. diag reports the forward's parameters. */
+TNS_API int tns_dexpmv_grad(tns_context* ctx, int n, int nb, const double* a, int lda, const double* b, int ldb,
+                            double t, const double* g, int ldg, double* ga, int ldga, double* gb, int ldgb,
+                            tns_expmv_diag* diag);
+TNS_API int tns_zexpmv_grad(tns_context* ctx, int n, int nb, const tns_complex* a, int lda, const tns_complex* b,
+                            int ldb, double t, const tns_complex* g, int ldg, tns_complex* ga, int ldga,
+                            tns_complex* gb, int ldgb, tns_expmv_diag* diag);
+
 /* ---- batched frequency sweep: many complex systems Z_i X_i = B_i of uneven
  * order, factored and solved in one launch of a captured hipGraph. No single
  * counterpart in the reference: per system it is BlockedLu.Factor + Solve
