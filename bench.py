@@ -221,6 +221,51 @@ def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
             "rcond_range": [float(rc.min()), float(rc.max())]}
 
 
+def grad_leg(ws, sizes=(256, 1024)):
+    """Gradients of L = ||C exp(tA) B - Y||_F^2 (t = 1, ||A||_1 = 20, B n x 4,
+    C 8 x n, Y 8 x 4) for real and complex A: forward Expmv + the adjoint
+    (dL/dA, dL/dB), with the error against the 2n x 2n block exponential
+    (computed by this library's Expm, on the device)."""
+    import torch
+
+    from csharp_linearalgebra_amd import workloads as W
+
+    out = []
+    for cplx in (False, True):
+        for n in sizes:
+            a, b, c, y = W.adjoint_problem(n, cplx, target=20.0, seed=0)
+            dA, dB = to_device(a), to_device(b)
+            f, _ = ws.expmv(a, b, 1.0)
+            g = 2.0 * c.conj().T @ (c @ f - y)
+            dG = to_device(g)
+            ga = empty_device(n, n, cplx)
+            gb = empty_device(n, 4, cplx)
+            dF = empty_device(n, 4, cplx)
+            torch.cuda.synchronize()
+            ws.expmv_backward(dA, dB, 1.0, dG, out_a=ga, out_b=gb)
+            ws.synchronize()
+            t0 = time.perf_counter()
+            ws.expmv(dA, dB, 1.0, out=dF)
+            ws.synchronize()
+            t_fwd = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ws.expmv_backward(dA, dB, 1.0, dG, out_a=ga, out_b=gb)
+            ws.synchronize()
+            t_bwd = time.perf_counter() - t0
+            # Block-exponential reference of dL/dA = L(A^H, G B^H).
+            blk = np.zeros((2 * n, 2 * n), dtype=np.complex128 if cplx else np.float64)
+            blk[:n, :n] = a.conj().T
+            blk[n:, n:] = a.conj().T
+            blk[:n, n:] = g @ b.conj().T
+            e, _ = ws.expm(blk)
+            ra = e[:n, n:]
+            got = ga.owner.cpu().numpy().T
+            err = float(np.abs(got - ra).max() / np.abs(ra).max())
+            out.append({"n": n, "complex": cplx, "forward_ms": round(t_fwd * 1e3, 2),
+                        "backward_ms": round(t_bwd * 1e3, 2), "grad_a_rel_error_vs_block_expm": err})
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
     if not os.path.exists(path):
@@ -349,6 +394,9 @@ def main():
 
     # ---- batched frequency sweep ------------------------------------------------
     result["batched_sweep"] = sweep_leg(ws)
+
+    # ---- gradients through exp(tA)B ---------------------------------------------
+    result["expmv_gradients"] = grad_leg(ws)
 
     if not args.no_cpu:
         import oracle as O

@@ -68,9 +68,9 @@ std::vector<int> assign_streams(const std::vector<int>& n, int streams)
 // Capture body(i) for every system i on its stream into one graph. Every
 // forked stream receives at least one system (never more streams than
 // systems), so no branch of the capture is empty.
-template <typename F>
+template <typename F, typename Pre = void (*)(), typename Post = void (*)()>
 hipGraphExec_t capture(Context* ctx, const std::vector<int>& n, int streams, std::vector<std::pair<void*, size_t>>* hold,
-                       F&& body)
+                       F&& body, Pre pre = [] {}, Post post = [] {})
 {
     if (n.empty()) fail(TNS_EINVAL, "an empty batch has nothing to capture");
     streams = std::max(1, std::min(streams, (int)n.size()));
@@ -83,6 +83,7 @@ hipGraphExec_t capture(Context* ctx, const std::vector<int>& n, int streams, std
     TNS_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
     ctx->hold = hold;
     try {
+        pre();  // on the origin stream, before the fork
         TNS_HIP(hipEventRecord(fork.e, origin));
         for (int s = 0; s < streams; s++) TNS_HIP(hipStreamWaitEvent(side.s[s], fork.e, 0));
         for (size_t i = 0; i < n.size(); i++) {
@@ -94,6 +95,7 @@ hipGraphExec_t capture(Context* ctx, const std::vector<int>& n, int streams, std
             TNS_HIP(hipEventRecord(join[s].e, side.s[s]));
             TNS_HIP(hipStreamWaitEvent(origin, join[s].e, 0));
         }
+        post();  // on the origin stream, after the join
         TNS_HIP(hipStreamEndCapture(origin, &graph));
     } catch (...) {
         ctx->stream = origin;
@@ -184,8 +186,8 @@ void BatchPlan::tune()
             zcopy2d(ctx, n[i], nrhs, b[i], ldb[i], sb[i].get(), n[i]);
         }
     };
-    const int stream_options[] = {4, 8, 16};
-    const int cfg_options[] = {-1, 1, 2};
+    const int stream_options[] = {8, 16, 32, 64};
+    const int cfg_options[] = {-1, 2};
     double best = 1e300;
     int best_s = 8, best_c = -1;
     for (int s : stream_options)
@@ -203,7 +205,7 @@ void BatchPlan::tune()
             }
             gemm_force_config(-1, -1);
             double t = 1e300;
-            for (int rep = 0; rep < 2; rep++) {
+            for (int rep = 0; rep < 3; rep++) {
                 restore();
                 TNS_HIP(hipStreamSynchronize(ctx->stream));
                 const double t0 = now_ms();
@@ -252,19 +254,24 @@ void BatchPlan::build_rcond(int t)
     TNS_HIP(hipHostMalloc(&h, sizeof(zd) * 2 * total, hipHostMallocDefault));
     h_panels = static_cast<zd*>(h);
     d_panels = DevBuf<zd>(ctx, 2 * total);
+    // One copy of every input panel in, the solves side by side, one copy
+    // of every result out (the panels are contiguous in both buffers).
     auto body = [&](int i, int trans) {
         const int ni = n[i];
-        zd* hin = h_panels + rc_off[i];
-        zd* hout = h_panels + total + rc_off[i];
         zd* dx = d_panels.get() + rc_off[i];
         zd* dy = d_panels.get() + total + rc_off[i];
-        TNS_HIP(hipMemcpyAsync(dx, hin, sizeof(zd) * ni * t, hipMemcpyHostToDevice, ctx->stream));
         zcopy2d(ctx, ni, t, dx, ni, dy, ni);
         zgetrs(ctx, trans, ni, t, a[i], lda[i], perm.get() + offset[i], dy, ni);
-        TNS_HIP(hipMemcpyAsync(hout, dy, sizeof(zd) * ni * t, hipMemcpyDeviceToHost, ctx->stream));
     };
-    apply_exec = capture(ctx, n, streams, &rc_held, [&](int i) { body(i, 0); });
-    adjoint_exec = capture(ctx, n, streams, &rc_held, [&](int i) { body(i, 1); });
+    auto pre = [&]() {
+        TNS_HIP(hipMemcpyAsync(d_panels.get(), h_panels, sizeof(zd) * total, hipMemcpyHostToDevice, ctx->stream));
+    };
+    auto post = [&]() {
+        TNS_HIP(hipMemcpyAsync(h_panels + total, d_panels.get() + total, sizeof(zd) * total, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    };
+    apply_exec = capture(ctx, n, streams, &rc_held, [&](int i) { body(i, 0); }, pre, post);
+    adjoint_exec = capture(ctx, n, streams, &rc_held, [&](int i) { body(i, 1); }, pre, post);
 }
 
 void BatchPlan::rcond(double* out, int t)
@@ -368,7 +375,7 @@ std::unique_ptr<BatchPlan> make_batch(Context* ctx, int count, const int* n, zd*
         p->tune();
         ctx->batch_tuning[key] = {p->streams, p->gemm_cfg};
     } else {
-        p->build(8, -1);
+        p->build(16, -1);
     }
     return p;
 }

@@ -16,7 +16,7 @@ struct BatchPlan {
     std::vector<zd*> a, b;                 // the caller's device matrices
     DevBuf<int> piv, perm, info;
     DevBuf<double> anorm;
-    int streams = 8, gemm_cfg = -1;        // the chosen capture fan-out and 4M tile
+    int streams = 16, gemm_cfg = -1;       // the chosen capture fan-out and 4M tile
     double tuned_ms = 0.0;                 // the tuner's best time, if it ran
     bool solved = false;
     hipGraphExec_t solve_exec = nullptr;
@@ -43,7 +43,7 @@ struct BatchPlan {
     void pivots(int index, int* out) const;
 };
 
-// tune: 0 build with the defaults (8 streams, heuristic tiles); 1 use the
+// tune: 0 build with the defaults (16 streams, heuristic tiles); 1 use the
 // context's remembered choice for this batch shape, tuning if there is
 // none; 2 tune again.
 std::unique_ptr<BatchPlan> make_batch(Context* ctx, int count, const int* n, zd* const* a, const int* lda,

@@ -48,7 +48,7 @@ def test_sweep_matches_the_oracle(ws, tune):
     sing = plan.solve()
     rc = plan.reciprocal_condition()
     streams, cfg, tuned = plan.config
-    assert streams in (4, 8, 16) and cfg in (-1, 0, 1, 2)
+    assert streams in (1, 4, 8, 14, 16, 32, 64) and cfg in (-1, 0, 1, 2)
     if tune == 2:
         assert tuned > 0.0
     for i, (a, b) in enumerate(zip(mats, rhs)):
