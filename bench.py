@@ -200,7 +200,7 @@ def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
     t0 = time.perf_counter()
     rc = plan.reciprocal_condition()
     rc_s = time.perf_counter() - t0
-    streams, cfg, tuned_ms = plan.config
+    block, cfg, tuned_ms = plan.config
     orders = [m.shape[0] for m in mats]
     # Residual check of every system (host): max backward error.
     worst = 0.0
@@ -215,7 +215,7 @@ def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
             "solve_ms": round(best * 1e3, 2), "solves_per_s": round(count / best, 1),
             "gflops": round(W.sweep_flops(orders, nrhs) / best / 1e9, 1),
             "rcond_ms": round(rc_s * 1e3, 2), "solves_plus_rcond_per_s": round(count / (best + rc_s), 1),
-            "tuned": {"streams": streams, "zgemm_config": cfg, "graph_ms": round(tuned_ms, 2),
+            "tuned": {"panel_width": block, "zgemm_tile": "64x64" if cfg == 1 else "64x32", "graph_ms": round(tuned_ms, 2),
                       "tuning_s": round(tune_s, 2)},
             "singular": int((sing >= 0).sum()), "max_backward_error": worst,
             "rcond_range": [float(rc.min()), float(rc.max())]}

@@ -1,101 +1,62 @@
 // Batched solves of many complex systems of uneven order (a frequency
 // sweep Z(w) X = B): every system's 1-norm, LU with partial pivoting
 // (BlockedLu.Factor, BlockedLu.cs:45-141) and solve (BlockedLu.Solve,
-// 148-206) are captured once into one hipGraph and run as one launch. The
-// capture forks over S streams, so independent systems run side by side
-// (a single system's panels occupy one CU); systems go to the stream with
-// the least work so far, largest first. The reciprocal condition numbers
-// (Condition.ReciprocalOne, NormEstimate.cs:432-466) run the estimator of
-// every system in lockstep: each round is one replay of a captured graph
-// of inverse products (all systems' A^-1 X, or all A^-H X) between pinned
-// host panels, then the host logic of every estimate advances.
+// 148-206), as one captured hipGraph run as one launch.
 //
-// The autotuner times the graph for a few stream counts and 4M GEMM tile
-// shapes on scratch copies of the systems and keeps the fastest, per
+// The graph is a chain of grouped launches: each block step of the LU is
+// one launch per kernel for all systems at once (their panels side by side,
+// one workgroup each; their interchanges, triangular solves and trailing
+// updates as flat grids over every system's columns or tiles). Systems are
+// taken largest first; step j covers the systems of order > j * nb, so the
+// work of one launch spreads over the whole chip instead of one system's
+// panel occupying one CU while the rest wait, and the graph has a few
+// hundred nodes instead of a hundred per system. Each system's part of a
+// step is exactly what its own factorization would launch (getrf_impl and
+// panel() in lu.hip): the same panel chunks, move lists, solves and
+// products, so pivots and factors are those of the one-system path.
+//
+// The reciprocal condition numbers (Condition.ReciprocalOne,
+// NormEstimate.cs:432-466) run the estimator of every system in lockstep:
+// each round is one replay of a captured graph (one copy of all probe
+// panels in, one grouped solve of all systems, one copy out), then the host
+// logic of every estimate advances.
+//
+// The autotuner times the graph for panel widths {16, 32} and two grouped
+// GEMM tiles on scratch copies of the systems and keeps the fastest, per
 // context and batch shape.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "algos.hpp"
 #include "batch.hpp"
+#include "grouped.hpp"
 
 namespace tns {
 
 namespace {
 
-// Streams for capture forks; default priority (non-blocking).
-struct Streams {
-    std::vector<hipStream_t> s;
-    explicit Streams(int count)
-    {
-        s.resize(count, nullptr);
-        for (auto& x : s) TNS_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-    }
-    ~Streams()
-    {
-        for (auto x : s)
-            if (x) (void)hipStreamDestroy(x);
-    }
-};
-
-struct Event {
-    hipEvent_t e = nullptr;
-    Event() { TNS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
-    ~Event()
-    {
-        if (e) (void)hipEventDestroy(e);
-    }
-};
-
-// Largest systems first, each to the least-loaded stream (cost n^3).
-std::vector<int> assign_streams(const std::vector<int>& n, int streams)
+double now_ms()
 {
-    std::vector<int> order(n.size()), where(n.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
-    std::vector<double> load(streams, 0.0);
-    for (int i : order) {
-        const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        where[i] = s;
-        load[s] += (double)n[i] * n[i] * n[i];
-    }
-    return where;
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Capture body(i) for every system i on its stream into one graph. Every
-// forked stream receives at least one system (never more streams than
-// systems), so no branch of the capture is empty.
-template <typename F, typename Pre = void (*)(), typename Post = void (*)()>
-hipGraphExec_t capture(Context* ctx, const std::vector<int>& n, int streams, std::vector<std::pair<void*, size_t>>* hold,
-                       F&& body, Pre pre = [] {}, Post post = [] {})
+// Ints per system for a panel's move list: 1 + 2 * (2 * widest chunk).
+constexpr int LIST_INTS = 1 + 2 * 64;
+
+// Capture body() on ctx->stream into a graph and instantiate it.
+template <typename F>
+hipGraphExec_t capture(Context* ctx, std::vector<std::pair<void*, size_t>>* hold, F&& body)
 {
-    if (n.empty()) fail(TNS_EINVAL, "an empty batch has nothing to capture");
-    streams = std::max(1, std::min(streams, (int)n.size()));
-    Streams side(streams);
-    std::vector<int> where = assign_streams(n, streams);
     hipStream_t origin = ctx->stream;
-    Event fork;
-    std::vector<Event> join(streams);
     hipGraph_t graph = nullptr;
     TNS_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
     ctx->hold = hold;
     try {
-        pre();  // on the origin stream, before the fork
-        TNS_HIP(hipEventRecord(fork.e, origin));
-        for (int s = 0; s < streams; s++) TNS_HIP(hipStreamWaitEvent(side.s[s], fork.e, 0));
-        for (size_t i = 0; i < n.size(); i++) {
-            ctx->stream = side.s[where[i]];
-            body((int)i);
-        }
-        ctx->stream = origin;
-        for (int s = 0; s < streams; s++) {
-            TNS_HIP(hipEventRecord(join[s].e, side.s[s]));
-            TNS_HIP(hipStreamWaitEvent(origin, join[s].e, 0));
-        }
-        post();  // on the origin stream, after the join
+        body();
         TNS_HIP(hipStreamEndCapture(origin, &graph));
     } catch (...) {
         ctx->stream = origin;
@@ -114,12 +75,45 @@ hipGraphExec_t capture(Context* ctx, const std::vector<int>& n, int streams, std
     return exec;
 }
 
-double now_ms()
+// Flat launches: give each record its first workgroup; returns the total.
+template <typename R, typename Blocks>
+int assign_starts(std::vector<R>& recs, Blocks&& blocks_of)
 {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    int total = 0;
+    for (auto& r : recs) {
+        r.start = total;
+        total += blocks_of(r);
+    }
+    return total;
 }
 
 }  // namespace
+
+void Schedule::upload(Context* ctx)
+{
+    dev = DevBuf<unsigned char>(ctx, host.empty() ? 16 : host.size());
+    if (!host.empty())
+        TNS_HIP(hipMemcpyAsync(dev.get(), host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void Schedule::record(Context* ctx) const
+{
+    for (const Launch& l : launches) {
+        const void* p = dev.get() + l.offset;
+        switch (l.kind) {
+        case INIT: zgrouped_init(ctx, static_cast<const InitRec<zd>*>(p), l.count); break;
+        case PANEL: zgrouped_panel(ctx, static_cast<const PanelRec<zd>*>(p), l.count, l.param); break;
+        case MOVES: zgrouped_moves(ctx, static_cast<const MovesRec<zd>*>(p), l.count, l.blocks, l.param); break;
+        case TRSM: zgrouped_trsm(ctx, static_cast<const TrsmRec<zd>*>(p), l.count, l.blocks); break;
+        case GEMM: zgrouped_gemm(ctx, static_cast<const GemmRec<zd>*>(p), l.count, l.blocks, l.param); break;
+        case SOLVE:
+            zgrouped_solve(ctx, static_cast<const SolveRec<zd>*>(p), l.count, l.param, l.param2, l.param3);
+            break;
+        default: fail(TNS_EINVAL, "unknown grouped launch");
+        }
+    }
+}
 
 BatchPlan::~BatchPlan() { release(); }
 
@@ -136,74 +130,187 @@ void BatchPlan::release()
     rc_held.clear();
     if (h_panels) (void)hipHostFree(h_panels);
     h_panels = nullptr;
+    solve_sched = Schedule();
+    apply_sched = Schedule();
+    adjoint_sched = Schedule();
     piv = DevBuf<int>();
     perm = DevBuf<int>();
     info = DevBuf<int>();
+    lists = DevBuf<int>();
     anorm = DevBuf<double>();
     d_panels = DevBuf<zd>();
 }
 
-// The per-system work of a solve: ||A||_1 (before the factors overwrite
-// it), LU in place, then X in place of B.
-void BatchPlan::record_solve(int i, zd* ai, int ldai, zd* bi, int ldbi)
+// getrf_impl + getrs_impl of every grouped system, step by step. Block step
+// jb factors the m x w panel at (jb, jb) in column chunks of the width the
+// register kernel allows for its height (panel() in lu.hip): chunk c0
+// factors m - c0 rows, moves its interchanges to every other column,
+// solves and updates the panel columns to its right; then U12 := L11^-1 A12
+// and A22 -= A21 U12. Chunks start at multiples of their width, so the
+// launches of step jb go by chunk offset c0 = 0, 4, 8, ... and, within it,
+// panel, moves, solve, update: each system meets its own operations in its
+// own order.
+void BatchPlan::plan_solve(Schedule& s, const std::vector<zd*>& A, const std::vector<zd*>& B,
+                           const std::vector<int>& LDA, const std::vector<int>& LDB, int nb, int cfg) const
 {
-    const int ni = n[i];
-    znorm(ctx, '1', ni, ni, ai, ldai, anorm.get() + i);
-    zgetrf(ctx, ni, ni, ai, ldai, 0, piv.get() + offset[i], perm.get() + offset[i], info.get() + i);
-    zgetrs(ctx, 0, ni, nrhs, ai, ldai, perm.get() + offset[i], bi, ldbi);
+    s.launches.clear();
+    s.host.clear();
+    if (order.empty()) return;
+    auto add = [&](int kind, size_t off, int cnt, int blocks, int param, int param2 = 0, int param3 = 0) {
+        if (cnt > 0) s.launches.push_back({kind, off, cnt, blocks, param, param2, param3});
+    };
+    const int maxn = n[order[0]];
+    int* d_piv = piv.get();
+    int* d_perm = perm.get();
+    int* d_info = info.get();
+    int* d_lists = lists.get();
+    double* d_anorm = anorm.get();
+    int bm = 64, bn = 32;
+    grouped_gemm_tile(cfg, bm, bn);
+
+    {
+        std::vector<InitRec<zd>> init;
+        for (int i : order) init.push_back({A[i], d_perm + offset[i], d_info + i, d_anorm + i, LDA[i], n[i]});
+        add(Schedule::INIT, s.put(init), (int)init.size(), 0, 0);
+    }
+    auto emit_trsm_gemm = [&](std::vector<TrsmRec<zd>>& tr, std::vector<GemmRec<zd>>& gm) {
+        const int tb = assign_starts(tr, [](const TrsmRec<zd>& r) {
+            return (r.ncols + GROUPED_TRSM_COLUMNS - 1) / GROUPED_TRSM_COLUMNS;
+        });
+        add(Schedule::TRSM, s.put(tr), (int)tr.size(), tb, 0);
+        const int gb = assign_starts(gm, [&](const GemmRec<zd>& r) {
+            return ((r.m + bm - 1) / bm) * ((r.n + bn - 1) / bn);
+        });
+        add(Schedule::GEMM, s.put(gm), (int)gm.size(), gb, cfg);
+    };
+
+    for (int jb = 0; jb < maxn; jb += nb) {
+        for (int c0 = 0; c0 < nb; c0 += 4) {
+            std::vector<PanelRec<zd>> pr[4];
+            int maxm[4] = {0, 0, 0, 0};
+            std::vector<MovesRec<zd>> mv;
+            int longest = 0;
+            std::vector<TrsmRec<zd>> tr;
+            std::vector<GemmRec<zd>> gm;
+            for (int i : order) {
+                const int ni = n[i];
+                if (ni <= jb) break;
+                const int m = ni - jb, w = nb < m ? nb : m;
+                const int cw = zpanel_chunk_width(m);
+                if (c0 >= w || c0 % cw != 0) continue;
+                const int cwid = cw < w - c0 ? cw : w - c0;
+                const int ld = LDA[i];
+                zd* top = A[i] + jb + (size_t)jb * ld;
+                zd* sub = top + c0 + (size_t)c0 * ld;
+                const int mc = m - c0;
+                const int cls = zpanel_class(mc);
+                int* list = d_lists + (size_t)i * LIST_INTS;
+                pr[cls].push_back({sub, d_piv + offset[i] + jb + c0, d_perm + offset[i] + jb + c0, list, d_info + i, ld,
+                                   mc, cwid, jb + c0});
+                maxm[cls] = std::max(maxm[cls], mc);
+                if (ni - cwid > 0) {
+                    mv.push_back({A[i] + jb + c0, list, ld, jb + c0, jb + c0 + cwid, ni, 0});
+                    longest = std::max(longest, 2 * cwid);
+                }
+                if (c0 + cwid < w) {
+                    tr.push_back({sub, sub + (size_t)cwid * ld, ld, ld, cwid, w - c0 - cwid, 0});
+                    if (mc > cwid)
+                        gm.push_back({sub + cwid, sub + (size_t)cwid * ld, sub + cwid + (size_t)cwid * ld, ld, ld, ld,
+                                      mc - cwid, w - c0 - cwid, cwid, 0});
+                }
+            }
+            for (int c = 0; c < 4; c++) add(Schedule::PANEL, s.put(pr[c]), (int)pr[c].size(), 0, maxm[c]);
+            if (!mv.empty()) {
+                const int cpb = grouped_moves_columns(longest);
+                const int blocks = assign_starts(mv, [&](const MovesRec<zd>& r) {
+                    return (r.c0 + (r.ncols - r.c1) + cpb - 1) / cpb;
+                });
+                add(Schedule::MOVES, s.put(mv), (int)mv.size(), blocks, longest);
+            }
+            emit_trsm_gemm(tr, gm);
+        }
+        std::vector<TrsmRec<zd>> tr;
+        std::vector<GemmRec<zd>> gm;
+        for (int i : order) {
+            const int ni = n[i];
+            if (ni <= jb) break;
+            const int m = ni - jb, w = nb < m ? nb : m;
+            if (jb + w >= ni) continue;
+            const int ld = LDA[i];
+            zd* top = A[i] + jb + (size_t)jb * ld;
+            tr.push_back({top, top + (size_t)w * ld, ld, ld, w, ni - jb - w, 0});
+            gm.push_back({top + w, top + (size_t)w * ld, top + w + (size_t)w * ld, ld, ld, ld, m - w, ni - jb - w, w, 0});
+        }
+        emit_trsm_gemm(tr, gm);
+    }
+
+    std::vector<SolveRec<zd>> sv;
+    for (int i : order)
+        sv.push_back({A[i], d_perm + offset[i], B[i], B[i], d_info + i, LDA[i], LDB[i], LDB[i], n[i], nrhs});
+    add(Schedule::SOLVE, s.put(sv), (int)sv.size(), 0, 0, grouped_solve_group(ctx, maxn), maxn);
 }
 
-void BatchPlan::build(int streams_, int gemm_cfg_)
+hipGraphExec_t BatchPlan::capture_solve(Schedule& s, const std::vector<zd*>& A, const std::vector<zd*>& B,
+                                        const std::vector<int>& LDA, const std::vector<int>& LDB, int nb, int cfg,
+                                        std::vector<std::pair<void*, size_t>>* hold)
+{
+    plan_solve(s, A, B, LDA, LDB, nb, cfg);
+    s.upload(ctx);
+    return capture(ctx, hold, [&]() {
+        s.record(ctx);
+        // Systems too large to group: the one-system path, in the same graph.
+        for (int i : single) {
+            znorm(ctx, '1', n[i], n[i], A[i], LDA[i], anorm.get() + i);
+            zgetrf(ctx, n[i], n[i], A[i], LDA[i], 0, piv.get() + offset[i], perm.get() + offset[i], info.get() + i);
+            zgetrs(ctx, 0, n[i], nrhs, A[i], LDA[i], perm.get() + offset[i], B[i], LDB[i]);
+        }
+    });
+}
+
+void BatchPlan::build(int nb, int cfg)
 {
     if (solve_exec) (void)hipGraphExecDestroy(solve_exec);
     solve_exec = nullptr;
     for (auto& h : held) ctx->pool.give(h.first, h.second);
     held.clear();
-    streams = std::max(1, std::min(streams_, count));
-    gemm_cfg = gemm_cfg_;
+    block = nb;
+    gemm_cfg = cfg;
     if (count == 0) return;
-    gemm_force_config(-1, gemm_cfg);
-    try {
-        solve_exec = capture(ctx, n, streams, &held, [&](int i) { record_solve(i, a[i], lda[i], b[i], ldb[i]); });
-    } catch (...) {
-        gemm_force_config(-1, -1);
-        throw;
-    }
-    gemm_force_config(-1, -1);
+    solve_exec = capture_solve(solve_sched, a, b, lda, ldb, nb, cfg, &held);
 }
 
 void BatchPlan::tune()
 {
     // Scratch copies: the candidates factor these, never the caller's data.
     std::vector<DevBuf<zd>> sa, sb;
+    std::vector<zd*> pa(count), pb(count);
+    std::vector<int> la(count), lb(count);
     for (int i = 0; i < count; i++) {
-        sa.emplace_back(ctx, (size_t)n[i] * n[i]);
-        sb.emplace_back(ctx, (size_t)n[i] * nrhs);
+        la[i] = lb[i] = std::max(1, n[i]);
+        sa.emplace_back(ctx, (size_t)la[i] * std::max(1, n[i]));
+        sb.emplace_back(ctx, (size_t)lb[i] * std::max(1, nrhs));
+        pa[i] = sa[i].get();
+        pb[i] = sb[i].get();
     }
     auto restore = [&]() {
         for (int i = 0; i < count; i++) {
-            zcopy2d(ctx, n[i], n[i], a[i], lda[i], sa[i].get(), n[i]);
-            zcopy2d(ctx, n[i], nrhs, b[i], ldb[i], sb[i].get(), n[i]);
+            zcopy2d(ctx, n[i], n[i], a[i], lda[i], pa[i], la[i]);
+            zcopy2d(ctx, n[i], nrhs, b[i], ldb[i], pb[i], lb[i]);
         }
     };
-    const int stream_options[] = {8, 16, 32, 64};
-    const int cfg_options[] = {-1, 2};
     double best = 1e300;
-    int best_s = 8, best_c = -1;
-    for (int s : stream_options)
-        for (int c : cfg_options) {
-            if (s > count && s != stream_options[0]) continue;  // the same capture as a smaller count
+    int best_nb = 32, best_cfg = 2;
+    for (int nb : {16, 32})
+        for (int cfg : {1, 2}) {
             std::vector<std::pair<void*, size_t>> hold;
-            gemm_force_config(-1, c);
+            Schedule sched;
             hipGraphExec_t exec = nullptr;
             try {
-                exec = capture(ctx, n, s, &hold, [&](int i) { record_solve(i, sa[i].get(), n[i], sb[i].get(), n[i]); });
+                exec = capture_solve(sched, pa, pb, la, lb, nb, cfg, &hold);
             } catch (...) {
-                gemm_force_config(-1, -1);
                 for (auto& h : hold) ctx->pool.give(h.first, h.second);
                 throw;
             }
-            gemm_force_config(-1, -1);
             double t = 1e300;
             for (int rep = 0; rep < 3; rep++) {
                 restore();
@@ -217,12 +324,12 @@ void BatchPlan::tune()
             for (auto& h : hold) ctx->pool.give(h.first, h.second);
             if (t < best) {
                 best = t;
-                best_s = s;
-                best_c = c;
+                best_nb = nb;
+                best_cfg = cfg;
             }
         }
     tuned_ms = best;
-    build(best_s, best_c);
+    build(best_nb, best_cfg);
 }
 
 void BatchPlan::solve(int* singular_columns)
@@ -239,7 +346,9 @@ void BatchPlan::solve(int* singular_columns)
 }
 
 // Pinned host panels (in and out, n_i x t each) and their device twins,
-// and the two graphs of inverse products between them.
+// and the two graphs of inverse products between them: one copy of every
+// input panel in, one grouped solve of every system, one copy of every
+// result out (the panels are contiguous in both buffers).
 void BatchPlan::build_rcond(int t)
 {
     rc_t = t;
@@ -251,27 +360,39 @@ void BatchPlan::build_rcond(int t)
     }
     rc_off[count] = total;
     void* h = nullptr;
-    TNS_HIP(hipHostMalloc(&h, sizeof(zd) * 2 * total, hipHostMallocDefault));
+    TNS_HIP(hipHostMalloc(&h, sizeof(zd) * std::max<size_t>(1, 2 * total), hipHostMallocDefault));
     h_panels = static_cast<zd*>(h);
-    d_panels = DevBuf<zd>(ctx, 2 * total);
-    // One copy of every input panel in, the solves side by side, one copy
-    // of every result out (the panels are contiguous in both buffers).
-    auto body = [&](int i, int trans) {
-        const int ni = n[i];
-        zd* dx = d_panels.get() + rc_off[i];
-        zd* dy = d_panels.get() + total + rc_off[i];
-        zcopy2d(ctx, ni, t, dx, ni, dy, ni);
-        zgetrs(ctx, trans, ni, t, a[i], lda[i], perm.get() + offset[i], dy, ni);
+    d_panels = DevBuf<zd>(ctx, std::max<size_t>(1, 2 * total));
+    int maxn = 0;
+    for (int i : order) maxn = std::max(maxn, n[i]);
+    const int group = grouped_solve_group(ctx, maxn);
+    auto graph = [&](Schedule& s, int trans) {
+        std::vector<SolveRec<zd>> sv;
+        for (int i : order) {
+            zd* dx = d_panels.get() + rc_off[i];
+            zd* dy = d_panels.get() + total + rc_off[i];
+            const int ld = std::max(1, n[i]);
+            sv.push_back({a[i], perm.get() + offset[i], dx, dy, nullptr, lda[i], ld, ld, n[i], t});
+        }
+        s.launches.clear();
+        s.host.clear();
+        if (!sv.empty()) s.launches.push_back({Schedule::SOLVE, s.put(sv), (int)sv.size(), 0, trans, group, maxn});
+        s.upload(ctx);
+        return capture(ctx, &rc_held, [&]() {
+            TNS_HIP(hipMemcpyAsync(d_panels.get(), h_panels, sizeof(zd) * total, hipMemcpyHostToDevice, ctx->stream));
+            s.record(ctx);
+            for (int i : single) {
+                zd* dx = d_panels.get() + rc_off[i];
+                zd* dy = d_panels.get() + total + rc_off[i];
+                zcopy2d(ctx, n[i], t, dx, n[i], dy, n[i]);
+                zgetrs(ctx, trans, n[i], t, a[i], lda[i], perm.get() + offset[i], dy, n[i]);
+            }
+            TNS_HIP(hipMemcpyAsync(h_panels + total, d_panels.get() + total, sizeof(zd) * total,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        });
     };
-    auto pre = [&]() {
-        TNS_HIP(hipMemcpyAsync(d_panels.get(), h_panels, sizeof(zd) * total, hipMemcpyHostToDevice, ctx->stream));
-    };
-    auto post = [&]() {
-        TNS_HIP(hipMemcpyAsync(h_panels + total, d_panels.get() + total, sizeof(zd) * total, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    };
-    apply_exec = capture(ctx, n, streams, &rc_held, [&](int i) { body(i, 0); }, pre, post);
-    adjoint_exec = capture(ctx, n, streams, &rc_held, [&](int i) { body(i, 1); }, pre, post);
+    apply_exec = graph(apply_sched, 0);
+    adjoint_exec = graph(adjoint_sched, 1);
 }
 
 void BatchPlan::rcond(double* out, int t)
@@ -352,11 +473,19 @@ std::unique_ptr<BatchPlan> make_batch(Context* ctx, int count, const int* n, zd*
     p->b.assign(b, b + count);
     p->offset.assign(count + 1, 0);
     for (int i = 0; i < count; i++) p->offset[i + 1] = p->offset[i] + n[i];
+    // Grouped: orders the register panel kernel holds and whose solve fits
+    // one workgroup's LDS; the rest go one by one.
+    for (int i = 0; i < count; i++) {
+        const bool grouped = zpanel_chunk_width(std::max(1, n[i])) > 0 && grouped_solve_group(ctx, n[i]) > 0;
+        (grouped ? p->order : p->single).push_back(i);
+    }
+    std::stable_sort(p->order.begin(), p->order.end(), [&](int x, int y) { return n[x] > n[y]; });
     const size_t total = (size_t)p->offset[count];
-    p->piv = DevBuf<int>(ctx, total);
-    p->perm = DevBuf<int>(ctx, total);
-    p->info = DevBuf<int>(ctx, count);
-    p->anorm = DevBuf<double>(ctx, count);
+    p->piv = DevBuf<int>(ctx, std::max<size_t>(1, total));
+    p->perm = DevBuf<int>(ctx, std::max<size_t>(1, total));
+    p->info = DevBuf<int>(ctx, (size_t)std::max(1, count));
+    p->lists = DevBuf<int>(ctx, (size_t)std::max(1, count) * LIST_INTS);
+    p->anorm = DevBuf<double>(ctx, (size_t)std::max(1, count));
     // Shape key of the tuner's memory: count, largest order, total n^3.
     double work = 0.0;
     int largest = 0;
@@ -368,14 +497,14 @@ std::unique_ptr<BatchPlan> make_batch(Context* ctx, int count, const int* n, zd*
                             std::to_string((long long)(work / 1e6)) + ":" + std::to_string(nrhs);
     auto it = ctx->batch_tuning.find(key);
     if (count == 0) {
-        p->build(1, -1);
+        p->build(32, 2);
     } else if (it != ctx->batch_tuning.end() && tune != 2) {
         p->build(it->second.first, it->second.second);
     } else if (tune) {
         p->tune();
-        ctx->batch_tuning[key] = {p->streams, p->gemm_cfg};
+        ctx->batch_tuning[key] = {p->block, p->gemm_cfg};
     } else {
-        p->build(16, -1);
+        p->build(32, 2);
     }
     return p;
 }

@@ -927,10 +927,10 @@ TNS_API int tns_zbatch_pivots(tns_batch* batch, int index, int* pivots)
     return guarded(batch->ctx, [&](Context*) { batch->plan->pivots(index, pivots); });
 }
 
-TNS_API int tns_zbatch_info(tns_batch* batch, int* streams, int* gemm_config, double* tuned_ms)
+TNS_API int tns_zbatch_info(tns_batch* batch, int* block, int* gemm_config, double* tuned_ms)
 {
     if (!batch) return TNS_EINVAL;
-    if (streams) *streams = batch->plan->streams;
+    if (block) *block = batch->plan->block;
     if (gemm_config) *gemm_config = batch->plan->gemm_cfg;
     if (tuned_ms) *tuned_ms = batch->plan->tuned_ms;
     return TNS_OK;

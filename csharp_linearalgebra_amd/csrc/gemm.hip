@@ -19,6 +19,7 @@
 // remapped so each XCD receives a contiguous band of tiles, grouped 8 tile
 // rows at a time, and the A row-panels and B column-panels a band shares
 // stay in that XCD's L2.
+#include "grouped.hpp"
 #include "tile_order.hpp"
 #include "tns_internal.hpp"
 
@@ -244,10 +245,9 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
 // the write-back (ComplexKernels.cs:213-240). flags: 1 unit alpha, 2 real
 // alpha, 4 zero beta, 8 unit beta. TA: 0 A, 1 A^T, 2 A^H.
 template <int BM, int BN, int BK, int WM, int WN, int TA>
-__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, 2)
-zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A, int lda,
-             const double2* __restrict__ B, int ldb, double br_, double bi_, double2* __restrict__ C,
-             int ldc, int flags, int tiles_m, int tiles_n)
+__device__ __forceinline__ void zgemm_tile(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A,
+                                           int lda, const double2* __restrict__ B, int ldb, double br_, double bi_,
+                                           double2* __restrict__ C, int ldc, int flags, int m0, int n0)
 {
     using S = Shape<BM, BN, BK, WM, WN>;
     using P = Planes<BM, BN, BK, TA>;
@@ -256,9 +256,6 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
     __shared__ double Br[P::B_SIZE];
     __shared__ double Bi[P::B_SIZE];
 
-    int tmi, tni;
-    tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
-    const int m0 = tmi * BM, n0 = tni * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % S::WAVES_M, wn = wave / S::WAVES_M;
 
@@ -380,6 +377,42 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
                 }
             }
         }
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int TA>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, 2)
+zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A, int lda,
+             const double2* __restrict__ B, int ldb, double br_, double bi_, double2* __restrict__ C,
+             int ldc, int flags, int tiles_m, int tiles_n)
+{
+    int tmi, tni;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
+    zgemm_tile<BM, BN, BK, WM, WN, TA>(m, n, k, ar_, ai_, A, lda, B, ldb, br_, bi_, C, ldc, flags, tmi * BM,
+                                       tni * BN);
+}
+
+// C_g -= A_g B_g for many independent products of different shapes in one
+// launch (the trailing updates of a batch of LUs): record g owns workgroups
+// [start_g, start_g + tiles_g), its tiles in column-major order. alpha = -1
+// and beta = 1 take the same write-back as zgemm_ex gives them.
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, 2)
+zgemm_grouped_kernel(const GemmRec<zd>* __restrict__ recs, int count)
+{
+    int lo = 0, hi = count - 1;
+    const int bid = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (recs[mid].start <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmRec<zd> r = recs[lo];
+    const int local = bid - r.start;
+    const int tiles_m = (r.m + BM - 1) / BM;
+    zgemm_tile<BM, BN, BK, WM, WN, 0>(r.m, r.n, r.k, -1.0, 0.0, reinterpret_cast<const double2*>(r.a), r.lda,
+                                      reinterpret_cast<const double2*>(r.b), r.ldb, 1.0, 0.0,
+                                      reinterpret_cast<double2*>(r.c), r.ldc, 2 | 8, (local % tiles_m) * BM,
+                                      (local / tiles_m) * BN);
 }
 
 __global__ void dscale_kernel(int m, int n, double beta, double* __restrict__ c, int ldc)
@@ -518,6 +551,24 @@ void dgemm(Context* ctx, int m, int n, int k, double alpha, const double* a, int
            double beta, double* c, int ldc)
 {
     dgemm_ex(ctx, 0, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+void grouped_gemm_tile(int cfg, int& bm, int& bn)
+{
+    bm = 64;
+    bn = cfg == 1 ? 64 : 32;
+}
+
+void zgrouped_gemm(Context* ctx, const GemmRec<zd>* recs, int count, int blocks, int cfg)
+{
+    if (count <= 0 || blocks <= 0) return;
+    if (cfg == 1)
+        hipLaunchKernelGGL((zgemm_grouped_kernel<64, 64, 16, 32, 32>), dim3(blocks), dim3(256), 0, ctx->stream, recs,
+                           count);
+    else
+        hipLaunchKernelGGL((zgemm_grouped_kernel<64, 32, 16, 32, 16>), dim3(blocks), dim3(256), 0, ctx->stream, recs,
+                           count);
+    TNS_HIP(hipGetLastError());
 }
 
 void zgemm(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda, const cplx* b, int ldb, cplx beta,

@@ -31,6 +31,7 @@
 //    off-diagonal update as a GEMM.
 #include <climits>
 
+#include "grouped.hpp"
 #include "tns_internal.hpp"
 #include "tns_scalar.hpp"
 
@@ -212,9 +213,9 @@ __device__ __forceinline__ zd blend(zd x, zd y, bool take) { return zd{blend(x.x
 // nonzero pattern); done slots are masked off.
 // ---------------------------------------------------------------------------
 template <typename T, int NB, int R, int NT>
-__global__ void __launch_bounds__(NT)
-getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
-                 int* __restrict__ list, int cap, int* __restrict__ info)
+__device__ __forceinline__ void getf2_reg_body(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv,
+                                               int off, int* __restrict__ perm, int* __restrict__ list, int cap,
+                                               int* __restrict__ info)
 {
     static_assert(NB <= 64 && NB % 4 == 0, "the nonzero mask is one ballot; columns go in groups of 4");
     constexpr int GW = NB < 8 ? NB : 8;  // columns per group of pivot-row reads
@@ -411,6 +412,22 @@ getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv
     }
 }
 
+template <typename T, int NB, int R, int NT>
+__global__ void __launch_bounds__(NT)
+getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
+                 int* __restrict__ list, int cap, int* __restrict__ info)
+{
+    getf2_reg_body<T, NB, R, NT>(m, w, a, lda, piv, off, perm, list, cap, info);
+}
+
+// The same panel for many matrices: workgroup g factors record g.
+template <typename T, int NB, int R, int NT>
+__global__ void __launch_bounds__(NT) getf2_grouped_kernel(const PanelRec<T>* __restrict__ recs)
+{
+    const PanelRec<T> r = recs[blockIdx.x];
+    getf2_reg_body<T, NB, R, NT>(r.m, r.w, r.a, r.lda, r.piv, r.off, r.perm, r.list, 2 * r.w, r.info);
+}
+
 // Panel factorization straight from global memory, for panels taller than
 // the register-resident kernel holds. One workgroup, column by column.
 template <typename T>
@@ -527,12 +544,12 @@ compact_moves_kernel(int count, int base, int* __restrict__ rowid, int* __restri
 // every thread reads its source element, one barrier, every thread writes
 // its destination, so all reads of a column precede its writes.
 template <typename T, int QM>
-__global__ void __launch_bounds__(256)
-apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const int* __restrict__ list)
+__device__ __forceinline__ void apply_moves_body(T* __restrict__ a, int lda, int c0, int c1, int ncols,
+                                                 const int* __restrict__ list, int block)
 {
     constexpr int CPB = 256 / QM;
     const int q = threadIdx.x % QM;
-    const int idx = blockIdx.x * CPB + threadIdx.x / QM;
+    const int idx = block * CPB + threadIdx.x / QM;
     const int left = c0, right = ncols - c1;
     const int count = list[0] < QM ? list[0] : QM;
     const bool active = idx < left + right && q < count;
@@ -545,6 +562,34 @@ apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const 
     }
     __syncthreads();
     if (active) col[list[1 + 2 * q]] = val;
+}
+
+template <typename T, int QM>
+__global__ void __launch_bounds__(256)
+apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const int* __restrict__ list)
+{
+    apply_moves_body<T, QM>(a, lda, c0, c1, ncols, list, blockIdx.x);
+}
+
+// Index of the record whose workgroup range holds `bid`: the last record
+// with start <= bid (records are in increasing start order, start[0] = 0).
+template <typename Rec>
+__device__ __forceinline__ int find_record(const Rec* __restrict__ recs, int count, int bid)
+{
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (recs[mid].start <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, int QM>
+__global__ void __launch_bounds__(256) apply_moves_grouped_kernel(const MovesRec<T>* __restrict__ recs, int count)
+{
+    const MovesRec<T> r = recs[find_record(recs, count, blockIdx.x)];
+    apply_moves_body<T, QM>(r.a, r.lda, r.c0, r.c1, r.ncols, r.list, blockIdx.x - r.start);
 }
 
 // Larger move lists (never produced by nb <= 64, kept for generality).
@@ -574,8 +619,8 @@ __global__ void apply_moves_slow_kernel(T* __restrict__ a, int lda, int c0, int 
 // order for every element. The step loop is unrolled, so the owner's
 // register index p / G is a constant.
 template <typename T, int BS, int CB, bool LOWER, int TR, bool UNIT>
-__global__ void __launch_bounds__(256)
-trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
+__device__ __forceinline__ void trsm_block_body(int w, int ncols, const T* __restrict__ A, int lda,
+                                                T* __restrict__ B, int ldb, int block)
 {
     constexpr int G = 256 / CB;
     constexpr int Q = BS / G;
@@ -584,7 +629,7 @@ trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restr
     __shared__ T D[BS][BS + 1];  // D[i][k] = op(A)(i, k)
     __shared__ T X[BS][CB + 1];
     const int tid = threadIdx.x;
-    const int col0 = blockIdx.x * CB;
+    const int col0 = block * CB;
     const T zero = s_zero(T{});
 
     for (int e = tid; e < BS * BS; e += 256) {
@@ -638,6 +683,22 @@ trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restr
         const int i = e % BS, cc = e / BS;
         if (i < w && col0 + cc < ncols) B[i + (size_t)(col0 + cc) * ldb] = X[i][cc];
     }
+}
+
+template <typename T, int BS, int CB, bool LOWER, int TR, bool UNIT>
+__global__ void __launch_bounds__(256)
+trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
+{
+    trsm_block_body<T, BS, CB, LOWER, TR, UNIT>(w, ncols, A, lda, B, ldb, blockIdx.x);
+}
+
+// U12 := L11^-1 A12 of many matrices in one launch.
+template <typename T>
+__global__ void __launch_bounds__(256) trsm_grouped_kernel(const TrsmRec<T>* __restrict__ recs, int count)
+{
+    const TrsmRec<T> r = recs[find_record(recs, count, blockIdx.x)];
+    trsm_block_body<T, tri_block<T>(), GROUPED_TRSM_COLUMNS, true, 0, true>(r.w, r.ncols, r.l, r.ldl, r.b, r.ldb,
+                                                                          blockIdx.x - r.start);
 }
 
 // Whole triangular solve for few right-hand sides: one workgroup of 256
@@ -856,6 +917,195 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
         }
     }
     for (int i = tid; i < n; i += 256) xg[i] = xs[i];
+}
+
+// trsv_lds_kernel for G right-hand sides at once, over columns already in
+// LDS (xs[c * n + i], c < nr <= G): every load of op(A) serves all of them.
+// Each column receives exactly trsv_lds_kernel's operations, in its order:
+// the left-looking dot products keep its lane split and reduction, wave c
+// solves column c's diagonal block, and the right-looking update keeps its
+// per-column skip of zero entries.
+template <typename T, bool LOWER, int TR, bool UNIT, int G>
+__device__ void trsv_lds_group(int n, const T* __restrict__ A, int lda, T* xs, int nr)
+{
+    constexpr int BS = tri_block<T>();
+    static_assert(G >= 1 && G <= 4, "one wavefront per column for the diagonal blocks");
+    __shared__ T part[G][BS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto at = [&](int i, int k) -> T {
+        T v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
+        return TR == 2 ? s_conj(v) : v;
+    };
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    const T zero = s_zero(T{});
+    const int nblk = (n + BS - 1) / BS;
+    for (int bq = 0; bq < nblk; bq++) {
+        const int blk = FORWARD ? bq : nblk - 1 - bq;
+        const int k0 = blk * BS;
+        const int kb = n - k0 < BS ? n - k0 : BS;
+        if (TR != 0) {
+            const int lo = FORWARD ? 0 : k0 + kb;
+            const int hi = FORWARD ? k0 : n;
+            for (int i = wave; i < kb; i += 4) {
+                T s0[G], s1[G];
+#pragma unroll
+                for (int c = 0; c < G; c++) s0[c] = s1[c] = zero;
+                int k = lo + lane;
+                for (; k + 64 < hi; k += 128) {
+                    const T a0 = at(k0 + i, k), a1 = at(k0 + i, k + 64);
+#pragma unroll
+                    for (int c = 0; c < G; c++) {
+                        if (c >= nr) continue;
+                        s0[c] = s_add(s0[c], s_mul(a0, xs[c * n + k]));
+                        s1[c] = s_add(s1[c], s_mul(a1, xs[c * n + k + 64]));
+                    }
+                }
+                if (k < hi) {
+                    const T a0 = at(k0 + i, k);
+#pragma unroll
+                    for (int c = 0; c < G; c++)
+                        if (c < nr) s0[c] = s_add(s0[c], s_mul(a0, xs[c * n + k]));
+                }
+#pragma unroll
+                for (int c = 0; c < G; c++) {
+                    T sum = s_add(s0[c], s1[c]);
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1) sum = s_add(sum, shfl_xor_t(sum, off));
+                    if (lane == 0) part[c][i] = sum;
+                }
+            }
+            __syncthreads();
+        }
+        if (wave < nr) {
+            const int c = wave;
+            const bool mine = lane < kb;
+            T d[BS];
+#pragma unroll
+            for (int k = 0; k < BS; k++) d[k] = (mine && k < kb) ? at(k0 + lane, k0 + k) : zero;
+            T xi = load_if(xs + c * n + k0 + lane, mine);
+            if (TR != 0 && mine) xi = s_sub(xi, part[c][lane]);
+#pragma unroll
+            for (int s = 0; s < BS; s++) {
+                const int p = FORWARD ? s : BS - 1 - s;
+                if (p >= kb) continue;
+                if (!UNIT && lane == p) xi = s_div(xi, d[p]);
+                const T xp = readlane_t(xi, p);
+                if (s_iszero(xp)) continue;
+                const bool ahead = FORWARD ? (lane > p && lane < kb) : (lane < p);
+                if (ahead) xi = s_add(xi, s_mul(s_neg(xp), d[p]));
+            }
+            if (mine) xs[c * n + k0 + lane] = xi;
+        }
+        __syncthreads();
+        if (TR == 0) {
+            const int r0 = FORWARD ? k0 + kb : 0;
+            const int r1 = FORWARD ? n : k0;
+            for (int i = r0 + tid; i < r1; i += 512) {
+                const int i2 = i + 256;
+                const bool two = i2 < r1;
+                T acc[G], acc2[G];
+#pragma unroll
+                for (int c = 0; c < G; c++) {
+                    acc[c] = load_if(xs + c * n + i, c < nr);
+                    acc2[c] = load_if(xs + c * n + i2, c < nr && two);
+                }
+                for (int q = 0; q < kb; q++) {
+                    const int p = FORWARD ? q : kb - 1 - q;
+                    const T ai = A[i + (size_t)(k0 + p) * lda];
+                    const T ai2 = load_if(A + i2 + (size_t)(k0 + p) * lda, two);
+#pragma unroll
+                    for (int c = 0; c < G; c++) {
+                        if (c >= nr) continue;
+                        const T xp = xs[c * n + k0 + p];
+                        if (s_iszero(xp)) continue;
+                        const T nx = s_neg(xp);
+                        acc[c] = s_add(acc[c], s_mul(nx, ai));
+                        if (two) acc2[c] = s_add(acc2[c], s_mul(nx, ai2));
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < G; c++) {
+                    if (c >= nr) continue;
+                    xs[c * n + i] = acc[c];
+                    if (two) xs[c * n + i2] = acc2[c];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// LU solves of many systems, one workgroup per system, G right-hand sides
+// at a time in LDS (Lu.cs:336-378, BlockedLu.cs:126-141): P x gathered on
+// the way in, L forward (unit), U backward; or, for the adjoint, U^H
+// forward, L^H backward (unit) and the permutation scattered on the way
+// out. The same operations per column as getrs_impl's.
+template <typename T, int TRANS, int G>
+__global__ void __launch_bounds__(256) getrs_grouped_kernel(const SolveRec<T>* __restrict__ recs)
+{
+    extern __shared__ unsigned char getrs_smem[];
+    T* xs = reinterpret_cast<T*>(getrs_smem);
+    const SolveRec<T> r = recs[blockIdx.x];
+    const int tid = threadIdx.x, n = r.n;
+    if (tid == 0 && r.info != nullptr && *r.info == INT_MAX) *r.info = -1;
+    constexpr int TR = sizeof(T) == 16 ? 2 : 1;
+    for (int c0 = 0; c0 < r.nrhs; c0 += G) {
+        const int nr = r.nrhs - c0 < G ? r.nrhs - c0 : G;
+        for (int e = tid; e < nr * n; e += 256) {
+            const int c = e / n, i = e % n;
+            const int src = TRANS ? i : r.perm[i];
+            xs[c * n + i] = r.x[src + (size_t)(c0 + c) * r.ldx];
+        }
+        __syncthreads();
+        if (!TRANS) {
+            trsv_lds_group<T, true, 0, true, G>(n, r.lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, false, 0, false, G>(n, r.lu, r.ldlu, xs, nr);
+        } else {
+            trsv_lds_group<T, false, TR, false, G>(n, r.lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, true, TR, true, G>(n, r.lu, r.ldlu, xs, nr);
+        }
+        for (int e = tid; e < nr * n; e += 256) {
+            const int c = e / n, i = e % n;
+            const int dst = TRANS ? r.perm[i] : i;
+            r.y[dst + (size_t)(c0 + c) * r.ldy] = xs[c * n + i];
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double lu_nan_max(double a, double b)
+{
+    if (isnan(a) || isnan(b)) return NAN;
+    return a > b ? a : b;
+}
+
+// The start of many LUs: ||A||_1 (each column summed as the norm kernels
+// sum it: lanes strided by 64, then a butterfly), perm = identity, info =
+// the no-zero-pivot sentinel. One workgroup per matrix.
+template <typename T>
+__global__ void __launch_bounds__(256) lu_init_grouped_kernel(const InitRec<T>* __restrict__ recs)
+{
+    __shared__ double red[4];
+    const InitRec<T> r = recs[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < r.n; i += 256) r.perm[i] = i;
+    double best = 0.0;
+    for (int j = wave; j < r.n; j += 4) {
+        const T* col = r.a + (size_t)j * r.lda;
+        double s = 0.0;
+        for (int i = lane; i < r.n; i += 64) s += s_abs(col[i]);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+        best = lu_nan_max(best, s);
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    if (tid == 0) {
+        double v = red[0];
+        for (int q = 1; q < 4; q++) v = lu_nan_max(v, red[q]);
+        *r.anorm = v;
+        *r.info = INT_MAX;
+    }
 }
 
 // dst[i] = src[perm[i]] (gather) or dst[perm[i]] = src[i] (scatter).
@@ -1275,6 +1525,105 @@ void dcopy2d(Context* ctx, int m, int n, const double* src, int lds, double* dst
 void zcopy2d(Context* ctx, int m, int n, const cplx* src, int lds, cplx* dst, int ldd)
 {
     copy2d<zd>(ctx, m, n, src, lds, dst, ldd);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches (grouped.hpp).
+// ---------------------------------------------------------------------------
+int zpanel_chunk_width(int m) { return reg_panel_width<zd>(m); }
+
+// The getf2_reg variant for height m, in getf2_reg's order.
+int zpanel_class(int m)
+{
+    if (m <= 512) return 0;
+    if (m <= 1024) return 1;
+    if (m <= 2048) return 2;
+    if (m <= 4096) return 3;
+    return -1;
+}
+
+void zgrouped_init(Context* ctx, const InitRec<zd>* recs, int count)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL((lu_init_grouped_kernel<zd>), dim3(count), dim3(256), 0, ctx->stream, recs);
+    TNS_HIP(hipGetLastError());
+}
+
+namespace {
+template <int NB, int R>
+void launch_grouped_panel(Context* ctx, const PanelRec<zd>* recs, int count, int max_m)
+{
+    const int need = (max_m + R - 1) / R;
+    int nt = ((need + 63) / 64) * 64;
+    if (nt < 64) nt = 64;
+    hipLaunchKernelGGL((getf2_grouped_kernel<zd, NB, R, 512>), dim3(count), dim3(nt), 0, ctx->stream, recs);
+}
+}  // namespace
+
+void zgrouped_panel(Context* ctx, const PanelRec<zd>* recs, int count, int max_m)
+{
+    if (count <= 0) return;
+    switch (zpanel_class(max_m)) {
+    case 0: launch_grouped_panel<32, 1>(ctx, recs, count, max_m); break;
+    case 1: launch_grouped_panel<16, 2>(ctx, recs, count, max_m); break;
+    case 2: launch_grouped_panel<8, 4>(ctx, recs, count, max_m); break;
+    case 3: launch_grouped_panel<4, 8>(ctx, recs, count, max_m); break;
+    default: fail(TNS_EINVAL, "a panel taller than the register kernel holds");
+    }
+    TNS_HIP(hipGetLastError());
+}
+
+int grouped_moves_columns(int max_moves) { return max_moves <= 32 ? 256 / 32 : 256 / 64; }
+
+void zgrouped_moves(Context* ctx, const MovesRec<zd>* recs, int count, int blocks, int max_moves)
+{
+    if (count <= 0 || blocks <= 0) return;
+    if (max_moves <= 32)
+        hipLaunchKernelGGL((apply_moves_grouped_kernel<zd, 32>), dim3(blocks), dim3(256), 0, ctx->stream, recs, count);
+    else if (max_moves <= 64)
+        hipLaunchKernelGGL((apply_moves_grouped_kernel<zd, 64>), dim3(blocks), dim3(256), 0, ctx->stream, recs, count);
+    else
+        fail(TNS_EINVAL, "a move list longer than the grouped kernel holds");
+    TNS_HIP(hipGetLastError());
+}
+
+void zgrouped_trsm(Context* ctx, const TrsmRec<zd>* recs, int count, int blocks)
+{
+    if (count <= 0 || blocks <= 0) return;
+    hipLaunchKernelGGL((trsm_grouped_kernel<zd>), dim3(blocks), dim3(256), 0, ctx->stream, recs, count);
+    TNS_HIP(hipGetLastError());
+}
+
+namespace {
+// Static LDS of getrs_grouped_kernel beside its right-hand sides: the two
+// solves' partial sums (G x diagonal block each), with room to spare.
+constexpr size_t GROUPED_SOLVE_STATIC = 2 * 4 * 32 * sizeof(zd) + 1024;
+}  // namespace
+
+int grouped_solve_group(Context* ctx, int nmax)
+{
+    for (int g : {4, 2, 1})
+        if ((size_t)g * nmax * sizeof(zd) + GROUPED_SOLVE_STATIC <= (size_t)ctx->max_lds) return g;
+    return 0;
+}
+
+void zgrouped_solve(Context* ctx, const SolveRec<zd>* recs, int count, int trans, int group, int nmax)
+{
+    if (count <= 0) return;
+    const size_t bytes = (size_t)group * (nmax > 0 ? nmax : 1) * sizeof(zd);
+#define TNS_GS(TRANS_, G_)                                                                                         \
+    hipLaunchKernelGGL((getrs_grouped_kernel<zd, TRANS_, G_>), dim3(count), dim3(256), bytes, ctx->stream, recs)
+    if (!trans) {
+        if (group >= 4) TNS_GS(0, 4);
+        else if (group == 2) TNS_GS(0, 2);
+        else TNS_GS(0, 1);
+    } else {
+        if (group >= 4) TNS_GS(1, 4);
+        else if (group == 2) TNS_GS(1, 2);
+        else TNS_GS(1, 1);
+    }
+#undef TNS_GS
+    TNS_HIP(hipGetLastError());
 }
 
 }  // namespace tns

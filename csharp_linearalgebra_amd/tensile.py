@@ -443,7 +443,8 @@ class ExpmvFunction:
 
 class BatchedSystems:
     """Many complex systems Z_i X_i = B_i of uneven order, factored and
-    solved in one launch of a captured hipGraph (the frequency sweep). The
+    solved in one launch of a captured hipGraph of grouped kernels, each
+    block step of every system's LU in one launch (the frequency sweep). The
     matrices must be device memory (DeviceMatrix); solve() overwrites each
     Z_i with its LU factors and each B_i with X_i. tune: 0 defaults, 1 the
     workspace's remembered choice for this batch shape (tuning on scratch
@@ -494,6 +495,7 @@ class BatchedSystems:
 
     @property
     def config(self):
+        """(LU panel width, grouped GEMM tile, the tuner's best graph ms)."""
         s, g, t = C.c_int(), C.c_int(), C.c_double()
         self._ws.check(self._ws._lib.tns_zbatch_info(self._h, C.byref(s), C.byref(g), C.byref(t)))
         return s.value, g.value, t.value

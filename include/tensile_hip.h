@@ -292,10 +292,10 @@ TNS_API int tns_zbatch_create(tns_context* ctx, int count, const int* n, tns_com
 TNS_API int tns_zbatch_solve(tns_batch* batch, int* singular_columns);
 TNS_API int tns_zbatch_rcond(tns_batch* batch, int columns, double* rcond);
 TNS_API int tns_zbatch_pivots(tns_batch* batch, int index, int* pivots);
-/* What the autotuner chose: streams the capture forks over, the 4M GEMM tile
- * (-1 the heuristic), and the best graph time it measured (0 if it did not
- * run). */
-TNS_API int tns_zbatch_info(tns_batch* batch, int* streams, int* gemm_config, double* tuned_ms);
+/* What the autotuner chose: the LU panel width (16 or 32), the tile of the
+ * grouped 4M trailing update (1: 64 x 64, 2: 64 x 32), and the best graph
+ * time it measured (0 if it did not run). No counterpart in the reference. */
+TNS_API int tns_zbatch_info(tns_batch* batch, int* block, int* gemm_config, double* tuned_ms);
 TNS_API int tns_zbatch_destroy(tns_batch* batch);
 
 #ifdef __cplusplus

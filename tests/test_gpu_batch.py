@@ -1,5 +1,5 @@
 """Batched frequency sweep: many complex systems of uneven order factored
-and solved in one captured-graph launch, rcond for each. Per system the
+and solved in one captured-graph launch of grouped kernels, rcond for each. Per system the
 result is BlockedLu.Factor + Solve (BlockedLu.cs:45-206) and
 Condition.ReciprocalOne (NormEstimate.cs:432-466), checked against the
 oracle's restatement of the same: identical pivots, solutions within the
@@ -47,8 +47,8 @@ def test_sweep_matches_the_oracle(ws, tune):
     plan = ws.batched_systems(dm, db, tune=tune)
     sing = plan.solve()
     rc = plan.reciprocal_condition()
-    streams, cfg, tuned = plan.config
-    assert streams in (1, 4, 8, 14, 16, 32, 64) and cfg in (-1, 0, 1, 2)
+    block, cfg, tuned = plan.config
+    assert block in (16, 32) and cfg in (1, 2)
     if tune == 2:
         assert tuned > 0.0
     for i, (a, b) in enumerate(zip(mats, rhs)):
@@ -62,6 +62,40 @@ def test_sweep_matches_the_oracle(ws, tune):
         assert np.abs(a @ x - b).max() <= bound, i
         rc_ref = O.gecon(lu)
         assert np.abs(x - ref).max() <= 64 * n * np.finfo(float).eps / rc_ref * np.abs(ref).max(), i
+        assert abs(rc[i] - rc_ref) <= 1e-10 * rc_ref, (i, rc[i], rc_ref)
+    plan.close()
+
+
+@pytest.mark.parametrize("tune", [0, 2])
+def test_tall_panels_in_a_batch(ws, tune):
+    """Orders whose panels are factored in column chunks (above 512 rows the
+    register kernel holds 16 columns, above 1024 8) beside small ones, and
+    systems of equal order: every grouped launch class at once."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    orders = [1030, 700, 513, 512, 64, 64, 33, 1, 2, 17]
+    mats = []
+    for n in orders:
+        w = rng.uniform(0.5, 2.0)
+        mats.append(np.asfortranarray(rng.uniform(-1, 1, (n, n)) + 1j * w * rng.uniform(-1, 1, (n, n))))
+    rhs = [np.asfortranarray(rng.uniform(-1, 1, (n, 3)) + 1j * rng.uniform(-1, 1, (n, 3))) for n in orders]
+    dm = [to_dev(m) for m in mats]
+    db = [to_dev(b) for b in rhs]
+    torch.cuda.synchronize()
+    plan = ws.batched_systems(dm, db, tune=tune)
+    sing = plan.solve()
+    rc = plan.reciprocal_condition()
+    for i, (a, b) in enumerate(zip(mats, rhs)):
+        lu = O.getrf(a)
+        assert sing[i] == lu.singular_column == -1
+        assert np.array_equal(plan.pivots(i), lu.pivots), i
+        x = from_dev(db[i])
+        n = a.shape[0]
+        ref = O.getrs(lu, b)
+        bound = 16 * n * np.finfo(float).eps * np.abs(a).sum(1).max() * np.abs(ref).max()
+        assert np.abs(a @ x - b).max() <= bound, i
+        rc_ref = O.gecon(lu)
         assert abs(rc[i] - rc_ref) <= 1e-10 * rc_ref, (i, rc[i], rc_ref)
     plan.close()
 
