@@ -19,6 +19,8 @@
 // remapped so each XCD receives a contiguous band of tiles, grouped 8 tile
 // rows at a time, and the A row-panels and B column-panels a band shares
 // stay in that XCD's L2.
+#include <type_traits>
+
 #include "grouped.hpp"
 #include "tile_order.hpp"
 #include "tns_internal.hpp"
@@ -224,21 +226,60 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
 
     // D block layout of v_mfma_f64_16x16x4_f64 (one-element groups): lane l,
     // register r holds row 4*r + l/16, column l%16 of the 16x16 block.
+    // C is written in groups of GB blocks: the group's old values are all
+    // read first, so their loads are in flight together, and every new value
+    // is formed before the first store (a read-modify-write per element
+    // waits out one memory latency each, which dominates the thin-k updates
+    // of a factorization). Tiles inside C take straight-line code: with
+    // per-element branches between them, every store waits for the one
+    // before it.
+    auto write_back = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        constexpr int NBLK = S::TM * S::TN;
+        constexpr int GB = NBLK < 4 ? NBLK : 4;
 #pragma unroll
-    for (int i = 0; i < S::TM; i++)
+        for (int g0 = 0; g0 < NBLK; g0 += GB) {
+            double val[GB][4];
 #pragma unroll
-        for (int j = 0; j < S::TN; j++) {
-            const int col = n0 + wn * WN + j * 16 + (lane & 15);
-            if (col >= n) continue;
+            for (int g = 0; g < GB; g++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
-                if (row >= m) continue;
-                double* p = C + row + (size_t)col * ldc;
-                double v = alpha * acc[i][j][r];
-                *p = beta == 0.0 ? v : beta * *p + v;
+                for (int r = 0; r < 4; r++) val[g][r] = 0.0;
+            if (beta != 0.0) {
+#pragma unroll
+                for (int g = 0; g < GB; g++) {
+                    const int i = (g0 + g) / S::TN, j = (g0 + g) % S::TN;
+                    const int col = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
+                        const double* p = C + row + (size_t)col * ldc;
+                        val[g][r] = FULL ? *p : load_if(p, row < m && col < n);
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < GB; g++) {
+                const int i = (g0 + g) / S::TN, j = (g0 + g) % S::TN;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const double v = alpha * acc[i][j][r];
+                    val[g][r] = beta == 0.0 ? v : beta * val[g][r] + v;
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < GB; g++) {
+                const int i = (g0 + g) / S::TN, j = (g0 + g) % S::TN;
+                const int col = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
+                    if (FULL || (row < m && col < n)) C[row + (size_t)col * ldc] = val[g][r];
+                }
             }
         }
+    };
+    if (m0 + BM <= m && n0 + BN <= n) write_back(std::true_type{});
+    else write_back(std::false_type{});
 }
 
 // Complex, with ComplexKernels.Combine's scalar special cases folded into
@@ -353,30 +394,65 @@ __device__ __forceinline__ void zgemm_tile(int m, int n, int k, double ar_, doub
     }
 
     const bool unit_alpha = flags & 1, real_alpha = flags & 2, zero_beta = flags & 4, unit_beta = flags & 8;
+    // Written as in dgemm_kernel: groups of GB blocks, old C read first,
+    // new values formed before the first store, straight-line code for
+    // tiles inside C.
+    auto write_back = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        constexpr int NBLK = S::TM * S::TN;
+        constexpr int GB = NBLK < 2 ? NBLK : 2;
 #pragma unroll
-    for (int i = 0; i < S::TM; i++)
+        for (int g0 = 0; g0 < NBLK; g0 += GB) {
+            double vr[GB][4], vi[GB][4];
 #pragma unroll
-        for (int j = 0; j < S::TN; j++) {
-            const int col = n0 + wn * WN + j * 16 + (lane & 15);
-            if (col >= n) continue;
+            for (int g = 0; g < GB; g++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
-                if (row >= m) continue;
-                const double p = re[i][j][r], q = im[i][j][r];
-                double pr, pi;
-                if (unit_alpha) { pr = p; pi = q; }
-                else if (real_alpha) { pr = ar_ * p; pi = ar_ * q; }
-                else { pr = ar_ * p - ai_ * q; pi = ai_ * p + ar_ * q; }
-                double2* ptr = C + row + (size_t)col * ldc;
-                if (zero_beta) { *ptr = make_double2(pr, pi); }
-                else {
-                    double2 c = *ptr;
-                    if (unit_beta) *ptr = make_double2(c.x + pr, c.y + pi);
-                    else *ptr = make_double2((br_ * c.x - bi_ * c.y) + pr, (bi_ * c.x + br_ * c.y) + pi);
+                for (int r = 0; r < 4; r++) vr[g][r] = vi[g][r] = 0.0;
+            if (!zero_beta) {
+#pragma unroll
+                for (int g = 0; g < GB; g++) {
+                    const int i = (g0 + g) / S::TN, j = (g0 + g) % S::TN;
+                    const int col = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
+                        double2 c = make_double2(0.0, 0.0);
+                        if (FULL || (row < m && col < n)) c = C[row + (size_t)col * ldc];
+                        vr[g][r] = c.x;
+                        vi[g][r] = c.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < GB; g++) {
+                const int i = (g0 + g) / S::TN, j = (g0 + g) % S::TN;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const double p = re[i][j][r], q = im[i][j][r];
+                    double pr, pi;
+                    if (unit_alpha) { pr = p; pi = q; }
+                    else if (real_alpha) { pr = ar_ * p; pi = ar_ * q; }
+                    else { pr = ar_ * p - ai_ * q; pi = ai_ * p + ar_ * q; }
+                    const double cr = vr[g][r], ci = vi[g][r];
+                    if (zero_beta) { vr[g][r] = pr; vi[g][r] = pi; }
+                    else if (unit_beta) { vr[g][r] = cr + pr; vi[g][r] = ci + pi; }
+                    else { vr[g][r] = (br_ * cr - bi_ * ci) + pr; vi[g][r] = (bi_ * cr + br_ * ci) + pi; }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < GB; g++) {
+                const int i = (g0 + g) / S::TN, j = (g0 + g) % S::TN;
+                const int col = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = m0 + wm * WM + i * 16 + 4 * r + kq;
+                    if (FULL || (row < m && col < n)) C[row + (size_t)col * ldc] = make_double2(vr[g][r], vi[g][r]);
                 }
             }
         }
+    };
+    if (m0 + BM <= m && n0 + BN <= n) write_back(std::true_type{});
+    else write_back(std::false_type{});
 }
 
 template <int BM, int BN, int BK, int WM, int WN, int TA>
