@@ -11,9 +11,11 @@
 // work of one launch spreads over the whole chip instead of one system's
 // panel occupying one CU while the rest wait, and the graph has a few
 // hundred nodes instead of a hundred per system. Each system's part of a
-// step is exactly what its own factorization would launch (getrf_impl and
-// panel() in lu.hip): the same panel chunks, move lists, solves and
-// products, so pivots and factors are those of the one-system path.
+// step is the right-looking block step of its own factorization (getrf_impl
+// and panel() in lu.hip): the same panel kernel, chunk widths and move
+// lists; the block row U12 = L11^-1 A12 is solved in diagonal blocks of 32
+// with a product between them (trsm_impl's blocked order), and the trailing
+// update is one grouped 4M product.
 //
 // The reciprocal condition numbers (Condition.ReciprocalOne,
 // NormEstimate.cs:432-466) run the estimator of every system in lockstep:
@@ -21,7 +23,7 @@
 // panels in, one grouped solve of all systems, one copy out), then the host
 // logic of every estimate advances.
 //
-// The autotuner times the graph for panel widths {16, 32} and two grouped
+// The autotuner times the graph for panel widths {32, 64} and two grouped
 // GEMM tiles on scratch copies of the systems and keeps the fastest, per
 // context and batch shape.
 #include <algorithm>
@@ -101,12 +103,15 @@ void Schedule::record(Context* ctx) const
 {
     for (const Launch& l : launches) {
         const void* p = dev.get() + l.offset;
+        const int* starts = reinterpret_cast<const int*>(dev.get() + l.starts);
         switch (l.kind) {
         case INIT: zgrouped_init(ctx, static_cast<const InitRec<zd>*>(p), l.count); break;
         case PANEL: zgrouped_panel(ctx, static_cast<const PanelRec<zd>*>(p), l.count, l.param); break;
-        case MOVES: zgrouped_moves(ctx, static_cast<const MovesRec<zd>*>(p), l.count, l.blocks, l.param); break;
-        case TRSM: zgrouped_trsm(ctx, static_cast<const TrsmRec<zd>*>(p), l.count, l.blocks); break;
-        case GEMM: zgrouped_gemm(ctx, static_cast<const GemmRec<zd>*>(p), l.count, l.blocks, l.param); break;
+        case MOVES:
+            zgrouped_moves(ctx, static_cast<const MovesRec<zd>*>(p), starts, l.count, l.blocks, l.param);
+            break;
+        case TRSM: zgrouped_trsm(ctx, static_cast<const TrsmRec<zd>*>(p), starts, l.count, l.blocks); break;
+        case GEMM: zgrouped_gemm(ctx, static_cast<const GemmRec<zd>*>(p), starts, l.count, l.blocks, l.param); break;
         case SOLVE:
             zgrouped_solve(ctx, static_cast<const SolveRec<zd>*>(p), l.count, l.param, l.param2, l.param3);
             break;
@@ -156,8 +161,9 @@ void BatchPlan::plan_solve(Schedule& s, const std::vector<zd*>& A, const std::ve
     s.launches.clear();
     s.host.clear();
     if (order.empty()) return;
-    auto add = [&](int kind, size_t off, int cnt, int blocks, int param, int param2 = 0, int param3 = 0) {
-        if (cnt > 0) s.launches.push_back({kind, off, cnt, blocks, param, param2, param3});
+    auto add = [&](int kind, size_t off, int cnt, int blocks, int param, int param2 = 0, int param3 = 0,
+                   size_t starts = 0) {
+        if (cnt > 0) s.launches.push_back({kind, off, cnt, blocks, param, param2, param3, starts});
     };
     const int maxn = n[order[0]];
     int* d_piv = piv.get();
@@ -173,16 +179,21 @@ void BatchPlan::plan_solve(Schedule& s, const std::vector<zd*>& A, const std::ve
         for (int i : order) init.push_back({A[i], d_perm + offset[i], d_info + i, d_anorm + i, LDA[i], n[i]});
         add(Schedule::INIT, s.put(init), (int)init.size(), 0, 0);
     }
-    auto emit_trsm_gemm = [&](std::vector<TrsmRec<zd>>& tr, std::vector<GemmRec<zd>>& gm) {
+    auto emit_trsm = [&](std::vector<TrsmRec<zd>>& tr) {
         const int tb = assign_starts(tr, [](const TrsmRec<zd>& r) {
             return (r.ncols + GROUPED_TRSM_COLUMNS - 1) / GROUPED_TRSM_COLUMNS;
         });
-        add(Schedule::TRSM, s.put(tr), (int)tr.size(), tb, 0);
+        const size_t at = s.put(tr);
+        add(Schedule::TRSM, at, (int)tr.size(), tb, 0, 0, 0, s.put_starts(tr));
+    };
+    auto emit_gemm = [&](std::vector<GemmRec<zd>>& gm) {
         const int gb = assign_starts(gm, [&](const GemmRec<zd>& r) {
             return ((r.m + bm - 1) / bm) * ((r.n + bn - 1) / bn);
         });
-        add(Schedule::GEMM, s.put(gm), (int)gm.size(), gb, cfg);
+        const size_t at = s.put(gm);
+        add(Schedule::GEMM, at, (int)gm.size(), gb, cfg, 0, 0, s.put_starts(gm));
     };
+    constexpr int TB = 32;  // diagonal block of the complex triangular solves
 
     for (int jb = 0; jb < maxn; jb += nb) {
         for (int c0 = 0; c0 < nb; c0 += 4) {
@@ -225,11 +236,35 @@ void BatchPlan::plan_solve(Schedule& s, const std::vector<zd*>& A, const std::ve
                 const int blocks = assign_starts(mv, [&](const MovesRec<zd>& r) {
                     return (r.c0 + (r.ncols - r.c1) + cpb - 1) / cpb;
                 });
-                add(Schedule::MOVES, s.put(mv), (int)mv.size(), blocks, longest);
+                const size_t at = s.put(mv);
+                add(Schedule::MOVES, at, (int)mv.size(), blocks, longest, 0, 0, s.put_starts(mv));
             }
-            emit_trsm_gemm(tr, gm);
+            emit_trsm(tr);
+            emit_gemm(gm);
         }
-        std::vector<TrsmRec<zd>> tr;
+        // U12 := L11^-1 A12 by diagonal blocks of TB rows, each solved and
+        // then subtracted from the rows below it (trsm_impl's order), then
+        // A22 -= A21 U12.
+        for (int q0 = 0; q0 < nb; q0 += TB) {
+            std::vector<TrsmRec<zd>> tr;
+            std::vector<GemmRec<zd>> gm;
+            for (int i : order) {
+                const int ni = n[i];
+                if (ni <= jb) break;
+                const int m = ni - jb, w = nb < m ? nb : m;
+                if (jb + w >= ni || q0 >= w) continue;
+                const int kb = TB < w - q0 ? TB : w - q0;
+                const int ld = LDA[i];
+                zd* top = A[i] + jb + (size_t)jb * ld;
+                zd* bq = top + q0 + (size_t)w * ld;
+                tr.push_back({top + q0 + (size_t)q0 * ld, bq, ld, ld, kb, ni - jb - w, 0});
+                if (q0 + kb < w)
+                    gm.push_back({top + q0 + kb + (size_t)q0 * ld, bq, bq + kb, ld, ld, ld, w - q0 - kb, ni - jb - w, kb,
+                                  0});
+            }
+            emit_trsm(tr);
+            emit_gemm(gm);
+        }
         std::vector<GemmRec<zd>> gm;
         for (int i : order) {
             const int ni = n[i];
@@ -238,10 +273,9 @@ void BatchPlan::plan_solve(Schedule& s, const std::vector<zd*>& A, const std::ve
             if (jb + w >= ni) continue;
             const int ld = LDA[i];
             zd* top = A[i] + jb + (size_t)jb * ld;
-            tr.push_back({top, top + (size_t)w * ld, ld, ld, w, ni - jb - w, 0});
             gm.push_back({top + w, top + (size_t)w * ld, top + w + (size_t)w * ld, ld, ld, ld, m - w, ni - jb - w, w, 0});
         }
-        emit_trsm_gemm(tr, gm);
+        emit_gemm(gm);
     }
 
     std::vector<SolveRec<zd>> sv;
@@ -300,7 +334,7 @@ void BatchPlan::tune()
     };
     double best = 1e300;
     int best_nb = 32, best_cfg = 2;
-    for (int nb : {16, 32})
+    for (int nb : {32, 64})
         for (int cfg : {1, 2}) {
             std::vector<std::pair<void*, size_t>> hold;
             Schedule sched;
@@ -376,7 +410,7 @@ void BatchPlan::build_rcond(int t)
         }
         s.launches.clear();
         s.host.clear();
-        if (!sv.empty()) s.launches.push_back({Schedule::SOLVE, s.put(sv), (int)sv.size(), 0, trans, group, maxn});
+        if (!sv.empty()) s.launches.push_back({Schedule::SOLVE, s.put(sv), (int)sv.size(), 0, trans, group, maxn, 0});
         s.upload(ctx);
         return capture(ctx, &rc_held, [&]() {
             TNS_HIP(hipMemcpyAsync(d_panels.get(), h_panels, sizeof(zd) * total, hipMemcpyHostToDevice, ctx->stream));

@@ -19,10 +19,11 @@ struct Schedule {
         int kind;
         size_t offset;  // of the records, in bytes
         int count;      // records
-        int blocks;     // workgroups of a flat launch
+        int blocks;     // workgroups (GEMM: tiles) of a flat launch
         int param;      // PANEL: tallest m; MOVES: longest list; GEMM: tile; SOLVE: trans
         int param2;     // SOLVE: right-hand sides per workgroup pass; others 0
         int param3;     // SOLVE: largest order
+        size_t starts;  // flat launches: offset of the records' first workgroups (ints)
     };
     std::vector<Launch> launches;
     std::vector<unsigned char> host;
@@ -35,6 +36,15 @@ struct Schedule {
         host.resize(at + sizeof(R) * recs.size());
         if (!recs.empty()) std::memcpy(host.data() + at, recs.data(), sizeof(R) * recs.size());
         return at;
+    }
+    // The records' `start` fields as an int array (flat launches).
+    template <typename R>
+    size_t put_starts(const std::vector<R>& recs)
+    {
+        std::vector<int> st;
+        st.reserve(recs.size());
+        for (const R& r : recs) st.push_back(r.start);
+        return put(st);
     }
     // Copy the records to the device (before any capture that uses them).
     void upload(Context* ctx);

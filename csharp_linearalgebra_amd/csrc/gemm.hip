@@ -468,27 +468,32 @@ zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restr
 }
 
 // C_g -= A_g B_g for many independent products of different shapes in one
-// launch (the trailing updates of a batch of LUs): record g owns workgroups
-// [start_g, start_g + tiles_g), its tiles in column-major order. alpha = -1
-// and beta = 1 take the same write-back as zgemm_ex gives them.
+// launch (the trailing updates of a batch of LUs). Record g owns tiles
+// [starts[g], starts[g + 1]), in column-major order; each workgroup walks
+// the tiles t = blockIdx.x, blockIdx.x + gridDim.x, ..., so the grid is
+// sized to the chip, not to the tiles, and the record search moves forward
+// only. alpha = -1 and beta = 1 take the same write-back as zgemm_ex gives
+// them.
 template <int BM, int BN, int BK, int WM, int WN>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, 2)
-zgemm_grouped_kernel(const GemmRec<zd>* __restrict__ recs, int count)
+zgemm_grouped_kernel(const GemmRec<zd>* __restrict__ recs, const int* __restrict__ starts, int count, int total)
 {
-    int lo = 0, hi = count - 1;
-    const int bid = blockIdx.x;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (recs[mid].start <= bid) lo = mid;
-        else hi = mid - 1;
+    int g = 0;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int hi = count - 1;
+        while (g < hi) {
+            const int mid = (g + hi + 1) >> 1;
+            if (starts[mid] <= t) g = mid;
+            else hi = mid - 1;
+        }
+        const GemmRec<zd> r = recs[g];
+        const int local = t - starts[g];
+        const int tiles_m = (r.m + BM - 1) / BM;
+        zgemm_tile<BM, BN, BK, WM, WN, 0>(r.m, r.n, r.k, -1.0, 0.0, reinterpret_cast<const double2*>(r.a), r.lda,
+                                          reinterpret_cast<const double2*>(r.b), r.ldb, 1.0, 0.0,
+                                          reinterpret_cast<double2*>(r.c), r.ldc, 2 | 8, (local % tiles_m) * BM,
+                                          (local / tiles_m) * BN);
     }
-    const GemmRec<zd> r = recs[lo];
-    const int local = bid - r.start;
-    const int tiles_m = (r.m + BM - 1) / BM;
-    zgemm_tile<BM, BN, BK, WM, WN, 0>(r.m, r.n, r.k, -1.0, 0.0, reinterpret_cast<const double2*>(r.a), r.lda,
-                                      reinterpret_cast<const double2*>(r.b), r.ldb, 1.0, 0.0,
-                                      reinterpret_cast<double2*>(r.c), r.ldc, 2 | 8, (local % tiles_m) * BM,
-                                      (local / tiles_m) * BN);
 }
 
 __global__ void dscale_kernel(int m, int n, double beta, double* __restrict__ c, int ldc)
@@ -635,15 +640,18 @@ void grouped_gemm_tile(int cfg, int& bm, int& bn)
     bn = cfg == 1 ? 64 : 32;
 }
 
-void zgrouped_gemm(Context* ctx, const GemmRec<zd>* recs, int count, int blocks, int cfg)
+void zgrouped_gemm(Context* ctx, const GemmRec<zd>* recs, const int* starts, int count, int blocks, int cfg)
 {
     if (count <= 0 || blocks <= 0) return;
+    // Resident workgroups: 256 CUs, 2 (64 x 64) or 4 (64 x 32) per CU.
+    const int resident = cfg == 1 ? 512 : 1024;
+    const int grid = blocks < resident ? blocks : resident;
     if (cfg == 1)
-        hipLaunchKernelGGL((zgemm_grouped_kernel<64, 64, 16, 32, 32>), dim3(blocks), dim3(256), 0, ctx->stream, recs,
-                           count);
+        hipLaunchKernelGGL((zgemm_grouped_kernel<64, 64, 16, 32, 32>), dim3(grid), dim3(256), 0, ctx->stream, recs,
+                           starts, count, blocks);
     else
-        hipLaunchKernelGGL((zgemm_grouped_kernel<64, 32, 16, 32, 16>), dim3(blocks), dim3(256), 0, ctx->stream, recs,
-                           count);
+        hipLaunchKernelGGL((zgemm_grouped_kernel<64, 32, 16, 32, 16>), dim3(grid), dim3(256), 0, ctx->stream, recs,
+                           starts, count, blocks);
     TNS_HIP(hipGetLastError());
 }
 

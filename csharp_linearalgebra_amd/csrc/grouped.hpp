@@ -77,7 +77,8 @@ struct SolveRec {
 };
 
 // Launches (lu.hip, gemm.hip). `recs` is device memory; `count` records;
-// `blocks` the total workgroups of a flat launch.
+// `blocks` the total workgroups (GEMM: tiles) of a flat launch, and
+// `starts` (device, count ints) each record's first one.
 void zgrouped_init(Context* ctx, const InitRec<zd>* recs, int count);
 // The register panel kernel for height m: its column-chunk width (0 if no
 // register kernel holds m rows) and its variant (records of one launch share
@@ -86,12 +87,12 @@ int zpanel_chunk_width(int m);
 int zpanel_class(int m);
 void zgrouped_panel(Context* ctx, const PanelRec<zd>* recs, int count, int max_m);
 int grouped_moves_columns(int max_moves);  // columns per workgroup
-void zgrouped_moves(Context* ctx, const MovesRec<zd>* recs, int count, int blocks, int max_moves);
+void zgrouped_moves(Context* ctx, const MovesRec<zd>* recs, const int* starts, int count, int blocks, int max_moves);
 constexpr int GROUPED_TRSM_COLUMNS = 32;
-void zgrouped_trsm(Context* ctx, const TrsmRec<zd>* recs, int count, int blocks);
+void zgrouped_trsm(Context* ctx, const TrsmRec<zd>* recs, const int* starts, int count, int blocks);
 // Tile of grouped GEMM configuration cfg (1: 64 x 64, 2: 64 x 32).
 void grouped_gemm_tile(int cfg, int& bm, int& bn);
-void zgrouped_gemm(Context* ctx, const GemmRec<zd>* recs, int count, int blocks, int cfg);
+void zgrouped_gemm(Context* ctx, const GemmRec<zd>* recs, const int* starts, int count, int blocks, int cfg);
 // Right-hand-side columns held in LDS at once for systems up to order nmax
 // (0: too large for one workgroup's LDS).
 int grouped_solve_group(Context* ctx, int nmax);

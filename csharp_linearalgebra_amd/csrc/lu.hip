@@ -572,23 +572,23 @@ apply_moves_kernel(T* __restrict__ a, int lda, int c0, int c1, int ncols, const 
 }
 
 // Index of the record whose workgroup range holds `bid`: the last record
-// with start <= bid (records are in increasing start order, start[0] = 0).
-template <typename Rec>
-__device__ __forceinline__ int find_record(const Rec* __restrict__ recs, int count, int bid)
+// with starts[g] <= bid (starts increasing, starts[0] = 0).
+__device__ __forceinline__ int find_record(const int* __restrict__ starts, int count, int bid)
 {
     int lo = 0, hi = count - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (recs[mid].start <= bid) lo = mid;
+        if (starts[mid] <= bid) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
 
 template <typename T, int QM>
-__global__ void __launch_bounds__(256) apply_moves_grouped_kernel(const MovesRec<T>* __restrict__ recs, int count)
+__global__ void __launch_bounds__(256)
+apply_moves_grouped_kernel(const MovesRec<T>* __restrict__ recs, const int* __restrict__ starts, int count)
 {
-    const MovesRec<T> r = recs[find_record(recs, count, blockIdx.x)];
+    const MovesRec<T> r = recs[find_record(starts, count, blockIdx.x)];
     apply_moves_body<T, QM>(r.a, r.lda, r.c0, r.c1, r.ncols, r.list, blockIdx.x - r.start);
 }
 
@@ -694,9 +694,10 @@ trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restr
 
 // U12 := L11^-1 A12 of many matrices in one launch.
 template <typename T>
-__global__ void __launch_bounds__(256) trsm_grouped_kernel(const TrsmRec<T>* __restrict__ recs, int count)
+__global__ void __launch_bounds__(256)
+trsm_grouped_kernel(const TrsmRec<T>* __restrict__ recs, const int* __restrict__ starts, int count)
 {
-    const TrsmRec<T> r = recs[find_record(recs, count, blockIdx.x)];
+    const TrsmRec<T> r = recs[find_record(starts, count, blockIdx.x)];
     trsm_block_body<T, tri_block<T>(), GROUPED_TRSM_COLUMNS, true, 0, true>(r.w, r.ncols, r.l, r.ldl, r.b, r.ldb,
                                                                           blockIdx.x - r.start);
 }
@@ -860,6 +861,7 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
             for (int i = wave; i < kb; i += 4) {
                 T s0 = s_zero(T{}), s1 = s_zero(T{});
                 int k = lo + lane;
+#pragma unroll 4
                 for (; k + 64 < hi; k += 128) {
                     s0 = s_add(s0, s_mul(at(k0 + i, k), xs[k]));
                     s1 = s_add(s1, s_mul(at(k0 + i, k + 64), xs[k + 64]));
@@ -894,21 +896,35 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
         __syncthreads();
         if (TR == 0) {
             // Right-looking: push this block into the rows still to come, in
-            // the reference's axpy order, two rows at a time.
+            // the reference's axpy order, two rows at a time, the next U
+            // entries of each row loaded together before they are applied.
+            constexpr int U = sizeof(T) == 16 ? 4 : 8;
             const int r0 = FORWARD ? k0 + kb : 0;
             const int r1 = FORWARD ? n : k0;
             for (int i = r0 + tid; i < r1; i += 512) {
                 const int i2 = i + 256;
                 const bool two = i2 < r1;
                 T acc = xs[i];
-                T acc2 = two ? xs[i2] : s_zero(T{});
-                for (int q = 0; q < kb; q++) {
-                    const int p = FORWARD ? q : kb - 1 - q;
-                    const T xp = xs[k0 + p];
-                    if (s_iszero(xp)) continue;
-                    const T nx = s_neg(xp);
-                    acc = s_add(acc, s_mul(nx, at(i, k0 + p)));
-                    if (two) acc2 = s_add(acc2, s_mul(nx, at(i2, k0 + p)));
+                T acc2 = load_if(xs + i2, two);
+                for (int q = 0; q < kb; q += U) {
+                    T av[U], av2[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int p = FORWARD ? q + u : kb - 1 - (q + u);
+                        const bool in = q + u < kb;
+                        av[u] = load_if(A + i + (size_t)(k0 + p) * lda, in);
+                        av2[u] = load_if(A + i2 + (size_t)(k0 + p) * lda, in && two);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (q + u >= kb) break;
+                        const int p = FORWARD ? q + u : kb - 1 - (q + u);
+                        const T xp = xs[k0 + p];
+                        if (s_iszero(xp)) continue;
+                        const T nx = s_neg(xp);
+                        acc = s_add(acc, s_mul(nx, av[u]));
+                        if (two) acc2 = s_add(acc2, s_mul(nx, av2[u]));
+                    }
                 }
                 xs[i] = acc;
                 if (two) xs[i2] = acc2;
@@ -951,6 +967,7 @@ __device__ void trsv_lds_group(int n, const T* __restrict__ A, int lda, T* xs, i
 #pragma unroll
                 for (int c = 0; c < G; c++) s0[c] = s1[c] = zero;
                 int k = lo + lane;
+#pragma unroll 4
                 for (; k + 64 < hi; k += 128) {
                     const T a0 = at(k0 + i, k), a1 = at(k0 + i, k + 64);
 #pragma unroll
@@ -1009,18 +1026,29 @@ __device__ void trsv_lds_group(int n, const T* __restrict__ A, int lda, T* xs, i
                     acc[c] = load_if(xs + c * n + i, c < nr);
                     acc2[c] = load_if(xs + c * n + i2, c < nr && two);
                 }
-                for (int q = 0; q < kb; q++) {
-                    const int p = FORWARD ? q : kb - 1 - q;
-                    const T ai = A[i + (size_t)(k0 + p) * lda];
-                    const T ai2 = load_if(A + i2 + (size_t)(k0 + p) * lda, two);
+                constexpr int U = 4;
+                for (int q = 0; q < kb; q += U) {
+                    T av[U], av2[U];
 #pragma unroll
-                    for (int c = 0; c < G; c++) {
-                        if (c >= nr) continue;
-                        const T xp = xs[c * n + k0 + p];
-                        if (s_iszero(xp)) continue;
-                        const T nx = s_neg(xp);
-                        acc[c] = s_add(acc[c], s_mul(nx, ai));
-                        if (two) acc2[c] = s_add(acc2[c], s_mul(nx, ai2));
+                    for (int u = 0; u < U; u++) {
+                        const int p = FORWARD ? q + u : kb - 1 - (q + u);
+                        const bool in = q + u < kb;
+                        av[u] = load_if(A + i + (size_t)(k0 + p) * lda, in);
+                        av2[u] = load_if(A + i2 + (size_t)(k0 + p) * lda, in && two);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (q + u >= kb) break;
+                        const int p = FORWARD ? q + u : kb - 1 - (q + u);
+#pragma unroll
+                        for (int c = 0; c < G; c++) {
+                            if (c >= nr) continue;
+                            const T xp = xs[c * n + k0 + p];
+                            if (s_iszero(xp)) continue;
+                            const T nx = s_neg(xp);
+                            acc[c] = s_add(acc[c], s_mul(nx, av[u]));
+                            if (two) acc2[c] = s_add(acc2[c], s_mul(nx, av2[u]));
+                        }
                     }
                 }
 #pragma unroll
@@ -1083,17 +1111,27 @@ __device__ __forceinline__ double lu_nan_max(double a, double b)
 // sum it: lanes strided by 64, then a butterfly), perm = identity, info =
 // the no-zero-pivot sentinel. One workgroup per matrix.
 template <typename T>
-__global__ void __launch_bounds__(256) lu_init_grouped_kernel(const InitRec<T>* __restrict__ recs)
+__global__ void __launch_bounds__(1024) lu_init_grouped_kernel(const InitRec<T>* __restrict__ recs)
 {
-    __shared__ double red[4];
+    constexpr int W = 16;  // wavefronts
+    __shared__ double red[W];
     const InitRec<T> r = recs[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < r.n; i += 256) r.perm[i] = i;
+    for (int i = tid; i < r.n; i += 64 * W) r.perm[i] = i;
     double best = 0.0;
-    for (int j = wave; j < r.n; j += 4) {
+    for (int j = wave; j < r.n; j += W) {
         const T* col = r.a + (size_t)j * r.lda;
         double s = 0.0;
-        for (int i = lane; i < r.n; i += 64) s += s_abs(col[i]);
+        // Four entries loaded together, summed in row order.
+        int i = lane;
+        for (; i + 192 < r.n; i += 256) {
+            const T x0 = col[i], x1 = col[i + 64], x2 = col[i + 128], x3 = col[i + 192];
+            s += s_abs(x0);
+            s += s_abs(x1);
+            s += s_abs(x2);
+            s += s_abs(x3);
+        }
+        for (; i < r.n; i += 64) s += s_abs(col[i]);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
         best = lu_nan_max(best, s);
@@ -1102,7 +1140,7 @@ __global__ void __launch_bounds__(256) lu_init_grouped_kernel(const InitRec<T>* 
     __syncthreads();
     if (tid == 0) {
         double v = red[0];
-        for (int q = 1; q < 4; q++) v = lu_nan_max(v, red[q]);
+        for (int q = 1; q < W; q++) v = lu_nan_max(v, red[q]);
         *r.anorm = v;
         *r.info = INT_MAX;
     }
@@ -1545,7 +1583,7 @@ int zpanel_class(int m)
 void zgrouped_init(Context* ctx, const InitRec<zd>* recs, int count)
 {
     if (count <= 0) return;
-    hipLaunchKernelGGL((lu_init_grouped_kernel<zd>), dim3(count), dim3(256), 0, ctx->stream, recs);
+    hipLaunchKernelGGL((lu_init_grouped_kernel<zd>), dim3(count), dim3(1024), 0, ctx->stream, recs);
     TNS_HIP(hipGetLastError());
 }
 
@@ -1575,22 +1613,24 @@ void zgrouped_panel(Context* ctx, const PanelRec<zd>* recs, int count, int max_m
 
 int grouped_moves_columns(int max_moves) { return max_moves <= 32 ? 256 / 32 : 256 / 64; }
 
-void zgrouped_moves(Context* ctx, const MovesRec<zd>* recs, int count, int blocks, int max_moves)
+void zgrouped_moves(Context* ctx, const MovesRec<zd>* recs, const int* starts, int count, int blocks, int max_moves)
 {
     if (count <= 0 || blocks <= 0) return;
     if (max_moves <= 32)
-        hipLaunchKernelGGL((apply_moves_grouped_kernel<zd, 32>), dim3(blocks), dim3(256), 0, ctx->stream, recs, count);
+        hipLaunchKernelGGL((apply_moves_grouped_kernel<zd, 32>), dim3(blocks), dim3(256), 0, ctx->stream, recs, starts,
+                           count);
     else if (max_moves <= 64)
-        hipLaunchKernelGGL((apply_moves_grouped_kernel<zd, 64>), dim3(blocks), dim3(256), 0, ctx->stream, recs, count);
+        hipLaunchKernelGGL((apply_moves_grouped_kernel<zd, 64>), dim3(blocks), dim3(256), 0, ctx->stream, recs, starts,
+                           count);
     else
         fail(TNS_EINVAL, "a move list longer than the grouped kernel holds");
     TNS_HIP(hipGetLastError());
 }
 
-void zgrouped_trsm(Context* ctx, const TrsmRec<zd>* recs, int count, int blocks)
+void zgrouped_trsm(Context* ctx, const TrsmRec<zd>* recs, const int* starts, int count, int blocks)
 {
     if (count <= 0 || blocks <= 0) return;
-    hipLaunchKernelGGL((trsm_grouped_kernel<zd>), dim3(blocks), dim3(256), 0, ctx->stream, recs, count);
+    hipLaunchKernelGGL((trsm_grouped_kernel<zd>), dim3(blocks), dim3(256), 0, ctx->stream, recs, starts, count);
     TNS_HIP(hipGetLastError());
 }
 

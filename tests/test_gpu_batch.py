@@ -48,7 +48,7 @@ def test_sweep_matches_the_oracle(ws, tune):
     sing = plan.solve()
     rc = plan.reciprocal_condition()
     block, cfg, tuned = plan.config
-    assert block in (16, 32) and cfg in (1, 2)
+    assert block in (32, 64) and cfg in (1, 2)
     if tune == 2:
         assert tuned > 0.0
     for i, (a, b) in enumerate(zip(mats, rhs)):
