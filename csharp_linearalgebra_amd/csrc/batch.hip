@@ -27,10 +27,15 @@
 // GEMM tiles on scratch copies of the systems and keeps the fastest, per
 // context and batch shape.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstring>
+#include <exception>
+#include <memory>
+#include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "algos.hpp"
@@ -75,6 +80,36 @@ hipGraphExec_t capture(Context* ctx, std::vector<std::pair<void*, size_t>>* hold
     (void)hipGraphDestroy(graph);
     TNS_HIP(e);
     return exec;
+}
+
+// fn(i) for i in [0, count) on up to 16 host threads (the estimates of a
+// batch advance independently). The first exception is rethrown here.
+template <typename F>
+void parallel_for(int count, F&& fn)
+{
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int workers = std::min(std::min(hw, 16), (count + 31) / 32);
+    if (workers <= 1) {
+        for (int i = 0; i < count; i++) fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::exception_ptr error;
+    std::mutex error_gate;
+    auto work = [&]() {
+        try {
+            for (int i; (i = next.fetch_add(1)) < count;) fn(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(error_gate);
+            if (!error) error = std::current_exception();
+            next.store(count);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; w++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (error) std::rethrow_exception(error);
 }
 
 // Flat launches: give each record its first workgroup; returns the total.
@@ -450,37 +485,41 @@ void BatchPlan::rcond(double* out, int t)
     TNS_HIP(hipStreamSynchronize(ctx->stream));
 
     const size_t total = rc_off[count];
-    std::vector<NormEstRun<zd>> runs;
-    runs.reserve(count);
-    for (int i = 0; i < count; i++) runs.emplace_back(n[i], t, 5, 20260915);
+    std::vector<std::unique_ptr<NormEstRun<zd>>> runs(count);
     std::vector<char> active(count);
-    for (int i = 0; i < count; i++) active[i] = sing[i] < 0 && an[i] != 0.0 && runs[i].want() != 0;
+    parallel_for(count, [&](int i) {
+        runs[i] = std::make_unique<NormEstRun<zd>>(n[i], t, 5, 20260915);
+        active[i] = sing[i] < 0 && an[i] != 0.0 && runs[i]->want() != 0;
+    });
     for (int round = 0;; round++) {
         bool any = false;
         const int kind = round % 2 == 0 ? 1 : 2;
         for (int i = 0; i < count; i++) {
             if (!active[i]) continue;
-            if (runs[i].want() != kind) fail(TNS_ESTATE, "estimates out of step");
-            std::memcpy(h_panels + rc_off[i], runs[i].input().data(), sizeof(zd) * runs[i].input().size());
+            if (runs[i]->want() != kind) fail(TNS_ESTATE, "estimates out of step");
             any = true;
         }
         if (!any) break;
+        parallel_for(count, [&](int i) {
+            if (active[i])
+                std::memcpy(h_panels + rc_off[i], runs[i]->input().data(), sizeof(zd) * runs[i]->input().size());
+        });
         TNS_HIP(hipGraphLaunch(kind == 1 ? apply_exec : adjoint_exec, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
-        for (int i = 0; i < count; i++) {
-            if (!active[i]) continue;
-            std::vector<zd>& o = runs[i].output();
+        parallel_for(count, [&](int i) {
+            if (!active[i]) return;
+            std::vector<zd>& o = runs[i]->output();
             std::memcpy(o.data(), h_panels + total + rc_off[i], sizeof(zd) * o.size());
-            runs[i].resume();
-            if (runs[i].want() == 0) active[i] = 0;
-        }
+            runs[i]->resume();
+            if (runs[i]->want() == 0) active[i] = 0;
+        });
     }
     for (int i = 0; i < count; i++) {
         if (sing[i] >= 0 || an[i] == 0.0 || n[i] == 0) {
             out[i] = 0.0;
             continue;
         }
-        const double est = runs[i].result().value;
+        const double est = runs[i]->result().value;
         out[i] = est == 0.0 ? 0.0 : 1.0 / (an[i] * est);
     }
 }
@@ -540,6 +579,9 @@ std::unique_ptr<BatchPlan> make_batch(Context* ctx, int count, const int* n, zd*
     } else {
         p->build(32, 2);
     }
+    // The estimator's graphs for the reference's two probe columns, built
+    // with the plan so rcond() only runs them.
+    if (count > 0) p->build_rcond(2);
     return p;
 }
 
