@@ -160,7 +160,9 @@ def mixed_leg(ws, n, kappas, nrhs=32):
             key = "fp32" if prec == 32 else "bf16"
             row[key] = {"ms": round(best * 1e3, 2), "speedup": round(t64 / best, 3),
                         "backward_error": inf.backward_error, "iterations": inf.iterations,
-                        "fell_back": inf.fell_back, "first_error": inf.first_error}
+                        "fell_back": inf.fell_back, "first_error": inf.first_error,
+                        "factor_ms": round(inf.factor_ms, 2), "refine_ms": round(inf.refine_ms, 2),
+                        "fallback_ms": round(inf.fallback_ms, 2)}
         out.append(row)
         del a, b, at, bt, f, x64
         torch.cuda.empty_cache()

@@ -82,8 +82,17 @@ template <int BM, int BN, int BK, int WM, int WN, int TA>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, (BM / WM) * (BN / WN) * 64 > 256 ? 1 : 2)
 dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, int lda,
              const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
-             int tiles_m, int tiles_n, int aligned)
+             int tiles_m, int tiles_n, int aligned, int kc, long long cs)
 {
+    // Split over k (gridDim.y > 1): part blockIdx.y takes k in [y kc,
+    // (y + 1) kc) and writes its product to its own C at C + y cs.
+    {
+        const int kbeg = blockIdx.y * kc;
+        k = (k < kbeg + kc ? k : kbeg + kc) - kbeg;
+        A += TA == 0 ? (size_t)kbeg * lda : (size_t)kbeg;
+        B += kbeg;
+        C += blockIdx.y * cs;
+    }
     using S = Shape<BM, BN, BK, WM, WN>;
     using P = Planes<BM, BN, BK, TA>;
     __shared__ double As[P::A_SIZE];
@@ -519,14 +528,48 @@ __global__ void zscale_kernel(int m, int n, double br, double bi, double2* __res
 
 int g_real_cfg = -1, g_cplx_cfg = -1;
 
+// Sum of the split-k parts: c = beta c + alpha sum_p w_p (p in order).
+__global__ void dsplit_reduce_kernel(int m, int n, int parts, const double* __restrict__ w, double alpha, double beta,
+                                     double* __restrict__ c, int ldc)
+{
+    const size_t total = (size_t)m * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int p = 0; p < parts; p++) acc += w[p * total + e];
+        double* q = c + e % m + (e / m) * ldc;
+        const double v = alpha * acc;
+        *q = beta == 0.0 ? v : beta * *q + v;
+    }
+}
+
+// One launch, or, when the tiles of C leave most of the chip idle and k is
+// long (the thin products of refinement residuals), the product split over
+// k into parts written to scratch and summed by dsplit_reduce_kernel.
 template <int BM, int BN, int BK, int WM, int WN, int TA>
 void launch_d(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda, const double* b, int ldb,
               double beta, double* c, int ldc)
 {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     const int aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0;
-    hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tm * tn), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
-                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn, aligned);
+    const int tiles = tm * tn;
+    if (tiles <= 256 && k >= 2048) {
+        int parts = 1024 / tiles;
+        parts = parts < 32 ? parts : 32;
+        parts = parts < k / 256 ? parts : k / 256;
+        const int kc = ((k + parts - 1) / parts + BK - 1) / BK * BK;
+        parts = (k + kc - 1) / kc;
+        DevBuf<double> w(ctx, (size_t)parts * m * n);
+        hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tiles, parts),
+                           dim3(Shape<BM, BN, BK, WM, WN>::NT), 0, ctx->stream, m, n, k, 1.0, a, lda, b, ldb, 0.0,
+                           w.get(), m, tm, tn, aligned, kc, (long long)m * n);
+        const size_t total = (size_t)m * n;
+        const int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+        hipLaunchKernelGGL(dsplit_reduce_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, parts, w.get(), alpha,
+                           beta, c, ldc);
+        return;
+    }
+    hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tiles), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
+                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn, aligned, k, 0LL);
 }
 
 template <int BM, int BN, int BK, int WM, int WN, int TA>

@@ -10,6 +10,8 @@
 // four 16 x 16 MFMA blocks per wave); the tile order is the XCD-banded,
 // grouped order of the FP64 kernels. Loads are clamped into the matrix and
 // zeroed at the edges, never branched around.
+#include <type_traits>
+
 #include "tile_order.hpp"
 #include "tns_internal.hpp"
 
@@ -52,8 +54,17 @@ struct LpPlanes<LP_BF16> {
 template <int MODE>
 __global__ void __launch_bounds__(LNT, 2)
 lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int lda, const float* __restrict__ B,
-              int ldb, float beta, float* __restrict__ C, int ldc, int tiles_m, int tiles_n)
+              int ldb, float beta, float* __restrict__ C, int ldc, int tiles_m, int tiles_n, int kc, long long cs)
 {
+    // Split over k (gridDim.y > 1): part blockIdx.y takes k in [y kc,
+    // (y + 1) kc) and writes its product to its own C at C + y cs.
+    {
+        const int kbeg = blockIdx.y * kc;
+        k = (k < kbeg + kc ? k : kbeg + kc) - kbeg;
+        A += (size_t)kbeg * lda;
+        B += kbeg;
+        C += blockIdx.y * cs;
+    }
     using P = LpPlanes<MODE>;
     __shared__ __align__(16) unsigned As[P::WORDS];
     __shared__ __align__(16) unsigned Bs[P::WORDS];
@@ -153,22 +164,70 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
     }
 
     // D layout of both 16x16 FP32-accumulating forms: lane l, register r
-    // holds row 4*(l/16) + r, column l%16 of the block.
+    // holds row 4*(l/16) + r, column l%16 of the block. The old C values
+    // are all read first (their loads in flight together) and every new
+    // value formed before the first store; tiles inside C take straight-line
+    // code, so no store waits for the one before it.
+    auto write_back = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        float val[2][2][4];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
-            if (col >= n) continue;
+            for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * 32 + i * 16 + 4 * kq + r;
-                if (row >= m) continue;
-                float* p = C + row + (size_t)col * ldc;
-                const float v = alpha * acc[i][j][r];
-                *p = beta == 0.f ? v : beta * *p + v;
-            }
+                for (int r = 0; r < 4; r++) val[i][j][r] = 0.f;
+        if (beta != 0.f) {
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = m0 + wm * 32 + i * 16 + 4 * kq + r;
+                        const float* p = C + row + (size_t)col * ldc;
+                        val[i][j][r] = FULL ? *p : load_if(p, row < m && col < n);
+                    }
+                }
         }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float v = alpha * acc[i][j][r];
+                    val[i][j][r] = beta == 0.f ? v : beta * val[i][j][r] + v;
+                }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = m0 + wm * 32 + i * 16 + 4 * kq + r;
+                    if (FULL || (row < m && col < n)) C[row + (size_t)col * ldc] = val[i][j][r];
+                }
+            }
+    };
+    if (m0 + LBM <= m && n0 + LBN <= n) write_back(std::true_type{});
+    else write_back(std::false_type{});
+}
+
+// Sum of the split-k parts: c = beta c + alpha sum_p w_p (p in order).
+__global__ void lp_split_reduce_kernel(int m, int n, int parts, const float* __restrict__ w, float alpha, float beta,
+                                       float* __restrict__ c, int ldc)
+{
+    const size_t total = (size_t)m * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+        for (int p = 0; p < parts; p++) acc += w[p * total + e];
+        float* q = c + e % m + (e / m) * ldc;
+        const float v = alpha * acc;
+        *q = beta == 0.f ? v : beta * *q + v;
+    }
 }
 
 __global__ void lp_scale_kernel(int m, int n, float beta, float* __restrict__ c, int ldc)
@@ -195,12 +254,28 @@ void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const fl
         return;
     }
     const int tm = (m + LBM - 1) / LBM, tn = (n + LBN - 1) / LBN;
-    if (mode == LP_BF16)
-        hipLaunchKernelGGL(lpgemm_kernel<LP_BF16>, dim3(tm * tn), dim3(LNT), 0, ctx->stream, m, n, k, alpha, a, lda, b,
-                           ldb, beta, c, ldc, tm, tn);
-    else
-        hipLaunchKernelGGL(lpgemm_kernel<LP_FP32>, dim3(tm * tn), dim3(LNT), 0, ctx->stream, m, n, k, alpha, a, lda, b,
-                           ldb, beta, c, ldc, tm, tn);
+    const int tiles = tm * tn;
+    auto kernel = mode == LP_BF16 ? lpgemm_kernel<LP_BF16> : lpgemm_kernel<LP_FP32>;
+    if (tiles <= 256 && k >= 256) {
+        // Few tiles over a long k (the thin products of the triangular
+        // solves): split k into parts written to scratch, then summed.
+        int parts = 1024 / tiles;
+        parts = parts < 32 ? parts : 32;
+        parts = parts < k / 128 ? parts : k / 128;
+        const int kc = ((k + parts - 1) / parts + LBK - 1) / LBK * LBK;
+        parts = (k + kc - 1) / kc;
+        DevBuf<float> w(ctx, (size_t)parts * m * n);
+        hipLaunchKernelGGL(kernel, dim3(tiles, parts), dim3(LNT), 0, ctx->stream, m, n, k, 1.f, a, lda, b, ldb, 0.f,
+                           w.get(), m, tm, tn, kc, (long long)m * n);
+        const size_t total = (size_t)m * n;
+        const int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+        hipLaunchKernelGGL(lp_split_reduce_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, parts, w.get(),
+                           alpha, beta, c, ldc);
+        TNS_HIP(hipGetLastError());
+        return;
+    }
+    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(LNT), 0, ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                       tm, tn, k, 0LL);
     TNS_HIP(hipGetLastError());
 }
 

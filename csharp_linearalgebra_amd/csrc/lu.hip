@@ -412,6 +412,7 @@ __device__ __forceinline__ void getf2_reg_body(int m, int w, T* __restrict__ a, 
     }
 }
 
+
 template <typename T, int NB, int R, int NT>
 __global__ void __launch_bounds__(NT)
 getf2_reg_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
@@ -1146,6 +1147,52 @@ __global__ void __launch_bounds__(1024) lu_init_grouped_kernel(const InitRec<T>*
     }
 }
 
+// Inverses of the W x W diagonal blocks of LU factors (the last block may
+// be smaller; it is bordered by the identity): workgroup b inverts block
+// b0 + b, writing L_kk^-1 (unit lower) and then U_kk^-1 (upper), each W x W
+// column-major, at inv + (b0 + b) * 2 W^2. Thread j < W forms column j of
+// L^-1, thread W + j column j of U^-1, by substitution against the block
+// in LDS (every thread reads the same entry at once).
+template <typename T, int W>
+__global__ void __launch_bounds__(2 * W) tri_inverse_kernel(int n, const T* __restrict__ lu, int ld, int b0,
+                                                            T* __restrict__ inv)
+{
+    __shared__ T As[W][W + 1];
+    const int blk = b0 + blockIdx.x, k0 = blk * W;
+    const int kb = W < n - k0 ? W : n - k0;
+    const T* a = lu + k0 + (size_t)k0 * ld;
+    for (int e = threadIdx.x; e < W * W; e += 2 * W) {
+        const int i = e % W, k = e / W;
+        const bool in = i < kb && k < kb;
+        As[i][k] = in ? a[i + (size_t)k * ld] : (i == k ? s_one(T{}) : s_zero(T{}));
+    }
+    __syncthreads();
+    const bool upper = threadIdx.x >= W;
+    const int j = upper ? threadIdx.x - W : threadIdx.x;
+    T y[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) y[i] = i == j ? s_one(T{}) : s_zero(T{});
+    if (!upper) {
+#pragma unroll
+        for (int p = 0; p < W; p++) {
+            const T ny = s_neg(y[p]);
+#pragma unroll
+            for (int i = p + 1; i < W; i++) y[i] = s_add(y[i], s_mul(ny, As[i][p]));
+        }
+    } else {
+#pragma unroll
+        for (int p = W - 1; p >= 0; p--) {
+            y[p] = s_div(y[p], As[p][p]);
+            const T ny = s_neg(y[p]);
+#pragma unroll
+            for (int i = 0; i < p; i++) y[i] = s_add(y[i], s_mul(ny, As[i][p]));
+        }
+    }
+    T* out = inv + (size_t)blk * 2 * W * W + (upper ? W * W : 0) + (size_t)j * W;
+#pragma unroll
+    for (int i = 0; i < W; i++) out[i] = y[i];
+}
+
 // dst[i] = src[perm[i]] (gather) or dst[perm[i]] = src[i] (scatter).
 template <typename T>
 __global__ void permute_rows_kernel(int n, int ncols, const int* __restrict__ perm, const T* __restrict__ src, int lds,
@@ -1476,6 +1523,169 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
     TNS_HIP(hipGetLastError());
 }
 
+// LU with the pivots of each block step chosen inside its NB x NB
+// diagonal block (block-local partial pivoting), right-looking in blocks of
+// NB = 64 columns: the diagonal block by the register panel kernel, its
+// interchanges moved across the whole rows, the inverses of its two
+// triangles, then the rows below it L21 = A21 U11^-1 and the block row
+// U12 = L11^-1 A12 as products with those inverses (in place: each product
+// is one tile wide in the dimension it overwrites, so every tile reads its
+// operand before it writes), and the trailing update, all on sgemm_lp. The
+// pivot search sees 64 rows instead of the whole column, so no step waits
+// on a tall panel; it is for matrices whose randomized form needs no wider
+// search (the mixed-precision solve), where the refinement absorbs the
+// growth that remains. P A = L U with perm as in getrf_impl; inv keeps
+// every diagonal block's inverses for the solves; a zero pivot sets info.
+constexpr int LOCAL_NB = 64;
+
+template <typename T>
+void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_perm, T* inv, int* d_info)
+{
+    constexpr int NB = LOCAL_NB;
+    static_assert(sizeof(T) == 4, "the register kernels hold 64 columns of float");
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info, INT_MAX);
+    if (n <= 0) {
+        hipLaunchKernelGGL(finish_info_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info);
+        return;
+    }
+    DevBuf<int> rc(ctx, (size_t)n);
+    DevBuf<int> list(ctx, 4 * (size_t)NB + 4);
+    hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, d_perm, rc.get());
+    const T one = Ops<T>::one(), zero{};
+    for (int jb = 0; jb < n; jb += NB) {
+        const int w = NB < n - jb ? NB : n - jb;
+        T* pa = a + jb + (size_t)jb * lda;
+        panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb, n - jb - w);
+        hipLaunchKernelGGL((tri_inverse_kernel<T, NB>), dim3(1), dim3(2 * NB), 0, ctx->stream, n, a, lda, jb / NB, inv);
+        const int rest = n - jb - w;
+        if (rest <= 0) continue;
+        const T* inv_l = inv + (size_t)(jb / NB) * 2 * NB * NB;
+        const T* inv_u = inv_l + NB * NB;
+        // The two products with the inverses stay FP32 in both modes.
+        sgemm_lp(ctx, LP_FP32, rest, w, w, one, pa + w, lda, inv_u, NB, zero, pa + w, lda);
+        sgemm_lp(ctx, LP_FP32, w, rest, w, one, inv_l, NB, pa + (size_t)w * lda, lda, zero, pa + (size_t)w * lda, lda);
+        Ops<T>::gemm(ctx, 0, rest, rest, w, Ops<T>::neg_one(), pa + w, lda, pa + (size_t)w * lda, lda, one,
+                     pa + w + (size_t)w * lda, lda);
+    }
+    hipLaunchKernelGGL(finish_info_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info);
+    TNS_HIP(hipGetLastError());
+}
+
+// One diagonal super block (rows r0 .. r0 + rows, rows a multiple of W <=
+// 1024) of a triangular solve with the factors of getrf_local_impl, for
+// TC right-hand sides per workgroup (columns blockIdx.x * TC ...): the
+// block's part of x sits in LDS while its W-blocks are solved in turn,
+// x_j := inv_jj x_j (the diagonal blocks' inverses), and each result is
+// subtracted from the block's rows still to come. One launch does the work
+// of 2 rows / W launches of the block-by-block form. LOWER: unit lower,
+// forward; else upper, backward.
+template <typename T, int W, int TC, bool LOWER>
+__global__ void __launch_bounds__(256) trsv_super_kernel(int r0, int rows, int t, const T* __restrict__ lu, int ld,
+                                                         const T* __restrict__ inv, T* __restrict__ x, int ldx)
+{
+    extern __shared__ unsigned char super_smem[];
+    T* xs = reinterpret_cast<T*>(super_smem);  // xs[i * TC + c], i < rows
+    __shared__ T iv[W][W + 1];
+    const int tid = threadIdx.x, c0 = blockIdx.x * TC;
+    const int nc = t - c0 < TC ? t - c0 : TC;
+    for (int e = tid; e < rows * TC; e += 256) {
+        const int i = e / TC, c = e % TC;
+        xs[e] = load_if(x + r0 + i + (size_t)(c0 + c) * ldx, c < nc);
+    }
+    const int nb = rows / W;
+    for (int s = 0; s < nb; s++) {
+        const int jb = LOWER ? s : nb - 1 - s;
+        const int k0 = jb * W;
+        const T* src = inv + (size_t)((r0 + k0) / W) * 2 * W * W + (LOWER ? 0 : W * W);
+        __syncthreads();
+        for (int e = tid; e < W * W; e += 256) iv[e % W][e / W] = src[e];
+        __syncthreads();
+        // x_j := inv_jj x_j, two outputs per thread.
+        T y[(W * TC + 255) / 256];
+#pragma unroll
+        for (int u = 0; u < (W * TC + 255) / 256; u++) {
+            const int e = tid + u * 256;
+            const int i = e / TC, c = e % TC;
+            T acc = s_zero(T{});
+            if (e < W * TC)
+                for (int p = 0; p < W; p++) acc = s_add(acc, s_mul(iv[i][p], xs[(k0 + p) * TC + c]));
+            y[u] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < (W * TC + 255) / 256; u++) {
+            const int e = tid + u * 256;
+            if (e < W * TC) xs[(k0 + e / TC) * TC + e % TC] = y[u];
+        }
+        __syncthreads();
+        // The block's other rows still to come: one row per thread, all TC
+        // columns, entries of the factor read once each.
+        const int lo = LOWER ? k0 + W : 0, hi = LOWER ? rows : k0;
+        for (int i = lo + tid; i < hi; i += 256) {
+            T acc[TC];
+#pragma unroll
+            for (int c = 0; c < TC; c++) acc[c] = xs[i * TC + c];
+            const T* f = lu + (r0 + i) + (size_t)(r0 + k0) * ld;
+            // Sixteen entries of the row in flight at a time.
+            for (int p0 = 0; p0 < W; p0 += 16) {
+                T a[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) a[u] = f[(size_t)(p0 + u) * ld];
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    const T na = s_neg(a[u]);
+#pragma unroll
+                    for (int c = 0; c < TC; c++) acc[c] = s_add(acc[c], s_mul(na, xs[(k0 + p0 + u) * TC + c]));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < TC; c++) xs[i * TC + c] = acc[c];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < rows * TC; e += 256) {
+        const int i = e / TC, c = e % TC;
+        if (c < nc) x[r0 + i + (size_t)(c0 + c) * ldx] = xs[e];
+    }
+}
+
+// P A = L U solve with the factors of getrf_local_impl (order N a multiple
+// of LOCAL_NB): P b, then L forward and U backward in super blocks of up to
+// 512 rows, each by trsv_super_kernel and the rows beyond it by one
+// product.
+template <typename T>
+void getrs_local_impl(Context* ctx, int N, int nrhs, const T* lu, int ldlu, const int* d_perm, const T* inv, T* b,
+                      int ldb)
+{
+    constexpr int W = LOCAL_NB, TC = 2, SUPER = 512;
+    if (N <= 0 || nrhs <= 0) return;
+    if (N % W) fail(TNS_EINVAL, "the local LU's order is a multiple of its block");
+    DevBuf<T> bx(ctx, (size_t)N * nrhs);
+    T* x = bx.get();
+    const T one = Ops<T>::one();
+    permute_rows<T>(ctx, N, nrhs, d_perm, b, ldb, x, N, 0);
+    const dim3 grid((nrhs + TC - 1) / TC);
+    const int nsup = (N + SUPER - 1) / SUPER;
+    for (int q = 0; q < nsup; q++) {
+        const int r0 = q * SUPER, rows = SUPER < N - r0 ? SUPER : N - r0;
+        hipLaunchKernelGGL((trsv_super_kernel<T, W, TC, true>), grid, dim3(256), sizeof(T) * rows * TC, ctx->stream,
+                           r0, rows, nrhs, lu, ldlu, inv, x, N);
+        if (r0 + rows < N)
+            Ops<T>::gemm(ctx, 0, N - r0 - rows, nrhs, rows, Ops<T>::neg_one(), lu + (r0 + rows) + (size_t)r0 * ldlu,
+                         ldlu, x + r0, N, one, x + r0 + rows, N);
+    }
+    for (int q = nsup - 1; q >= 0; q--) {
+        const int r0 = q * SUPER, rows = SUPER < N - r0 ? SUPER : N - r0;
+        hipLaunchKernelGGL((trsv_super_kernel<T, W, TC, false>), grid, dim3(256), sizeof(T) * rows * TC, ctx->stream,
+                           r0, rows, nrhs, lu, ldlu, inv, x, N);
+        if (r0 > 0)
+            Ops<T>::gemm(ctx, 0, r0, nrhs, rows, Ops<T>::neg_one(), lu + (size_t)r0 * ldlu, ldlu, x + r0, N, one, x,
+                         N);
+    }
+    copy2d<T>(ctx, N, nrhs, x, N, b, ldb);
+    TNS_HIP(hipGetLastError());
+}
+
 template <typename T>
 void getrs_impl(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu, const int* d_perm, T* b, int ldb)
 {
@@ -1519,6 +1729,19 @@ void sgetrf(Context* ctx, int m, int n, float* a, int lda, int nb, int* d_piv, i
 void sgetrs(Context* ctx, int n, int nrhs, const float* lu, int ldlu, const int* d_perm, float* b, int ldb)
 {
     getrs_impl<float>(ctx, 0, n, nrhs, lu, ldlu, d_perm, b, ldb);
+}
+
+int local_lu_block() { return LOCAL_NB; }
+
+void sgetrf_local(Context* ctx, int n, float* a, int lda, int* d_piv, int* d_perm, float* inv, int* d_info)
+{
+    getrf_local_impl<float>(ctx, n, a, lda, d_piv, d_perm, inv, d_info);
+}
+
+void sgetrs_local(Context* ctx, int n, int nrhs, const float* lu, int ldlu, const int* d_perm, const float* inv,
+                  float* b, int ldb)
+{
+    getrs_local_impl<float>(ctx, n, nrhs, lu, ldlu, d_perm, inv, b, ldb);
 }
 
 void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb)

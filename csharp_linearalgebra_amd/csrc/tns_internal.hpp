@@ -179,6 +179,16 @@ void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const fl
 void sgetrf(Context* ctx, int m, int n, float* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info);
 // A X = B against FP32 factors (no transpose).
 void sgetrs(Context* ctx, int n, int nrhs, const float* lu, int ldlu, const int* d_perm, float* b, int ldb);
+// FP32 LU with block-local partial pivoting (each block step's pivots
+// from its B x B diagonal block, B = local_lu_block(); for a randomized
+// matrix of order n a multiple of B), products on sgemm_lp in
+// ctx->lp_mode; P A = L U with perm as sgetrf's, info the first zero pivot
+// or -1; inv (n / B * 2 B^2 floats) receives the inverses of the diagonal
+// blocks' triangles, which its solve for few right-hand sides uses.
+int local_lu_block();
+void sgetrf_local(Context* ctx, int n, float* a, int lda, int* d_piv, int* d_perm, float* inv, int* d_info);
+void sgetrs_local(Context* ctx, int n, int nrhs, const float* lu, int ldlu, const int* d_perm, const float* inv,
+                  float* b, int ldb);
 
 // Level-1/2 helpers (blas1.hip).
 void dnorm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* d_out);

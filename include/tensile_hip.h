@@ -223,14 +223,17 @@ TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* us
 /* ---- mixed precision: A X = B by an FP32 LU and FP64 iterative refinement.
  * No counterpart in the reference: it stands beside the FP64 solve
  * KernelEntry.FactorLu + SolveLu (KernelEntry.cs:59-95, Lu.cs:189-378),
- * which it falls back to. precision 32: FP32 factorisation on the FP32
- * matrix cores; 16: the same with its products on the BF16 matrix cores
- * (FP32 accumulation) after power-of-two row and column equilibration.
+ * which it falls back to. The low-precision factorisation is of a random
+ * butterfly transformation U^T A V of A (depth 2), in FP32 with each block
+ * step's pivots chosen inside its 64 x 64 diagonal block. precision 32: its
+ * products on the FP32 matrix cores; 16: on the BF16 matrix cores (FP32
+ * accumulation) after power-of-two row and column equilibration.
  * Refinement (residual and update in FP64) runs until the backward error
  * max_j ||b_j - A x_j||inf / (||A||inf ||x_j||inf + ||b_j||inf) <= tol, at
- * most max_iter steps; a step that fails to halve it, or an exactly zero
- * low-precision pivot, switches to the FP64 LU (fell_back = 1). A and B
- * are not modified. TNS_ESTATE if the FP64 LU also meets a zero pivot. */
+ * most max_iter steps; a step that fails to cut it by a tenth, or an exactly
+ * zero low-precision pivot, switches to the FP64 LU with partial pivoting
+ * (fell_back = 1). A and B are not modified. TNS_ESTATE if the FP64 LU also
+ * meets a zero pivot. */
 typedef struct {
     int precision;          /* 32 or 16, as requested */
     int iterations;         /* refinement steps taken */
