@@ -820,7 +820,8 @@ TNS_API int tns_sgemm(tns_context* ctx, int precision, int m, int n, int k, floa
                       const float* b, int ldb, float beta, float* c, int ldc)
 {
     return guarded(ctx, [&](Context* cx) {
-        if (precision != 32 && precision != 16) fail(TNS_EINVAL, "precision must be 32 or 16");
+        if (precision != 32 && precision != 16 && precision != 48)
+            fail(TNS_EINVAL, "precision must be 32, 16 or 48");
         if (m < 0 || n < 0 || k < 0) fail(TNS_EINVAL, "negative dimension");
         check_operand(c, m, n, ldc, "C");
         if (m == 0 || n == 0) return;
@@ -829,8 +830,8 @@ TNS_API int tns_sgemm(tns_context* ctx, int precision, int m, int n, int k, floa
         View<float> va = bring(cx, a, m, k, lda, true);
         View<float> vb = bring(cx, b, k, n, ldb, true);
         View<float> vc = bring(cx, c, m, n, ldc, beta != 0.f);
-        tns::sgemm_lp(cx, precision == 16 ? tns::LP_BF16 : tns::LP_FP32, m, n, k, alpha, va.d, va.ld, vb.d, vb.ld,
-                      beta, vc.d, vc.ld);
+        const int mode = precision == 16 ? tns::LP_BF16 : precision == 48 ? tns::LP_BF16X3 : tns::LP_FP32;
+        tns::sgemm_lp(cx, mode, m, n, k, alpha, va.d, va.ld, vb.d, vb.ld, beta, vc.d, vc.ld);
         send_back(cx, vc, c, m, n, ldc);
         tns::sync(cx);
     });

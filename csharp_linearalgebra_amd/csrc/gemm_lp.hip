@@ -1,9 +1,13 @@
 // Single-precision products for the mixed-precision factorisation: the
 // trailing updates C := beta*C + alpha*A*B of an FP32 LU (and of its
-// solves), on the matrix cores with FP32 accumulation. Two input modes:
-//   LP_FP32: v_mfma_f32_16x16x4f32, operands as stored;
-//   LP_BF16: v_mfma_f32_16x16x32_bf16, operands rounded to bfloat16 (to
-//            nearest, ties to even) as they are staged to LDS.
+// solves), on the matrix cores with FP32 accumulation. Three input modes:
+//   LP_FP32:   v_mfma_f32_16x16x4f32, operands as stored;
+//   LP_BF16:   v_mfma_f32_16x16x32_bf16, operands rounded to bfloat16 (to
+//              nearest, ties to even) as they are staged to LDS;
+//   LP_BF16X3: the same instruction three times per block, each operand
+//              split as x = hi + lo into two bfloat16 parts and the
+//              product taken as hi*hi + hi*lo + lo*hi (relative error
+//              about 2^-16, near FP32's, at the BF16 rate).
 // C and the accumulators stay FP32 in both. A is never transposed here.
 //
 // Tiles of 64 x 64 x 32 per workgroup of four waves (2 x 2, 32 x 32 each,
@@ -50,6 +54,23 @@ struct LpPlanes<LP_BF16> {
     static constexpr int KP = LBK + 8;
     static constexpr int WORDS = LBM * KP / 2;  // 32-bit words per plane
 };
+
+// BF16 x 3: the same rows, a plane of leading parts and, after it, a plane
+// of trailing parts.
+template <>
+struct LpPlanes<LP_BF16X3> {
+    static constexpr int KP = LBK + 8;
+    static constexpr int WORDS = LBM * KP;  // both parts
+    static constexpr int LO = LBM * KP;     // offset of the trailing parts, in bf16
+};
+
+// x = hi + lo + O(2^-16 |x|): hi = bf16(x), lo = bf16(x - hi) (the
+// difference is exact in FP32).
+__device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
+{
+    hi = to_bf16(x);
+    lo = to_bf16(x - __uint_as_float((unsigned)hi << 16));
+}
 
 template <int MODE>
 __global__ void __launch_bounds__(LNT, 2)
@@ -108,12 +129,17 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
             const int e = tid + r * LNT;
             const int i = e % LBM, kk = e / LBM;
             const int kb = e % LBK, j = e / LBK;
-            if (MODE == LP_FP32) {
+            if constexpr (MODE == LP_FP32) {
                 As[i * P::KP + kk] = __float_as_uint(ra[r]);
                 Bs[j * P::KP + kb] = __float_as_uint(rb[r]);
-            } else {
+            } else if constexpr (MODE == LP_BF16) {
                 reinterpret_cast<unsigned short*>(As)[i * P::KP + kk] = to_bf16(ra[r]);
                 reinterpret_cast<unsigned short*>(Bs)[j * P::KP + kb] = to_bf16(rb[r]);
+            } else {
+                unsigned short* as = reinterpret_cast<unsigned short*>(As);
+                unsigned short* bs = reinterpret_cast<unsigned short*>(Bs);
+                split_bf16(ra[r], as[i * P::KP + kk], as[P::LO + i * P::KP + kk]);
+                split_bf16(rb[r], bs[j * P::KP + kb], bs[P::LO + j * P::KP + kb]);
             }
         }
     };
@@ -126,7 +152,7 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
     const int frow = lane & 15, kq = lane >> 4;
     for (int t = 0; t < ktiles; ++t) {
         if (t + 1 < ktiles) gload((t + 1) * LBK);
-        if (MODE == LP_FP32) {
+        if constexpr (MODE == LP_FP32) {
 #pragma unroll
             for (int kk = 0; kk < LBK; kk += 4) {
                 float af[2], bf[2];
@@ -140,7 +166,7 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
                     for (int j = 0; j < 2; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
             }
-        } else {
+        } else if constexpr (MODE == LP_BF16) {
             const unsigned short* as = reinterpret_cast<const unsigned short*>(As);
             const unsigned short* bs = reinterpret_cast<const unsigned short*>(Bs);
             bf16x8 af[2], bfr[2];
@@ -155,6 +181,32 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
 #pragma unroll
                 for (int j = 0; j < 2; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        } else {
+            // Three products per block: the trailing parts first, then the
+            // leading ones (lo x lo, below 2^-16 of the product, is left out).
+            const unsigned short* as = reinterpret_cast<const unsigned short*>(As);
+            const unsigned short* bs = reinterpret_cast<const unsigned short*>(Bs);
+            bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int o = (wm * 32 + i * 16 + frow) * P::KP + 8 * kq;
+                ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(as + o));
+                al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(as + P::LO + o));
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int o = (wn * 32 + j * 16 + frow) * P::KP + 8 * kq;
+                bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(bs + o));
+                bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(bs + P::LO + o));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
         }
         __syncthreads();
         if (t + 1 < ktiles) {
@@ -255,7 +307,9 @@ void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const fl
     }
     const int tm = (m + LBM - 1) / LBM, tn = (n + LBN - 1) / LBN;
     const int tiles = tm * tn;
-    auto kernel = mode == LP_BF16 ? lpgemm_kernel<LP_BF16> : lpgemm_kernel<LP_FP32>;
+    auto kernel = mode == LP_BF16 ? lpgemm_kernel<LP_BF16>
+                  : mode == LP_BF16X3 ? lpgemm_kernel<LP_BF16X3>
+                                      : lpgemm_kernel<LP_FP32>;
     if (tiles <= 256 && k >= 256) {
         // Few tiles over a long k (the thin products of the triangular
         // solves): split k into parts written to scratch, then summed.

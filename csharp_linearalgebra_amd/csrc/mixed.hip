@@ -1,7 +1,9 @@
-// Mixed-precision solve A X = B: a random butterfly transformation of A
-// rounded to FP32, LU of it in FP32 with the pivots of each block step
-// taken inside its 64 x 64 diagonal block (products on the FP32 matrix
-// cores, or on the BF16 ones after row and column equilibration), then
+// Mixed-precision solve A X = B: power-of-two row and column equilibration
+// (when the scales spread widely), a random butterfly transformation, the
+// result rounded to FP32 and
+// factored in FP32 with the pivots of each block step taken inside its
+// 64 x 64 diagonal block (products on the FP32 matrix cores, or on the BF16
+// ones with each operand split into two bfloat16 parts), then
 // iterative refinement in FP64 — residuals r = b - A x
 // and the updates of x in FP64, corrections from the low-precision factors
 // — until the normwise backward error max_j ||r_j||inf / (||A||inf ||x_j||inf
@@ -260,7 +262,10 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
         return;
     }
     const auto t0 = clock::now();
-    const int mode = precision == 16 ? LP_BF16 : LP_FP32;
+    // BF16: the three-part products (operands split into two bfloat16
+    // parts): single-part BF16 products leave the block-locally pivoted
+    // factors too coarse for the refinement to contract.
+    const int mode = precision == 16 ? LP_BF16X3 : LP_FP32;
 
     DevBuf<double> d_tmp(ctx, 2 * (size_t)t + 8);
     DevBuf<double> r(ctx, (size_t)n * t);
@@ -275,18 +280,26 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
     int iters = 0;
     double berr = INFINITY;
     {
-        // Equilibration for BF16: powers of two, so R A C is exact.
-        DevBuf<double> rs, cs;
-        const double* prs = nullptr;
-        const double* pcs = nullptr;
-        if (mode == LP_BF16) {
-            rs = DevBuf<double>(ctx, n);
-            cs = DevBuf<double>(ctx, n);
-            hipLaunchKernelGGL(row_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, a, lda, rs.get());
-            hipLaunchKernelGGL(col_scale_kernel, dim3(n), dim3(256), 0, ctx->stream, n, a, lda, rs.get(), cs.get());
-            prs = rs.get();
-            pcs = cs.get();
-        }
+        // Row and column equilibration by powers of two (R A C is exact),
+        // when the rows' or the columns' scales spread over more than 2^8:
+        // the butterflies mix rows and columns four at a time, which on a
+        // badly scaled matrix buries the small ones in the rounding of the
+        // large. Milder spreads are left alone: there equilibration only
+        // moves the factorization's error onto the rows that weigh least in
+        // the normwise backward error, and slows the refinement.
+        DevBuf<double> rs(ctx, n), cs(ctx, n);
+        hipLaunchKernelGGL(row_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, a, lda, rs.get());
+        hipLaunchKernelGGL(col_scale_kernel, dim3(n), dim3(256), 0, ctx->stream, n, a, lda, rs.get(), cs.get());
+        std::vector<double> hs(2 * (size_t)n);
+        read_doubles(ctx, rs.get(), hs.data(), n);
+        read_doubles(ctx, cs.get(), hs.data() + n, n);
+        auto spread = [](const double* v, int len) {
+            const auto mm = std::minmax_element(v, v + len);
+            return *mm.second / *mm.first;
+        };
+        const bool equilibrate = std::max(spread(hs.data(), n), spread(hs.data() + n, n)) > 256.0;
+        const double* prs = equilibrate ? rs.get() : nullptr;
+        const double* pcs = equilibrate ? cs.get() : nullptr;
         // The randomized matrix, order n4 >= n (the border is the
         // identity), a multiple of the factorization's block.
         const int blk = local_lu_block();

@@ -170,8 +170,9 @@ void zpivot_range(Context* ctx, int n, const cplx* a, int lda, double* d_out);
 
 // Mixed precision (gemm_lp.hip, lu.hip). Single-precision products on the
 // matrix cores with FP32 accumulation; mode LP_FP32 takes the operands as
-// they are, LP_BF16 rounds them to bfloat16 first. A untransposed.
-enum { LP_FP32 = 0, LP_BF16 = 1 };
+// they are, LP_BF16 rounds them to bfloat16 first, LP_BF16X3 splits each
+// into two bfloat16 parts and sums three products. A untransposed.
+enum { LP_FP32 = 0, LP_BF16 = 1, LP_BF16X3 = 2 };
 void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const float* a, int lda, const float* b,
               int ldb, float beta, float* c, int ldc);
 // FP32 LU with partial pivoting; its trailing updates use sgemm_lp in

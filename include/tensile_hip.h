@@ -224,10 +224,13 @@ TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* us
  * No counterpart in the reference: it stands beside the FP64 solve
  * KernelEntry.FactorLu + SolveLu (KernelEntry.cs:59-95, Lu.cs:189-378),
  * which it falls back to. The low-precision factorisation is of a random
- * butterfly transformation U^T A V of A (depth 2), in FP32 with each block
- * step's pivots chosen inside its 64 x 64 diagonal block. precision 32: its
- * products on the FP32 matrix cores; 16: on the BF16 matrix cores (FP32
- * accumulation) after power-of-two row and column equilibration.
+ * butterfly transformation U^T (R A C) V of A (depth 2, after power-of-two
+ * row and column equilibration R, C when the row or column scales spread
+ * over more than 2^8), in FP32 with each block step's pivots
+ * chosen inside its 64 x 64 diagonal block. precision 32: its products on
+ * the FP32 matrix cores; 16: its trailing updates on the BF16 matrix cores
+ * (FP32 accumulation, operands split into two bfloat16 parts, three
+ * products each).
  * Refinement (residual and update in FP64) runs until the backward error
  * max_j ||b_j - A x_j||inf / (||A||inf ||x_j||inf + ||b_j||inf) <= tol, at
  * most max_iter steps; a step that fails to cut it by a tenth, or an exactly
@@ -247,10 +250,13 @@ typedef struct {
 TNS_API int tns_dgesv_mixed(tns_context* ctx, int n, int nrhs, const double* a, int lda, const double* b, int ldb,
                             double* x, int ldx, int precision, double tol, int max_iter, tns_mixed_info* info);
 /* Its low-precision parts, exported so they can be checked on their own
- * (no counterparts in the reference). precision as above: C := beta*C +
- * alpha*A*B in FP32 with the products on the FP32 or BF16 matrix cores;
- * the FP32 LU with partial pivoting (pivots and diagnostics as tns_dgetrf);
- * and the solve against its factors. */
+ * (no counterparts in the reference). tns_sgemm: C := beta*C + alpha*A*B
+ * in FP32 with the products on the FP32 (precision 32) or BF16 (16)
+ * matrix cores, or on the BF16 ones with each operand split into two
+ * bfloat16 parts and three products per block (48: about 2^-16 relative
+ * error; the trailing updates of the mixed solve's precision 16). The FP32
+ * LU with partial pivoting (pivots and diagnostics as tns_dgetrf) and the
+ * solve against its factors take precision 32 or 16. */
 TNS_API int tns_sgemm(tns_context* ctx, int precision, int m, int n, int k, float alpha, const float* a, int lda,
                       const float* b, int ldb, float beta, float* c, int ldc);
 TNS_API int tns_sgetrf(tns_context* ctx, int precision, int n, float* a, int lda, int* pivots, int* singular_column);

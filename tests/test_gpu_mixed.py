@@ -116,6 +116,34 @@ def test_single_precision_products(ws, rng, precision, m, n, k):
     assert np.all(np.abs(c - ref) <= 2 * bound + 1e-30)
 
 
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (100, 37, 129), (256, 192, 300)])
+def test_three_part_bf16_products(ws, rng, m, n, k):
+    """Precision 48: operands split x = hi + lo into two bfloat16 parts and
+    the product summed as hi*hi + hi*lo + lo*hi in FP32. The error per
+    product is about 2^-16 of |a||b| (the dropped lo*lo and lo's rounding),
+    far inside one-part BF16's 2^-8: checked against the FP64 product, and
+    against the one-part error on the same data (a kernel that dropped a
+    cross term or rounded only once would fail the second bound)."""
+    import csharp_linearalgebra_amd as T
+    from tests._raw import addr
+
+    lib = T.load()
+    a = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(np.float32))
+    b = np.asfortranarray(rng.uniform(-1, 1, (k, n)).astype(np.float32))
+    c0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(np.float32))
+    out = {}
+    for prec in (48, 16):
+        c = c0.copy(order="F")
+        assert lib.tns_sgemm(ws.handle, prec, m, n, k, -1.0, addr(a), m, addr(b), k, 1.0, addr(c), m) == 0
+        out[prec] = c.astype(np.float64)
+    ref = c0.astype(np.float64) - a.astype(np.float64) @ b.astype(np.float64)
+    mag = np.abs(a).astype(np.float64) @ np.abs(b) + np.abs(c0)
+    err3 = np.abs(out[48] - ref)
+    assert np.all(err3 <= (k * 2.0 ** -15 + (k + 2) * np.finfo(np.float32).eps) * mag)
+    err1 = np.abs(out[16] - ref)
+    assert err3.max() * 20 < err1.max()
+
+
 @pytest.mark.parametrize("n", [48, 200, 700, 1100])
 def test_single_precision_lu_matches_lapack_sgetrf(ws, rng, n):
     import scipy.linalg as sl
