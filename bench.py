@@ -283,12 +283,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--roofline-only", action="store_true", help="run only the dominant kernel (for rocprofv3)")
+    ap.add_argument("--leg", choices=["sweep", "mixed", "grad"],
+                    help="run one extra leg alone and print its JSON (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this backend has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
     if args.roofline_only:
         roofline_only()
+        return
+    if args.leg:
+        import csharp_linearalgebra_amd as T
+
+        ws = T.Workspace(0)
+        if args.leg == "sweep":
+            out = sweep_leg(ws)
+        elif args.leg == "mixed":
+            out = mixed_leg(ws, 8192, (1e2,))
+        else:
+            out = grad_leg(ws)
+        print(json.dumps(out), flush=True)
         return
 
     import torch
@@ -393,6 +407,12 @@ def main():
 
     # ---- mixed-precision LU + FP64 refinement vs FP64 LU, n = 4096 and 8192 -------
     result["mixed_lu"] = {str(nm): mixed_leg(ws, nm, [1e2, 1e6, 1e10]) for nm in (4096, 8192)}
+    # The metric's second figure: MP-LU + IR against the FP64 LU, n = 8192,
+    # at the conditioning where refinement converges (kappa = 1e2).
+    row = result["mixed_lu"]["8192"][0]
+    result["mixed_speedup_n8192"] = {"kappa": row["kappa"], "fp64_ms": row["fp64_ms"],
+                                     "fp32": row["fp32"]["speedup"], "bf16": row["bf16"]["speedup"],
+                                     "backward_error_fp32": row["fp32"]["backward_error"]}
 
     # ---- batched frequency sweep ------------------------------------------------
     result["batched_sweep"] = sweep_leg(ws)

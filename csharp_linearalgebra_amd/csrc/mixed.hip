@@ -49,14 +49,27 @@ __device__ __forceinline__ double pow2_inverse(double x)
     return ldexp(1.0, -e);
 }
 
-// rs[i] = 2^-e for the largest |a_ij| of row i (one thread per row).
-__global__ void row_scale_kernel(int n, const double* __restrict__ a, int lda, double* __restrict__ rs)
+// Row maxima of |a_ij| in two passes: workgroup (x, y) takes rows x*256 ..
+// and the columns of chunk y, and folds its partial maximum into rmax[i]
+// (bits of a non-negative double order as unsigned integers; NaNs are
+// skipped, as fmax skips them); then rs[i] = 2^-e for the row's maximum.
+constexpr int ROW_CHUNK = 64;
+
+__global__ void __launch_bounds__(256) row_max_kernel(int n, const double* __restrict__ a, int lda,
+                                                      unsigned long long* __restrict__ rmax)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int j0 = blockIdx.y * ROW_CHUNK, j1 = j0 + ROW_CHUNK < n ? j0 + ROW_CHUNK : n;
+    double mx = 0.0;
+    for (int j = j0; j < j1; j++) mx = fmax(mx, fabs(a[i + (size_t)j * lda]));
+    atomicMax(rmax + i, (unsigned long long)__double_as_longlong(mx));
+}
+
+__global__ void row_scale_kernel(int n, const unsigned long long* __restrict__ rmax, double* __restrict__ rs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double mx = 0.0;
-    for (int j = 0; j < n; j++) mx = fmax(mx, fabs(a[i + (size_t)j * lda]));
-    rs[i] = pow2_inverse(mx);
+    if (i < n) rs[i] = pow2_inverse(__longlong_as_double((long long)rmax[i]));
 }
 
 // cs[j] = 2^-e for the largest |rs_i a_ij| of column j (a block per column).
@@ -288,7 +301,14 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
         // moves the factorization's error onto the rows that weigh least in
         // the normwise backward error, and slows the refinement.
         DevBuf<double> rs(ctx, n), cs(ctx, n);
-        hipLaunchKernelGGL(row_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, a, lda, rs.get());
+        {
+            DevBuf<unsigned long long> rmax(ctx, n);
+            TNS_HIP(hipMemsetAsync(rmax.get(), 0, sizeof(unsigned long long) * n, ctx->stream));
+            hipLaunchKernelGGL(row_max_kernel, dim3((n + 255) / 256, (n + ROW_CHUNK - 1) / ROW_CHUNK), dim3(256), 0,
+                               ctx->stream, n, a, lda, rmax.get());
+            hipLaunchKernelGGL(row_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, rmax.get(),
+                               rs.get());
+        }
         hipLaunchKernelGGL(col_scale_kernel, dim3(n), dim3(256), 0, ctx->stream, n, a, lda, rs.get(), cs.get());
         std::vector<double> hs(2 * (size_t)n);
         read_doubles(ctx, rs.get(), hs.data(), n);
