@@ -268,6 +268,37 @@ def grad_leg(ws, sizes=(256, 1024)):
     return out
 
 
+def kernels_leg(ws, n=8192):
+    """The memory-bound parts on their own, for rocprofv3: the 1-norm and
+    the infinity-norm of an n x n FP64 matrix (8 n^2 bytes each), the unit
+    lower triangular solve with one right-hand side (the triangle, 4 n^2
+    bytes), and an FP64 LU of order 4096 (its panels and trailing updates).
+    Returns the solve's HIP-event time on the library stream."""
+    import ctypes as C
+
+    import torch
+
+    lib = ws._lib
+    a = torch.rand((n, n), dtype=torch.float64, device="cuda") + torch.eye(n, dtype=torch.float64, device="cuda") * n
+    x = torch.rand((1, n), dtype=torch.float64, device="cuda")
+    out = C.c_double()
+    torch.cuda.synchronize()
+    for kind in (b"1", b"I"):
+        for _ in range(10):
+            ws.check(lib.tns_dnorm(ws.handle, C.c_char(kind), n, n, a.data_ptr(), n, C.byref(out)))
+    trsv = event_time(ws, lambda: ws.check(lib.tns_dtrsm(ws.handle, 0, n, 1, a.data_ptr(), n, x.data_ptr(), n)), 5)
+    m = 4096
+    f = torch.rand((m, m), dtype=torch.float64, device="cuda")
+    piv = torch.zeros(m, dtype=torch.int32, device="cuda")
+    from csharp_linearalgebra_amd import _lib as L
+
+    info = L.LuInfo()
+    torch.cuda.synchronize()
+    ws.check(lib.tns_dgetrf(ws.handle, m, m, f.data_ptr(), m, 0, piv.data_ptr(), C.byref(info)))
+    ws.synchronize()
+    return {"trsv_n%d_us" % n: round(trsv * 1e6, 1), "trsv_gbs": round(4.0 * n * n / trsv / 1e9, 1)}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
     if not os.path.exists(path):
@@ -283,7 +314,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--roofline-only", action="store_true", help="run only the dominant kernel (for rocprofv3)")
-    ap.add_argument("--leg", choices=["sweep", "mixed", "grad"],
+    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels"],
                     help="run one extra leg alone and print its JSON (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -300,6 +331,8 @@ def main():
             out = sweep_leg(ws)
         elif args.leg == "mixed":
             out = mixed_leg(ws, 8192, (1e2,))
+        elif args.leg == "kernels":
+            out = kernels_leg(ws)
         else:
             out = grad_leg(ws)
         print(json.dumps(out), flush=True)

@@ -936,6 +936,92 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
     for (int i = tid; i < n; i += 256) xg[i] = xs[i];
 }
 
+// One step of a few-right-hand-side solve op(A) X = B, A lower unit
+// (forward) or upper (backward), not transposed, spread over the chip: the
+// rows still to come are split into slabs of 256, one workgroup each, and
+// every row takes block [k0, k0 + kb)'s final x in the order trsv_lds_kernel
+// applies it (column by column, skipping zero entries of x); then the
+// workgroup of slab 0, which holds the next diagonal block [next0, next0 +
+// nextb), solves it as trsv_lds_kernel does (one wavefront per column,
+// readlane substitution). One launch per diagonal block, every element
+// computed by the same operations in the same order as trsv_lds_kernel, but
+// the updates read the triangle with the whole chip instead of one
+// workgroup per column. Columns blockIdx.y * NC ...
+template <typename T, bool LOWER, bool UNIT, int NC>
+__global__ void __launch_bounds__(256) trsv_step_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B,
+                                                        int ldb, int nrhs, int k0, int kb, int next0, int nextb)
+{
+    constexpr int BS = tri_block<T>();
+    constexpr int U = sizeof(T) == 16 ? 4 : 8;
+    static_assert(NC <= 4, "one wavefront per column for the diagonal block");
+    __shared__ T xb[BS][NC];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.y * NC;
+    const int nc = nrhs - c0 < NC ? nrhs - c0 : NC;
+    T* X = B + (size_t)c0 * ldb;
+    for (int e = tid; e < BS * NC; e += 256) {
+        const int p = e % BS, c = e / BS;
+        xb[p][c] = load_if(X + k0 + p + (size_t)c * ldb, p < kb && c < nc);
+    }
+    __syncthreads();
+    if (kb > 0) {
+        // This slab's rows: below the block (forward) or above it.
+        const int lo = LOWER ? k0 + kb + (int)blockIdx.x * 256 : k0 - ((int)blockIdx.x + 1) * 256;
+        const int i = lo + tid;
+        const bool row = LOWER ? i < n : (i >= 0 && i < k0 - (int)blockIdx.x * 256);
+        if (row) {
+            T acc[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[c] = load_if(X + i + (size_t)c * ldb, c < nc);
+            for (int q = 0; q < kb; q += U) {
+                T av[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int p = LOWER ? q + u : kb - 1 - (q + u);
+                    av[u] = load_if(A + i + (size_t)(k0 + p) * lda, q + u < kb);
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (q + u < kb) {
+                        const int p = LOWER ? q + u : kb - 1 - (q + u);
+#pragma unroll
+                        for (int c = 0; c < NC; c++) {
+                            if (c >= nc) continue;
+                            const T xp = xb[p][c];
+                            if (s_iszero(xp)) continue;
+                            acc[c] = s_add(acc[c], s_mul(s_neg(xp), av[u]));
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                if (c < nc) X[i + (size_t)c * ldb] = acc[c];
+        }
+    }
+    if (blockIdx.x != 0 || nextb <= 0) return;
+    __syncthreads();
+    if (wave < nc) {
+        T* x = X + (size_t)wave * ldb;
+        const bool mine = lane < nextb;
+        T d[BS];
+#pragma unroll
+        for (int k = 0; k < BS; k++) d[k] = load_if(A + (next0 + lane) + (size_t)(next0 + k) * lda, mine && k < nextb);
+        T xi = load_if(x + next0 + lane, mine);
+#pragma unroll
+        for (int st = 0; st < BS; st++) {
+            const int p = LOWER ? st : BS - 1 - st;
+            if (p >= nextb) continue;
+            if (!UNIT && lane == p) xi = s_div(xi, d[p]);
+            const T xp = readlane_t(xi, p);
+            if (s_iszero(xp)) continue;
+            const bool ahead = LOWER ? (lane > p && lane < nextb) : (lane < p);
+            if (ahead) xi = s_add(xi, s_mul(s_neg(xp), d[p]));
+        }
+        if (mine) x[next0 + lane] = xi;
+    }
+}
+
 // trsv_lds_kernel for G right-hand sides at once, over columns already in
 // LDS (xs[c * n + i], c < nr <= G): every load of op(A) serves all of them.
 // Each column receives exactly trsv_lds_kernel's operations, in its order:
@@ -1322,7 +1408,8 @@ template <typename T>
 void getf2_reg(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* list, int* info)
 {
     if constexpr (sizeof(T) != 16) {
-        if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        if (m <= 512 && w <= 32) launch_getf2_reg<T, 32, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
+        else if (m <= 512) launch_getf2_reg<T, 64, 1>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 1024) launch_getf2_reg<T, 32, 2>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 2048) launch_getf2_reg<T, 16, 4>(ctx, m, w, a, lda, piv, off, perm, list, info);
         else if (m <= 4096) launch_getf2_reg<T, 8, 8>(ctx, m, w, a, lda, piv, off, perm, list, info);
@@ -1385,8 +1472,34 @@ template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb)
 {
     if (n <= 0 || nrhs <= 0) return;
+    if ((nrhs <= 32 || n <= 64) && TR == 0 && n >= 2048) {
+        // Spread over the chip, one launch per diagonal block (the same
+        // operations per element as trsv_lds_kernel).
+        constexpr int BS = tri_block<T>(), NC = 4;
+        const int nblk = (n + BS - 1) / BS;
+        const int gy = (nrhs + NC - 1) / NC;
+        int k0 = 0, kb = 0;
+        for (int bq = 0; bq <= nblk; bq++) {
+            int next0 = 0, nextb = 0;
+            if (bq < nblk) {
+                const int blk = LOWER ? bq : nblk - 1 - bq;
+                next0 = blk * BS;
+                nextb = n - next0 < BS ? n - next0 : BS;
+            }
+            const int rows = kb == 0 ? 0 : (LOWER ? n - k0 - kb : k0);
+            if (nextb == 0 && rows == 0) break;
+            const int slabs = rows > 0 ? (rows + 255) / 256 : 1;
+            hipLaunchKernelGGL((trsv_step_kernel<T, LOWER, UNIT, NC>), dim3(slabs, gy), dim3(256), 0, ctx->stream, n, A,
+                               lda, B, ldb, nrhs, k0, kb, next0, nextb);
+            k0 = next0;
+            kb = nextb;
+            if (nextb == 0) break;
+        }
+        return;
+    }
     if (nrhs <= 32 || n <= 64) {
         const size_t xbytes = sizeof(T) * (size_t)n;
+
         if (xbytes + sizeof(T) * tri_block<T>() <= (size_t)ctx->max_lds) {
             hipLaunchKernelGGL((trsv_lds_kernel<T, LOWER, TR, UNIT>), dim3(nrhs), dim3(256), xbytes, ctx->stream, n, A,
                                lda, B, ldb);
@@ -1465,13 +1578,14 @@ void copy2d(Context* ctx, int m, int n, const T* src, int lds, T* dst, int ldd)
 // column, solve and update the panel columns to its right.
 template <typename T>
 void panel(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* rc, int* info, int* list,
-           int left, int right)
+           int left, int right, int cap = 0)
 {
     if (m <= 0 || w <= 0) return;
     const int jl = m < w ? m : w;
     const int ncols = left + w + right;
     T* rows = a - (size_t)left * lda;  // column 0 of the matrix, panel's top row
-    const int cw = reg_panel_width<T>(m);
+    int cw = reg_panel_width<T>(m);
+    if (cap > 0 && cw > cap) cw = cap;  // narrower chunks: a shorter update per column
     if (cw == 0) {
         hipLaunchKernelGGL((getf2_global_kernel<T>), dim3(1), dim3(1024), 0, ctx->stream, m, w, a, lda, piv, off, perm,
                            rc, info);
@@ -1555,7 +1669,7 @@ void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_per
     for (int jb = 0; jb < n; jb += NB) {
         const int w = NB < n - jb ? NB : n - jb;
         T* pa = a + jb + (size_t)jb * lda;
-        panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb, n - jb - w);
+        panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb, n - jb - w, 32);
         hipLaunchKernelGGL((tri_inverse_kernel<T, NB>), dim3(1), dim3(2 * NB), 0, ctx->stream, n, a, lda, jb / NB, inv);
         const int rest = n - jb - w;
         if (rest <= 0) continue;
@@ -1585,22 +1699,39 @@ __global__ void __launch_bounds__(256) trsv_super_kernel(int r0, int rows, int t
 {
     extern __shared__ unsigned char super_smem[];
     T* xs = reinterpret_cast<T*>(super_smem);  // xs[i * TC + c], i < rows
-    __shared__ T iv[W][W + 1];
+    __shared__ T iv[2][W][W + 1];              // this block's inverse, and the next one's
+    constexpr int IVT = W * W / 256;           // inverse entries per thread
     const int tid = threadIdx.x, c0 = blockIdx.x * TC;
     const int nc = t - c0 < TC ? t - c0 : TC;
+    const int nb = rows / W;
+    auto inverse_of = [&](int s) {
+        const int jb = LOWER ? s : nb - 1 - s;
+        return inv + (size_t)((r0 + jb * W) / W) * 2 * W * W + (LOWER ? 0 : W * W);
+    };
     for (int e = tid; e < rows * TC; e += 256) {
         const int i = e / TC, c = e % TC;
         xs[e] = load_if(x + r0 + i + (size_t)(c0 + c) * ldx, c < nc);
     }
-    const int nb = rows / W;
+    {
+        const T* src = inverse_of(0);
+#pragma unroll
+        for (int u = 0; u < IVT; u++) {
+            const int e = tid + u * 256;
+            iv[0][e % W][e / W] = src[e];
+        }
+    }
+    __syncthreads();
     for (int s = 0; s < nb; s++) {
         const int jb = LOWER ? s : nb - 1 - s;
-        const int k0 = jb * W;
-        const T* src = inv + (size_t)((r0 + k0) / W) * 2 * W * W + (LOWER ? 0 : W * W);
-        __syncthreads();
-        for (int e = tid; e < W * W; e += 256) iv[e % W][e / W] = src[e];
-        __syncthreads();
-        // x_j := inv_jj x_j, two outputs per thread.
+        const int k0 = jb * W, buf = s & 1;
+        // The next block's inverse, in flight while this block is worked.
+        T nxt[IVT];
+        if (s + 1 < nb) {
+            const T* src = inverse_of(s + 1);
+#pragma unroll
+            for (int u = 0; u < IVT; u++) nxt[u] = src[tid + u * 256];
+        }
+        // x_j := inv_jj x_j.
         T y[(W * TC + 255) / 256];
 #pragma unroll
         for (int u = 0; u < (W * TC + 255) / 256; u++) {
@@ -1608,7 +1739,7 @@ __global__ void __launch_bounds__(256) trsv_super_kernel(int r0, int rows, int t
             const int i = e / TC, c = e % TC;
             T acc = s_zero(T{});
             if (e < W * TC)
-                for (int p = 0; p < W; p++) acc = s_add(acc, s_mul(iv[i][p], xs[(k0 + p) * TC + c]));
+                for (int p = 0; p < W; p++) acc = s_add(acc, s_mul(iv[buf][i][p], xs[(k0 + p) * TC + c]));
             y[u] = acc;
         }
         __syncthreads();
@@ -1619,30 +1750,34 @@ __global__ void __launch_bounds__(256) trsv_super_kernel(int r0, int rows, int t
         }
         __syncthreads();
         // The block's other rows still to come: one row per thread, all TC
-        // columns, entries of the factor read once each.
+        // columns, the row's W entries of the factor loaded together.
         const int lo = LOWER ? k0 + W : 0, hi = LOWER ? rows : k0;
         for (int i = lo + tid; i < hi; i += 256) {
+            const T* f = lu + (r0 + i) + (size_t)(r0 + k0) * ld;
+            T a[W];
+#pragma unroll
+            for (int p = 0; p < W; p++) a[p] = f[(size_t)p * ld];
             T acc[TC];
 #pragma unroll
             for (int c = 0; c < TC; c++) acc[c] = xs[i * TC + c];
-            const T* f = lu + (r0 + i) + (size_t)(r0 + k0) * ld;
-            // Sixteen entries of the row in flight at a time.
-            for (int p0 = 0; p0 < W; p0 += 16) {
-                T a[16];
 #pragma unroll
-                for (int u = 0; u < 16; u++) a[u] = f[(size_t)(p0 + u) * ld];
+            for (int p = 0; p < W; p++) {
+                const T na = s_neg(a[p]);
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const T na = s_neg(a[u]);
-#pragma unroll
-                    for (int c = 0; c < TC; c++) acc[c] = s_add(acc[c], s_mul(na, xs[(k0 + p0 + u) * TC + c]));
-                }
+                for (int c = 0; c < TC; c++) acc[c] = s_add(acc[c], s_mul(na, xs[(k0 + p) * TC + c]));
             }
 #pragma unroll
             for (int c = 0; c < TC; c++) xs[i * TC + c] = acc[c];
         }
+        if (s + 1 < nb) {
+#pragma unroll
+            for (int u = 0; u < IVT; u++) {
+                const int e = tid + u * 256;
+                iv[buf ^ 1][e % W][e / W] = nxt[u];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int e = tid; e < rows * TC; e += 256) {
         const int i = e / TC, c = e % TC;
         if (c < nc) x[r0 + i + (size_t)(c0 + c) * ldx] = xs[e];
