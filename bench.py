@@ -268,6 +268,26 @@ def grad_leg(ws, sizes=(256, 1024)):
     return out
 
 
+def expm_leg(ws, n=1000, steps=3):
+    """The benchmark step alone (complex Expm, n = 1000, ||A||_1 = 50), one
+    warm-up and `steps` timed, for rocprofv3."""
+    import torch
+
+    from csharp_linearalgebra_amd import workloads as W
+
+    a, _ = W.expm_operator(n, 50.0, 8, seed=0)
+    dA = to_device(a)
+    dE = empty_device(n, n, True)
+    torch.cuda.synchronize()
+    ws.expm(dA, out=dE)
+    ws.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ws.expm(dA, out=dE)
+    ws.synchronize()
+    return {"expm_ms": round((time.perf_counter() - t0) / steps * 1e3, 3)}
+
+
 def kernels_leg(ws, n=8192):
     """The memory-bound parts on their own, for rocprofv3: the 1-norm and
     the infinity-norm of an n x n FP64 matrix (8 n^2 bytes each), the unit
@@ -314,7 +334,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--roofline-only", action="store_true", help="run only the dominant kernel (for rocprofv3)")
-    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels"],
+    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels", "expm"],
                     help="run one extra leg alone and print its JSON (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -333,6 +353,8 @@ def main():
             out = mixed_leg(ws, 8192, (1e2,))
         elif args.leg == "kernels":
             out = kernels_leg(ws)
+        elif args.leg == "expm":
+            out = expm_leg(ws)
         else:
             out = grad_leg(ws)
         print(json.dumps(out), flush=True)
