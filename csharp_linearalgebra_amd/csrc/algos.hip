@@ -844,27 +844,43 @@ void run(Context* ctx, const DevOp<T>& op, const T* dense, int lda, int nb, cons
     double hn[2];
 
     for (int i = 0; i < s; i++) {
-        zero_scalars(ctx, d_norms + 2, 1);
+        zero_scalars(ctx, d_norms, 3);
         panel_infnorm<T>(ctx, n, nb, work.get(), d_norms + 2);
-        double c1 = read_double(ctx, d_norms + 2);
-        for (int j = 0; j < m; j++) {
-            const double coef = t / (s * (double)(j + 1));
-            zero_scalars(ctx, d_norms, 2);
-            if (fused) {
-                panel_n_partials<T>(ctx, n, nb, dense, lda, work.get(), n, part.get(), ks);
-                taylor_step<T>(ctx, n, nb, ks, part.get(), coef, work.get(), f.get(), d_norms);
-            } else {
+        if (fused) {
+            // All m steps are queued at once; the exit test runs on the
+            // device after each, and once it holds the stage's remaining
+            // launches return at once. One read-back per stage.
+            int* d_state = ctx->d_int + 8;
+            TNS_HIP(hipMemsetAsync(d_state, 0, 2 * sizeof(int), ctx->stream));
+            for (int j = 0; j < m; j++) {
+                const double coef = t / (s * (double)(j + 1));
+                panel_n_partials<T>(ctx, n, nb, dense, lda, work.get(), n, part.get(), ks, d_state);
+                taylor_step<T>(ctx, n, nb, ks, part.get(), coef, work.get(), f.get(), d_norms, d_state);
+                taylor_exit(ctx, d_norms, d_state, kUnitRoundoff);
+            }
+            int hs[2] = {0, 0};
+            TNS_HIP(hipMemcpyAsync(hs, d_state, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+            TNS_HIP(hipStreamSynchronize(ctx->stream));
+            if (diag) {
+                diag->applications += hs[1];
+                diag->early_exits += hs[0];
+            }
+        } else {
+            double c1 = read_double(ctx, d_norms + 2);
+            for (int j = 0; j < m; j++) {
+                const double coef = t / (s * (double)(j + 1));
+                zero_scalars(ctx, d_norms, 2);
                 op.apply(nb, work.get(), next.get());
                 taylor_step<T>(ctx, n, nb, 1, next.get(), coef, work.get(), f.get(), d_norms);
+                read_doubles(ctx, d_norms, hn, 2);
+                const double c2 = hn[0];
+                if (diag) diag->applications++;
+                if (c1 + c2 <= kUnitRoundoff * hn[1]) {
+                    if (diag) diag->early_exits++;
+                    break;
+                }
+                c1 = c2;
             }
-            read_doubles(ctx, d_norms, hn, 2);
-            const double c2 = hn[0];
-            if (diag) diag->applications++;
-            if (c1 + c2 <= kUnitRoundoff * hn[1]) {
-                if (diag) diag->early_exits++;
-                break;
-            }
-            c1 = c2;
         }
         if (!unit_eta) mul_scalar<T>(ctx, extent, f.get(), eta);
         copy_t<T>(ctx, n, nb, f.get(), n, work.get(), n);

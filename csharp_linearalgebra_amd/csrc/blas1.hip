@@ -121,8 +121,9 @@ constexpr int PN_KL = 4;      // k-lanes per workgroup (one wavefront each)
 template <typename T, int TMAX>
 __global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __restrict__ a, int lda,
                                                       const T* __restrict__ x, int ldx, int kchunk,
-                                                      T* __restrict__ part)
+                                                      T* __restrict__ part, const int* __restrict__ skip)
 {
+    if (skip != nullptr && *skip) return;  // a finished Taylor stage: no work
     __shared__ T red[PN_KL][TMAX][PN_ROWS];
     const int lane = threadIdx.x & 63, kl = threadIdx.x >> 6;
     const int i = blockIdx.x * PN_ROWS + lane;
@@ -299,8 +300,9 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v)
 template <typename T>
 __global__ void __launch_bounds__(256) taylor_step_kernel(int n, int nb, int ksplit, const T* __restrict__ part,
                                                           double coef, T* __restrict__ work, T* __restrict__ f,
-                                                          double* __restrict__ norms)
+                                                          double* __restrict__ norms, const int* __restrict__ skip)
 {
+    if (skip != nullptr && *skip) return;  // a finished Taylor stage: no work
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double sw = 0.0, sf = 0.0;
@@ -329,6 +331,23 @@ __global__ void __launch_bounds__(256) panel_infnorm_kernel(int n, int nb, const
     double s = 0.0;
     for (int c = 0; c < nb; c++) s += s_abs(x[i + (size_t)c * n]);
     atomic_max_nonneg(out, s);
+}
+
+// After a Taylor step of a stage still running (state[0] == 0): one more
+// application; the stage ends (state[0] = 1) when c1 + c2 <= unit * ||F||
+// (c1 the previous step's ||work||inf, c2 this one's, in norms[2] and
+// norms[0], ||F||inf in norms[1]), else c1 := c2. norms[0..1] are zeroed
+// for the next step either way. One thread.
+__global__ void taylor_exit_kernel(double* norms, int* state, double unit)
+{
+    if (state[0] == 0) {
+        const double c2 = norms[0];
+        state[1] += 1;
+        if (norms[2] + c2 <= unit * norms[1]) state[0] = 1;
+        else norms[2] = c2;
+    }
+    norms[0] = 0.0;
+    norms[1] = 0.0;
 }
 
 __global__ void zero_doubles_kernel(double* p, int count)
@@ -382,19 +401,23 @@ inline int ksplit_for(int n)
 }
 
 template <typename T>
-void panel_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks)
+void panel_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks,
+                    const int* skip = nullptr)
 {
     int kchunk = (n + ks - 1) / ks;
     dim3 grid((n + PN_ROWS - 1) / PN_ROWS, ks);
     if (t <= 2)
-        hipLaunchKernelGGL((panel_n_kernel<T, 2>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part);
+        hipLaunchKernelGGL((panel_n_kernel<T, 2>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part,
+                           skip);
     else if (t <= 4)
-        hipLaunchKernelGGL((panel_n_kernel<T, 4>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part);
+        hipLaunchKernelGGL((panel_n_kernel<T, 4>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part,
+                           skip);
     else if (t <= 8 || sizeof(T) != 8)
-        hipLaunchKernelGGL((panel_n_kernel<T, 8>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part);
+        hipLaunchKernelGGL((panel_n_kernel<T, 8>), grid, dim3(256), 0, ctx->stream, n, t, a, lda, x, ldx, kchunk, part,
+                           skip);
     else
         hipLaunchKernelGGL((panel_n_kernel<T, (sizeof(T) == 8 ? 16 : 8)>), grid, dim3(256), 0, ctx->stream, n, t, a,
-                           lda, x, ldx, kchunk, part);
+                           lda, x, ldx, kchunk, part, skip);
 }
 
 template <typename T>
@@ -546,21 +569,30 @@ template int panel_ksplit<double>(int);
 template int panel_ksplit<zd>(int);
 
 template <typename T>
-void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks)
+void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks,
+                      const int* skip)
 {
-    panel_partials<T>(ctx, n, t, a, lda, x, ldx, part, ks);
+    panel_partials<T>(ctx, n, t, a, lda, x, ldx, part, ks, skip);
 }
-template void panel_n_partials<double>(Context*, int, int, const double*, int, const double*, int, double*, int);
-template void panel_n_partials<zd>(Context*, int, int, const zd*, int, const zd*, int, zd*, int);
+template void panel_n_partials<double>(Context*, int, int, const double*, int, const double*, int, double*, int,
+                                       const int*);
+template void panel_n_partials<zd>(Context*, int, int, const zd*, int, const zd*, int, zd*, int, const int*);
+
+void taylor_exit(Context* ctx, double* d_norms, int* d_state, double unit)
+{
+    hipLaunchKernelGGL(taylor_exit_kernel, dim3(1), dim3(1), 0, ctx->stream, d_norms, d_state, unit);
+}
 
 template <typename T>
-void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms)
+void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms,
+                 const int* skip)
 {
     hipLaunchKernelGGL((taylor_step_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nb, ks, part,
-                       coef, work, f, d_norms);
+                       coef, work, f, d_norms, skip);
 }
-template void taylor_step<double>(Context*, int, int, int, const double*, double, double*, double*, double*);
-template void taylor_step<zd>(Context*, int, int, int, const zd*, double, zd*, zd*, double*);
+template void taylor_step<double>(Context*, int, int, int, const double*, double, double*, double*, double*,
+                                  const int*);
+template void taylor_step<zd>(Context*, int, int, int, const zd*, double, zd*, zd*, double*, const int*);
 
 template <typename T>
 void panel_infnorm(Context* ctx, int n, int nb, const T* x, double* d_out)

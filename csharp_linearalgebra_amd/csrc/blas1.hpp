@@ -45,13 +45,20 @@ void zero_scalars(Context* ctx, double* d, int count);
 // second pass: partials into part (ks * t * n), then a finishing kernel.
 template <typename T>
 int panel_ksplit(int n);
+// skip (device, may be null): when *skip != 0 the launch does nothing.
 template <typename T>
-void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks);
+void panel_n_partials(Context* ctx, int n, int t, const T* a, int lda, const T* x, int ldx, T* part, int ks,
+                      const int* skip = nullptr);
 // Second pass fused with the Taylor step: work = coef * sum(part),
 // f += work, d_norms[0] = max(d_norms[0], ||work||_inf),
 // d_norms[1] = max(d_norms[1], ||f||_inf). Panels n x nb with ld n.
 template <typename T>
-void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms);
+void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms,
+                 const int* skip = nullptr);
+// The early-exit test of a Taylor stage on the device (one thread):
+// d_state[0] the stage's finished flag, d_state[1] its applications;
+// d_norms[0..2] = this step's ||work||, ||F||, the previous ||work||.
+void taylor_exit(Context* ctx, double* d_norms, int* d_state, double unit);
 // d_out[0] = max(d_out[0], ||x||_inf) for an n x nb panel with ld n.
 template <typename T>
 void panel_infnorm(Context* ctx, int n, int nb, const T* x, double* d_out);
