@@ -288,6 +288,26 @@ def expm_leg(ws, n=1000, steps=3):
     return {"expm_ms": round((time.perf_counter() - t0) / steps * 1e3, 3)}
 
 
+def expmv_leg(ws, n=1000, reps=3):
+    """The Expmv of the report alone (complex, n = 1000, B n x 8, t = 0.5),
+    one warm-up and `reps` timed, for rocprofv3."""
+    import torch
+
+    from csharp_linearalgebra_amd import workloads as W
+
+    a, b = W.expm_operator(n, 50.0, 8, seed=0)
+    dA, dB = to_device(a), to_device(b)
+    dF = empty_device(n, 8, True)
+    torch.cuda.synchronize()
+    ws.expmv(dA, dB, 0.5, out=dF)
+    ws.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ws.expmv(dA, dB, 0.5, out=dF)
+    ws.synchronize()
+    return {"expmv_ms": round((time.perf_counter() - t0) / reps * 1e3, 3)}
+
+
 def kernels_leg(ws, n=8192):
     """The memory-bound parts on their own, for rocprofv3: the 1-norm and
     the infinity-norm of an n x n FP64 matrix (8 n^2 bytes each), the unit
@@ -334,7 +354,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--roofline-only", action="store_true", help="run only the dominant kernel (for rocprofv3)")
-    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels", "expm"],
+    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels", "expm", "expmv"],
                     help="run one extra leg alone and print its JSON (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -355,6 +375,8 @@ def main():
             out = kernels_leg(ws)
         elif args.leg == "expm":
             out = expm_leg(ws)
+        elif args.leg == "expmv":
+            out = expmv_leg(ws)
         else:
             out = grad_leg(ws)
         print(json.dumps(out), flush=True)
