@@ -703,54 +703,6 @@ trsm_grouped_kernel(const TrsmRec<T>* __restrict__ recs, const int* __restrict__
                                                                           blockIdx.x - r.start);
 }
 
-// The diagonal-block solve of trsm_block_kernel (op = none) with one thread
-// per right-hand side: each thread's column of B in registers, the block of
-// op(A) in LDS read by every thread at once, so the BS steps need no
-// barriers. The workgroup's CW columns pass through LDS so the global reads
-// and writes stay coalesced. Same operations per element, in the same
-// order, as trsm_block_kernel (the reference's column-oriented
-// substitution, zero entries of x skipped).
-template <typename T, int BS, bool LOWER, bool UNIT>
-__global__ void __launch_bounds__(128) trsm_reg_kernel(int w, int ncols, const T* __restrict__ A, int lda,
-                                                       T* __restrict__ B, int ldb)
-{
-    constexpr int CW = 128;
-    __shared__ T D[BS][BS + 1];   // D[i][p] = A(i, p)
-    __shared__ T Xs[BS][CW + 1];  // Xs[i][c]: row i of the workgroup's column c
-    const int tid = threadIdx.x, c0 = blockIdx.x * CW;
-    for (int e = tid; e < BS * BS; e += CW) {
-        const int i = e % BS, p = e / BS;
-        D[i][p] = load_if(A + i + (size_t)p * lda, i < w && p < w);
-    }
-    for (int e = tid; e < BS * CW; e += CW) {
-        const int i = e % BS, c = e / BS;
-        Xs[i][c] = load_if(B + i + (size_t)(c0 + c) * ldb, i < w && c0 + c < ncols);
-    }
-    __syncthreads();
-    T x[BS];
-#pragma unroll
-    for (int i = 0; i < BS; i++) x[i] = Xs[i][tid];
-    // Rows and columns at or beyond w hold zeros: a zero x_p is skipped as
-    // the reference skips it, and rows beyond w are never stored.
-#pragma unroll
-    for (int st = 0; st < BS; st++) {
-        const int p = LOWER ? st : BS - 1 - st;
-        if (!UNIT && p < w) x[p] = s_div(x[p], D[p][p]);
-        if (!s_iszero(x[p])) {
-            const T nxp = s_neg(x[p]);
-#pragma unroll
-            for (int i = LOWER ? p + 1 : 0; i < (LOWER ? BS : p); i++) x[i] = s_add(x[i], s_mul(nxp, D[i][p]));
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < BS; i++) Xs[i][tid] = x[i];
-    __syncthreads();
-    for (int e = tid; e < BS * CW; e += CW) {
-        const int i = e % BS, c = e / BS;
-        if (i < w && c0 + c < ncols) B[i + (size_t)(c0 + c) * ldb] = Xs[i][c];
-    }
-}
-
 // Whole triangular solve for few right-hand sides: one workgroup of 256
 // threads per column of B. op(A) lower runs forward, upper backward, in
 // BS-row blocks. Each block's diagonal part is staged in LDS by all threads
@@ -1509,16 +1461,10 @@ void sync_columns(Context* ctx, int m, int base, T* a, int lda, int c0, int c1, 
 template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ldb)
 {
-    if constexpr (TR == 0) {
-        const int blocks = (ncols + 127) / 128;
-        hipLaunchKernelGGL((trsm_reg_kernel<T, tri_block<T>(), LOWER, UNIT>), dim3(blocks), dim3(128), 0, ctx->stream,
-                           w, ncols, A, lda, B, ldb);
-    } else {
-        constexpr int CB = 32;
-        const int blocks = (ncols + CB - 1) / CB;
-        hipLaunchKernelGGL((trsm_block_kernel<T, tri_block<T>(), CB, LOWER, TR, UNIT>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, w, ncols, A, lda, B, ldb);
-    }
+    constexpr int CB = 32;
+    const int blocks = (ncols + CB - 1) / CB;
+    hipLaunchKernelGGL((trsm_block_kernel<T, tri_block<T>(), CB, LOWER, TR, UNIT>), dim3(blocks), dim3(256), 0,
+                       ctx->stream, w, ncols, A, lda, B, ldb);
 }
 
 // op(A) X = B in place, A n x n triangular.
