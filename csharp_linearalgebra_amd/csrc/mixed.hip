@@ -5,11 +5,13 @@
 // 64 x 64 diagonal block (products on the FP32 matrix cores, or on the BF16
 // ones with each operand split into two bfloat16 parts), then
 // iterative refinement in FP64 — residuals r = b - A x
-// and the updates of x in FP64, corrections from the low-precision factors
-// — until the normwise backward error max_j ||r_j||inf / (||A||inf ||x_j||inf
-// + ||b_j||inf) is at most the tolerance. If a step fails to cut it by a
-// tenth, or a pivot is exactly zero in low precision, the solve falls back
-// to the FP64 LU with partial pivoting of lu.hip.
+// and the updates of x in FP64, corrections by FGMRES right-preconditioned
+// with the low-precision factors — until the normwise backward error
+// max_j ||r_j||inf / (||A||inf ||x_j||inf + ||b_j||inf) is at most the
+// tolerance. If a restart cycle fails to halve it, the step budget runs
+// out, the residuals shrink too slowly to beat the FP64 solve's cost, or a
+// pivot is exactly zero in low precision, the solve falls back to the FP64
+// LU with partial pivoting of lu.hip.
 //
 // The randomization (Parker, "Random butterfly transformations with
 // applications in computational linear algebra", UCLA CSD-950023, 1995;
@@ -240,10 +242,225 @@ int grid_for(size_t total)
     return (int)(b < 8192 ? b : 8192);
 }
 
+// Krylov blocks: `count` n x t FP64 matrices with leading dimension n, vs
+// apart. Workgroup (i, j): h[i + j ldh] = <v_i[:, j], w[:, j]>, one fixed
+// summation order.
+__global__ void __launch_bounds__(256) block_dots_kernel(int n, const double* __restrict__ v, size_t vs,
+                                                         const double* __restrict__ w, int ldw, double* __restrict__ h,
+                                                         int ldh)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x, j = blockIdx.y;
+    const double* vi = v + i * vs + (size_t)j * n;
+    const double* wj = w + (size_t)j * ldw;
+    double s = 0.0;
+    for (int row = threadIdx.x; row < n; row += 256) s += vi[row] * wj[row];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) h[i + (size_t)j * ldh] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+// w[:, j] += sign * sum_{i < count} v_i[:, j] c[i + j ldc] (sign +1 or -1).
+__global__ void __launch_bounds__(256) block_combine_kernel(int n, const double* __restrict__ v, size_t vs, int count,
+                                                            const double* __restrict__ c, int ldc, int sign,
+                                                            double* __restrict__ w, int ldw)
+{
+    const int row = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int i = 0; i < count; i++) acc += v[i * vs + (size_t)j * n + row] * c[i + (size_t)j * ldc];
+    double* p = w + row + (size_t)j * ldw;
+    *p = sign > 0 ? *p + acc : *p - acc;
+}
+// w[:, j] *= s[j].
+__global__ void __launch_bounds__(256) col_mul_kernel(int n, double* __restrict__ w, int ldw,
+                                                      const double* __restrict__ s)
+{
+    const int row = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (row < n) w[row + (size_t)j * ldw] *= s[j];
+}
+
+// The low-precision solve z := M^-1 v for t columns: M^-1 = C V (LU)^-1 U^T R
+// applied through the FP32 factors of the transformed matrix, each column
+// scaled by its largest entry on the way into FP32.
+struct LowSolve {
+    Context* ctx;
+    int n, n4, t;
+    const float* s;
+    const int* perm;
+    const float* inv;
+    const double *rs, *cs, *ub, *vb;
+    float* d;    // n4 x t
+    double* sc;  // t
+    // accumulate: z += M^-1 v; otherwise z := M^-1 v.
+    void apply(const double* v, int ldv, double* z, int ldz, bool accumulate) const
+    {
+        const int q = n4 / 4;
+        const dim3 cols((q + 255) / 256, t);
+        hipLaunchKernelGGL(col_absmax_kernel, dim3(t), dim3(256), 0, ctx->stream, n, v, ldv, sc);
+        hipLaunchKernelGGL(rbt_rhs_kernel, cols, dim3(256), 0, ctx->stream, n, n4, v, ldv, rs, sc, ub, d);
+        const int saved = ctx->lp_mode;
+        ctx->lp_mode = LP_FP32;  // the solves' products in FP32 in both precisions
+        try {
+            sgetrs_local(ctx, n4, t, s, n4, perm, inv, d, n4);
+        } catch (...) {
+            ctx->lp_mode = saved;
+            throw;
+        }
+        ctx->lp_mode = saved;
+        if (!accumulate) TNS_HIP(hipMemset2DAsync(z, sizeof(double) * (size_t)ldz, 0, sizeof(double) * (size_t)n, t,
+                                                  ctx->stream));
+        hipLaunchKernelGGL(rbt_accumulate_kernel, cols, dim3(256), 0, ctx->stream, n, n4, d, vb, cs, sc, z, ldz);
+        TNS_HIP(hipGetLastError());
+    }
+};
+
+// One restart cycle of FGMRES (Saad, "A flexible inner-outer preconditioned
+// GMRES algorithm", SIAM J. Sci. Comput. 14(2), 1993) on A e = r, right-
+// preconditioned by the low-precision solve, every column its own Krylov
+// space, all columns in step: z_k = M^-1 v_k, w = A z_k (FP64), w
+// orthogonalized against v_0..v_k by classical Gram-Schmidt applied twice,
+// the small least-squares problems by Givens rotations on the host; then
+// x += Z y. Because e is taken from the span of the z_k actually computed,
+// the low-precision solves may differ from one step to the next, and the
+// residual is minimized for the true A in FP64. A column stops when its
+// residual estimate |g_{k+1}| is at most target[j] or its Krylov space is
+// exhausted. Returns the steps (low-precision solves) taken, at most m.
+//
+// From the fourth step on, the cycle gives up (hopeless = true, x left as
+// it was) when the worst column's residual-to-target ratio, shrinking at
+// its rate over the last three steps, would take longer to reach 1 at the
+// measured time per step than fallback_ms, the estimated cost of the FP64
+// solve that replaces it.
+int fgmres_cycle(const LowSolve& M, const double* a, int lda, const double* r, double* x, int ldx, int m,
+                 const std::vector<double>& target, double* V, double* Z, double* d_h, double fallback_ms,
+                 bool* hopeless)
+{
+    using clock = std::chrono::steady_clock;
+    const auto t_start = clock::now();
+    *hopeless = false;
+    std::vector<double> ratio;
+    Context* ctx = M.ctx;
+    const int n = M.n, t = M.t, ldh = m + 1;
+    const size_t vs = (size_t)n * t;
+    const dim3 rows((n + 255) / 256, t);
+    // d_h: h1 and h2 ((m+1) x t each: the two Gram-Schmidt passes), the
+    // squared norms (t), the scales (t), y (m x t).
+    double* h1 = d_h;
+    double* h2 = h1 + (size_t)ldh * t;
+    double* hn = h2 + (size_t)ldh * t;
+    double* dsc = hn + t;
+    double* dy = dsc + t;
+    const size_t back = 2 * (size_t)ldh * t + t;  // read back per step
+
+    // v_0 = r / ||r||.
+    dcopy2d(ctx, n, t, r, n, V, n);
+    hipLaunchKernelGGL(block_dots_kernel, dim3(1, t), dim3(256), 0, ctx->stream, n, V, vs, V, n, hn, 1);
+    std::vector<double> hb(back), scale(t);
+    read_doubles(ctx, hn, hb.data(), t);
+    std::vector<double> R((size_t)ldh * m * t, 0.0), cr((size_t)m * t), sr((size_t)m * t), g((size_t)ldh * t, 0.0);
+    std::vector<int> basis(t, 0);
+    std::vector<char> active(t);
+    for (int j = 0; j < t; j++) {
+        const double beta = std::sqrt(hb[j]);
+        active[j] = beta > target[j] && beta == beta;
+        g[(size_t)j * ldh] = beta;
+        scale[j] = active[j] ? 1.0 / beta : 0.0;
+    }
+    int steps = 0;
+    for (int k = 0; k < m; k++) {
+        bool any = false;
+        for (int j = 0; j < t; j++) any |= active[j] != 0;
+        if (!any) break;
+        double* vk = V + (size_t)k * vs;
+        double* w = V + (size_t)(k + 1) * vs;
+        double* zk = Z + (size_t)k * vs;
+        TNS_HIP(hipMemcpyAsync(dsc, scale.data(), sizeof(double) * t, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(col_mul_kernel, rows, dim3(256), 0, ctx->stream, n, vk, n, dsc);
+        M.apply(vk, n, zk, n, false);
+        dgemm_ex(ctx, 0, n, t, n, 1.0, a, lda, zk, n, 0.0, w, n);
+        hipLaunchKernelGGL(block_dots_kernel, dim3(k + 1, t), dim3(256), 0, ctx->stream, n, V, vs, w, n, h1, ldh);
+        hipLaunchKernelGGL(block_combine_kernel, rows, dim3(256), 0, ctx->stream, n, V, vs, k + 1, h1, ldh, -1, w, n);
+        hipLaunchKernelGGL(block_dots_kernel, dim3(k + 1, t), dim3(256), 0, ctx->stream, n, V, vs, w, n, h2, ldh);
+        hipLaunchKernelGGL(block_combine_kernel, rows, dim3(256), 0, ctx->stream, n, V, vs, k + 1, h2, ldh, -1, w, n);
+        hipLaunchKernelGGL(block_dots_kernel, dim3(1, t), dim3(256), 0, ctx->stream, n, w, vs, w, n, hn, 1);
+        TNS_HIP(hipGetLastError());
+        read_doubles(ctx, h1, hb.data(), (int)back);
+        steps++;
+        double worst = 0.0;
+        for (int j = 0; j < t; j++) {
+            if (!active[j]) {
+                scale[j] = 0.0;
+                continue;
+            }
+            double* col = R.data() + ((size_t)j * m + k) * ldh;
+            for (int i = 0; i <= k; i++) col[i] = hb[i + (size_t)j * ldh] + hb[(size_t)ldh * t + i + (size_t)j * ldh];
+            const double sub = std::sqrt(hb[2 * (size_t)ldh * t + j]);
+            double* cj = cr.data() + (size_t)j * m;
+            double* sj = sr.data() + (size_t)j * m;
+            for (int i = 0; i < k; i++) {
+                const double u = cj[i] * col[i] + sj[i] * col[i + 1];
+                col[i + 1] = -sj[i] * col[i] + cj[i] * col[i + 1];
+                col[i] = u;
+            }
+            const double den = std::hypot(col[k], sub);
+            if (!(den > 0.0)) {  // no progress possible in this column
+                active[j] = 0;
+                scale[j] = 0.0;
+                continue;
+            }
+            cj[k] = col[k] / den;
+            sj[k] = sub / den;
+            col[k] = den;
+            double* gj = g.data() + (size_t)j * ldh;
+            gj[k + 1] = -sj[k] * gj[k];
+            gj[k] = cj[k] * gj[k];
+            basis[j] = k + 1;
+            const bool done = !(std::fabs(gj[k + 1]) > target[j]) || !(sub > 0.0);
+            if (!done) worst = std::max(worst, std::fabs(gj[k + 1]) / target[j]);
+            active[j] = !done;
+            scale[j] = done ? 0.0 : 1.0 / sub;
+        }
+        ratio.push_back(worst);
+        if (k >= 3 && worst > 1.0) {
+            const double rate = std::pow(worst / ratio[k - 3], 1.0 / 3.0);
+            const double need = rate < 1.0 ? std::log(worst) / -std::log(rate) : INFINITY;
+            const double per_step = std::chrono::duration<double, std::milli>(clock::now() - t_start).count() / steps;
+            if (!(need * per_step <= fallback_ms)) {
+                *hopeless = true;
+                return steps;
+            }
+        }
+    }
+    // y = R^-1 g per column, then x += Z y.
+    int kmax = 0;
+    std::vector<double> y((size_t)m * t, 0.0);
+    for (int j = 0; j < t; j++) {
+        const int kb = basis[j];
+        kmax = std::max(kmax, kb);
+        const double* Rj = R.data() + (size_t)j * m * ldh;
+        double* yj = y.data() + (size_t)j * m;
+        for (int i = kb - 1; i >= 0; i--) {
+            double s = g[i + (size_t)j * ldh];
+            for (int l = i + 1; l < kb; l++) s -= Rj[i + (size_t)l * ldh] * yj[l];
+            yj[i] = s / Rj[i + (size_t)i * ldh];
+        }
+    }
+    if (kmax > 0) {
+        TNS_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
+        TNS_HIP(hipStreamSynchronize(ctx->stream));  // y is a host temporary
+        hipLaunchKernelGGL(block_combine_kernel, rows, dim3(256), 0, ctx->stream, n, Z, vs, kmax, dy, m, 1, x, ldx);
+        TNS_HIP(hipGetLastError());
+    }
+    return steps;
+}
+
 // Backward error of the current x: r := b - A x, then
 // max_j ||r_j|| / (||A|| ||x_j|| + ||b_j||). NaN if any part is NaN.
 double backward_error(Context* ctx, int n, int t, const double* a, int lda, const double* b, int ldb, const double* x,
-                      int ldx, double* r, double anorm, const std::vector<double>& bnorm, double* d_tmp)
+                      int ldx, double* r, double anorm, const std::vector<double>& bnorm, double* d_tmp,
+                      std::vector<double>* xnorm = nullptr)
 {
     dcopy2d(ctx, n, t, b, ldb, r, n);
     dgemm_ex(ctx, 0, n, t, n, -1.0, a, lda, x, ldx, 1.0, r, n);
@@ -252,6 +469,7 @@ double backward_error(Context* ctx, int n, int t, const double* a, int lda, cons
     TNS_HIP(hipGetLastError());
     std::vector<double> h(2 * (size_t)t);
     read_doubles(ctx, d_tmp, h.data(), 2 * t);
+    if (xnorm) xnorm->assign(h.begin() + t, h.end());
     double worst = 0.0;
     for (int j = 0; j < t; j++) {
         const double den = anorm * h[t + j] + bnorm[j];
@@ -338,13 +556,15 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
         TNS_HIP(hipGetLastError());
         DevBuf<int> piv(ctx, n4), perm(ctx, n4), dinfo(ctx, 1);
         DevBuf<float> inv(ctx, (size_t)(n4 / blk) * 2 * blk * blk);
-        const int saved = ctx->lp_mode;
-        ctx->lp_mode = mode;
-        try {
+        {
+            // The factorization's products in the requested precision.
+            struct Mode {
+                Context* c;
+                int saved;
+                ~Mode() { c->lp_mode = saved; }
+            } restore{ctx, ctx->lp_mode};
+            ctx->lp_mode = mode;
             sgetrf_local(ctx, n4, s.get(), n4, piv.get(), perm.get(), inv.get(), dinfo.get());
-        } catch (...) {
-            ctx->lp_mode = saved;
-            throw;
         }
         int hinfo = -1;
         TNS_HIP(hipMemcpyAsync(&hinfo, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -353,39 +573,51 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
         singular = hinfo >= 0;
 
         if (!singular) {
-            // x = 0, r = b; each step: d = V (LU \ U^T r) (low precision),
-            // x += d, r = b - A x (FP64).
-            TNS_HIP(hipMemset2DAsync(x, sizeof(double) * (size_t)ldx, 0, sizeof(double) * (size_t)n, t, ctx->stream));
-            dcopy2d(ctx, n, t, b, ldb, r.get(), n);
+            // x = M^-1 b by the low-precision solve; then, while the
+            // backward error is above tol, FGMRES restart cycles on
+            // A e = b - A x (residuals and updates in FP64), each of which
+            // must at least halve the backward error.
             DevBuf<float> d(ctx, (size_t)n4 * t);
             DevBuf<double> sc(ctx, t);
-            const dim3 cols((q + 255) / 256, t);
-            double prev = INFINITY;
-            for (iters = 1; iters <= max_iter; iters++) {
-                hipLaunchKernelGGL(col_absmax_kernel, dim3(t), dim3(256), 0, ctx->stream, n, r.get(), n, sc.get());
-                hipLaunchKernelGGL(rbt_rhs_kernel, cols, dim3(256), 0, ctx->stream, n, n4, r.get(), n, prs, sc.get(), ub,
-                                   d.get());
-                // The correction's triangular solves take FP32 products in
-                // both modes.
-                ctx->lp_mode = LP_FP32;
-                sgetrs_local(ctx, n4, t, s.get(), n4, perm.get(), inv.get(), d.get(), n4);
-                hipLaunchKernelGGL(rbt_accumulate_kernel, cols, dim3(256), 0, ctx->stream, n, n4, d.get(), vb, pcs,
-                                   sc.get(), x, ldx);
-                TNS_HIP(hipGetLastError());
-                berr = backward_error(ctx, n, t, a, lda, b, ldb, x, ldx, r.get(), anorm, bnorm, d_tmp.get());
-                if (iters == 1) out.first_error = berr;
-                if (berr <= tol) {
-                    converged = true;
-                    break;
+            const LowSolve M{ctx, n, n4, t, s.get(), perm.get(), inv.get(), prs, pcs, ub, vb, d.get(), sc.get()};
+            TNS_HIP(hipMemset2DAsync(x, sizeof(double) * (size_t)ldx, 0, sizeof(double) * (size_t)n, t, ctx->stream));
+            M.apply(b, ldb, x, ldx, true);
+            iters = 1;
+            std::vector<double> xnorm;
+            berr = backward_error(ctx, n, t, a, lda, b, ldb, x, ldx, r.get(), anorm, bnorm, d_tmp.get(), &xnorm);
+            out.first_error = berr;
+            converged = berr <= tol;
+            if (!converged && iters < max_iter && berr == berr) {
+                // Restart length: at most 30 steps, and the Krylov blocks
+                // (2m + 1 of n x t) within 2 GiB.
+                const double blocks = 2147483648.0 / (8.0 * n * t);
+                const int mcap = std::max(2, std::min(30, (int)((blocks - 1.0) / 2.0)));
+                const int mall = std::min(mcap, max_iter - iters);
+                DevBuf<double> V(ctx, (size_t)(mall + 1) * n * t), Z(ctx, (size_t)mall * n * t);
+                DevBuf<double> h(ctx, (size_t)(2 * (mall + 1) + 2 + mall) * t);
+                std::vector<double> target(t);
+                // The FP64 LU and solve take about 2.5 times as long as the
+                // low-precision factorization (measured from n = 1024 to
+                // 8192); refinement steps that would cost more are not taken.
+                const double fallback_ms = 2.5 * out.factor_ms;
+                double prev = berr;
+                while (iters < max_iter) {
+                    const int m = std::min(mall, max_iter - iters);
+                    for (int j = 0; j < t; j++) target[j] = 0.5 * tol * (anorm * xnorm[j] + bnorm[j]);
+                    bool hopeless = false;
+                    iters += fgmres_cycle(M, a, lda, r.get(), x, ldx, m, target, V.get(), Z.get(), h.get(),
+                                          fallback_ms, &hopeless);
+                    if (hopeless) break;
+                    berr = backward_error(ctx, n, t, a, lda, b, ldb, x, ldx, r.get(), anorm, bnorm, d_tmp.get(), &xnorm);
+                    if (berr <= tol) {
+                        converged = true;
+                        break;
+                    }
+                    if (!(berr <= 0.5 * prev)) break;  // stalled, or NaN
+                    prev = berr;
                 }
-                // Stalled (or NaN): a step must cut the error by a tenth;
-                // slower contraction still beats the FP64 fallback's cost.
-                if (!(berr <= 0.9 * prev)) break;
-                prev = berr;
             }
-            if (iters > max_iter) iters = max_iter;
         }
-        ctx->lp_mode = saved;
     }
     out.iterations = iters;
     out.refine_ms = std::chrono::duration<double, std::milli>(clock::now() - t0).count() - out.factor_ms;

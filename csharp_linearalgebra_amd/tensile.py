@@ -279,9 +279,11 @@ class Workspace:
 
     def solve_mixed(self, a, b, precision: int = 32, tol: float = 1e-15, max_iterations: int = 30, out=None):
         """X with A X = B by a low-precision LU (FP32, or BF16 products with
-        equilibration when precision=16) and FP64 iterative refinement to a
-        backward error <= tol, falling back to the FP64 LU when refinement
-        stalls. Returns (X, MixedInfo). Real matrices only."""
+        equilibration when precision=16) and FP64 iterative refinement
+        (FGMRES preconditioned by the low-precision solve, at most
+        max_iterations solves) to a backward error <= tol, falling back to the
+        FP64 LU when refinement stalls or would cost more than it. Returns
+        (X, MixedInfo). Real matrices only."""
         if _is_complex(a, b):
             raise ArgumentError("The mixed-precision solve takes real matrices.")
         A = _Operand(a, False)

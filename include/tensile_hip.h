@@ -231,15 +231,18 @@ TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* us
  * the FP32 matrix cores; 16: its trailing updates on the BF16 matrix cores
  * (FP32 accumulation, operands split into two bfloat16 parts, three
  * products each).
- * Refinement (residual and update in FP64) runs until the backward error
+ * Refinement (residuals and updates in FP64; after the first solve, FGMRES
+ * restart cycles of at most 30 steps, right-preconditioned by the
+ * low-precision solve) runs until the backward error
  * max_j ||b_j - A x_j||inf / (||A||inf ||x_j||inf + ||b_j||inf) <= tol, at
- * most max_iter steps; a step that fails to cut it by a tenth, or an exactly
- * zero low-precision pivot, switches to the FP64 LU with partial pivoting
- * (fell_back = 1). A and B are not modified. TNS_ESTATE if the FP64 LU also
- * meets a zero pivot. */
+ * most max_iter low-precision solves in all. A cycle that fails to halve the
+ * error, residuals that shrink too slowly for the remaining steps to cost
+ * less than about 2.5 factorizations, or an exactly zero low-precision pivot
+ * switch to the FP64 LU with partial pivoting (fell_back = 1). A and B are
+ * not modified. TNS_ESTATE if the FP64 LU also meets a zero pivot. */
 typedef struct {
     int precision;          /* 32 or 16, as requested */
-    int iterations;         /* refinement steps taken */
+    int iterations;         /* low-precision solves taken */
     int fell_back;          /* 1: the result comes from the FP64 LU */
     double backward_error;  /* of the returned X */
     double factor_ms;       /* low-precision factorisation, host wall time */

@@ -39,11 +39,25 @@ def test_fp32_refinement_reaches_fp64_backward_error(ws, n, kappa):
 
 
 def test_fp32_falls_back_when_refinement_stalls(ws):
-    # kappa * u_fp32 >> 1: the corrections cannot contract.
+    # kappa * u_fp32 >> 1: the residuals shrink by about 1% a step, so the
+    # first FGMRES cycle gives up at its fourth step (the extrapolated steps
+    # would cost far more than the FP64 solve).
     a, b = W.conditioned(256, 1e10, 4, seed=5)
     x, info = ws.solve_mixed(a, b, precision=32)
     assert info.fell_back
+    assert info.iterations == 5, info
     assert info.backward_error <= 1e-15
+    assert berr(a, x, b) <= 2e-15
+
+
+def test_falls_back_when_the_solve_budget_runs_out(ws):
+    a, b = W.conditioned(384, 1e6, 4, seed=3)
+    x, info = ws.solve_mixed(a, b, precision=32, max_iterations=2)
+    assert info.fell_back and info.iterations == 2, info
+    assert berr(a, x, b) <= 2e-15
+    # One solve only: no refinement at all.
+    x, info = ws.solve_mixed(a, b, precision=32, max_iterations=1)
+    assert info.fell_back and info.iterations == 1, info
     assert berr(a, x, b) <= 2e-15
 
 
