@@ -118,13 +118,11 @@ constexpr int PN_KL = 4;      // k-lanes per workgroup (one wavefront each)
 // Y = A X, split over k: part[ks][c][i] = sum over the ks-th column chunk.
 // A wavefront covers 64 consecutive rows of one column of A at a time
 // (coalesced); X's row j is uniform across the wavefront.
-template <typename T, int TMAX>
-__global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __restrict__ a, int lda,
-                                                      const T* __restrict__ x, int ldx, int kchunk,
-                                                      T* __restrict__ part, const int* __restrict__ skip)
+// FULL: t == TMAX, so no column test guards a load.
+template <typename T, int TMAX, bool FULL>
+__device__ __forceinline__ void panel_n_body(int n, int t, const T* __restrict__ a, int lda, const T* __restrict__ x,
+                                             int ldx, int kchunk, T* __restrict__ part, T (&red)[PN_KL][TMAX][PN_ROWS])
 {
-    if (skip != nullptr && *skip) return;  // a finished Taylor stage: no work
-    __shared__ T red[PN_KL][TMAX][PN_ROWS];
     const int lane = threadIdx.x & 63, kl = threadIdx.x >> 6;
     const int i = blockIdx.x * PN_ROWS + lane;
     const int ks = blockIdx.y;
@@ -133,11 +131,29 @@ __global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __r
 #pragma unroll
     for (int c = 0; c < TMAX; c++) acc[c] = s_zero(T{});
     if (i < n) {
-        for (int j = j0 + kl; j < j1; j += PN_KL) {
+        // Eight columns' loads in flight at once, then their products in
+        // the same order as one at a time.
+        int j = j0 + kl;
+        for (; j + 7 * PN_KL < j1; j += 8 * PN_KL) {
+            T av[8], xv[8][TMAX];
+#pragma unroll
+            for (int u = 0; u < 8; u++) av[u] = a[i + (size_t)(j + u * PN_KL) * lda];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int c = 0; c < TMAX; c++) xv[u][c] = FULL ? x[j + u * PN_KL + (size_t)c * ldx] : s_zero(T{});
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int c = 0; c < TMAX; c++)
+                    if (FULL || c < t)
+                        acc[c] = s_add(acc[c], s_mul(av[u], FULL ? xv[u][c] : x[j + u * PN_KL + (size_t)c * ldx]));
+        }
+        for (; j < j1; j += PN_KL) {
             const T av = a[i + (size_t)j * lda];
 #pragma unroll
             for (int c = 0; c < TMAX; c++)
-                if (c < t) acc[c] = s_add(acc[c], s_mul(av, x[j + (size_t)c * ldx]));
+                if (FULL || c < t) acc[c] = s_add(acc[c], s_mul(av, x[j + (size_t)c * ldx]));
         }
     }
 #pragma unroll
@@ -146,7 +162,7 @@ __global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __r
     if (kl == 0 && i < n) {
 #pragma unroll
         for (int c = 0; c < TMAX; c++) {
-            if (c < t) {
+            if (FULL || c < t) {
                 T s = red[0][c][lane];
 #pragma unroll
                 for (int q = 1; q < PN_KL; q++) s = s_add(s, red[q][c][lane]);
@@ -154,6 +170,21 @@ __global__ void __launch_bounds__(256) panel_n_kernel(int n, int t, const T* __r
             }
         }
     }
+}
+
+// About one wavefront per SIMD at the orders these serve: registers for
+// loads in flight matter more than occupancy.
+template <typename T, int TMAX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) panel_n_kernel(int n, int t, const T* __restrict__ a, int lda,
+                                                      const T* __restrict__ x, int ldx, int kchunk,
+                                                      T* __restrict__ part, const int* __restrict__ skip)
+{
+    if (skip != nullptr && *skip) return;  // a finished Taylor stage: no work
+    __shared__ T red[PN_KL][TMAX][PN_ROWS];
+    if (t == TMAX)
+        panel_n_body<T, TMAX, true>(n, t, a, lda, x, ldx, kchunk, part, red);
+    else
+        panel_n_body<T, TMAX, false>(n, t, a, lda, x, ldx, kchunk, part, red);
 }
 
 template <typename T>
@@ -169,9 +200,10 @@ __global__ void __launch_bounds__(256) panel_finish_kernel(int n, int t, int ksp
 }
 
 // Y = op(A)^T-form: y_j = sum_i conj?(a_ij) x_i, one wavefront per column j.
-template <typename T, int TMAX, bool CONJ>
-__global__ void __launch_bounds__(256) panel_t_kernel(int n, int t, const T* __restrict__ a, int lda,
-                                                      const T* __restrict__ x, int ldx, T* __restrict__ y, int ldy)
+// FULL: t == TMAX, so no column test guards a load.
+template <typename T, int TMAX, bool CONJ, bool FULL>
+__device__ __forceinline__ void panel_t_body(int n, int t, const T* __restrict__ a, int lda, const T* __restrict__ x,
+                                             int ldx, T* __restrict__ y, int ldy)
 {
     const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -180,20 +212,50 @@ __global__ void __launch_bounds__(256) panel_t_kernel(int n, int t, const T* __r
     T acc[TMAX];
 #pragma unroll
     for (int c = 0; c < TMAX; c++) acc[c] = s_zero(T{});
-    for (int i = lane; i < n; i += 64) {
+    // Eight rows' loads in flight at once, then their products in the same
+    // order as one at a time.
+    int i = lane;
+    for (; i + 7 * 64 < n; i += 8 * 64) {
+        T av[8], xv[8][TMAX];
+#pragma unroll
+        for (int u = 0; u < 8; u++) av[u] = col[i + u * 64];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int c = 0; c < TMAX; c++) xv[u][c] = FULL ? x[i + u * 64 + (size_t)c * ldx] : s_zero(T{});
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (CONJ) av[u] = s_conj(av[u]);
+#pragma unroll
+            for (int c = 0; c < TMAX; c++)
+                if (FULL || c < t)
+                    acc[c] = s_add(acc[c], s_mul(av[u], FULL ? xv[u][c] : x[i + u * 64 + (size_t)c * ldx]));
+        }
+    }
+    for (; i < n; i += 64) {
         T av = col[i];
         if (CONJ) av = s_conj(av);
 #pragma unroll
         for (int c = 0; c < TMAX; c++)
-            if (c < t) acc[c] = s_add(acc[c], s_mul(av, x[i + (size_t)c * ldx]));
+            if (FULL || c < t) acc[c] = s_add(acc[c], s_mul(av, x[i + (size_t)c * ldx]));
     }
 #pragma unroll
     for (int c = 0; c < TMAX; c++) {
-        if (c < t) {
+        if (FULL || c < t) {
             T s = wave_sum(acc[c]);
             if (lane == 0) y[j + (size_t)c * ldy] = s;
         }
     }
+}
+
+template <typename T, int TMAX, bool CONJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) panel_t_kernel(int n, int t, const T* __restrict__ a, int lda,
+                                                      const T* __restrict__ x, int ldx, T* __restrict__ y, int ldy)
+{
+    if (t == TMAX)
+        panel_t_body<T, TMAX, CONJ, true>(n, t, a, lda, x, ldx, y, ldy);
+    else
+        panel_t_body<T, TMAX, CONJ, false>(n, t, a, lda, x, ldx, y, ldy);
 }
 
 // ---------------------------------------------------------------------------
