@@ -358,29 +358,60 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v)
 // 292-303), fused with the second pass of the panel product:
 //   next = sum of the k-split partials, work = coef*next, f += work,
 //   norms[0] = ||work||_inf, norms[1] = ||f||_inf.
-// One thread per row, all nb columns, so the row sums are local.
+// A workgroup takes 32 rows, eight threads to a row, each on one column
+// of a group of eight at a time (the partials' loads eight at once); the
+// row sums of |work| and |f| are then added by the row's first thread in
+// column order, as one thread per row would add them.
 template <typename T>
 __global__ void __launch_bounds__(256) taylor_step_kernel(int n, int nb, int ksplit, const T* __restrict__ part,
                                                           double coef, T* __restrict__ work, T* __restrict__ f,
                                                           double* __restrict__ norms, const int* __restrict__ skip)
 {
     if (skip != nullptr && *skip) return;  // a finished Taylor stage: no work
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double aw[32][9], af[32][9];
+    const int r = threadIdx.x >> 3, slot = threadIdx.x & 7;
+    const int i = blockIdx.x * 32 + r;
+    const size_t qs = (size_t)nb * n;  // between one k-chunk's partials and the next
     double sw = 0.0, sf = 0.0;
-    for (int c = 0; c < nb; c++) {
-        T s = part[(size_t)c * n + i];
-        for (int q = 1; q < ksplit; q++) s = s_add(s, part[((size_t)q * nb + c) * n + i]);
-        T w = s_scale(s, coef);
-        size_t e = i + (size_t)c * n;
-        work[e] = w;
-        T fv = s_add(f[e], w);
-        f[e] = fv;
-        sw += s_abs(w);
-        sf += s_abs(fv);
+    for (int c0 = 0; c0 < nb; c0 += 8) {
+        const int c = c0 + slot;
+        double mw = 0.0, mf = 0.0;
+        if (i < n && c < nb) {
+            const T* p = part + (size_t)c * n + i;
+            T s = p[0];
+            int q = 1;
+            for (; q + 7 < ksplit; q += 8) {
+                T v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = p[(size_t)(q + u) * qs];
+#pragma unroll
+                for (int u = 0; u < 8; u++) s = s_add(s, v[u]);
+            }
+            for (; q < ksplit; q++) s = s_add(s, p[(size_t)q * qs]);
+            const T w = s_scale(s, coef);
+            const size_t e = i + (size_t)c * n;
+            work[e] = w;
+            const T fv = s_add(f[e], w);
+            f[e] = fv;
+            mw = s_abs(w);
+            mf = s_abs(fv);
+        }
+        aw[r][slot] = mw;
+        af[r][slot] = mf;
+        __syncthreads();
+        if (slot == 0) {
+            const int cn = nb - c0 < 8 ? nb - c0 : 8;
+            for (int k = 0; k < cn; k++) {
+                sw += aw[r][k];
+                sf += af[r][k];
+            }
+        }
+        __syncthreads();
     }
-    atomic_max_nonneg(norms + 0, sw);
-    atomic_max_nonneg(norms + 1, sf);
+    if (slot == 0 && i < n) {
+        atomic_max_nonneg(norms + 0, sw);
+        atomic_max_nonneg(norms + 1, sf);
+    }
 }
 
 // ||x||_inf of an n x nb panel (ld n) into out[0] (which must start at 0).
@@ -649,7 +680,7 @@ template <typename T>
 void taylor_step(Context* ctx, int n, int nb, int ks, const T* part, double coef, T* work, T* f, double* d_norms,
                  const int* skip)
 {
-    hipLaunchKernelGGL((taylor_step_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nb, ks, part,
+    hipLaunchKernelGGL((taylor_step_kernel<T>), dim3((n + 31) / 32), dim3(256), 0, ctx->stream, n, nb, ks, part,
                        coef, work, f, d_norms, skip);
 }
 template void taylor_step<double>(Context*, int, int, int, const double*, double, double*, double*, double*,
