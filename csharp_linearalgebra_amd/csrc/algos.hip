@@ -600,7 +600,10 @@ void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out)
     if (info >= 0)
         fail(TNS_ESTATE, "The Pade denominator is exactly singular at column " + std::to_string(info) +
                              ", which indicates a non-finite input rather than a property of the algorithm.");
-    getrs_t<T>(ctx, 0, n, n, den.p(), n, perm.get(), out, n);
+    // The denominator is well conditioned at the degrees and scalings the
+    // selection allows, so its n-column solve takes the block-inverse form.
+    if constexpr (sizeof(T) == 8) dgetrs_blockinv(ctx, n, n, (const double*)den.p(), n, perm.get(), (double*)out, n);
+    else zgetrs_blockinv(ctx, n, n, (const zd*)den.p(), n, perm.get(), (zd*)out, n);
 }
 
 template <typename T>

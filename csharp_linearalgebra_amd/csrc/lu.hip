@@ -1279,6 +1279,46 @@ __global__ void __launch_bounds__(2 * W) tri_inverse_kernel(int n, const T* __re
     for (int i = 0; i < W; i++) out[i] = y[i];
 }
 
+// The same two inverses per diagonal block as tri_inverse_kernel, one
+// workgroup per block (blockIdx.x), the columns of the inverses kept in
+// LDS instead of registers: a W-element complex column in registers, with
+// the unrolled substitution, spills. Same operations per element, in the
+// same order.
+template <typename T, int W>
+__global__ void __launch_bounds__(2 * W) tri_inverse_lds_kernel(int n, const T* __restrict__ lu, int ld,
+                                                                T* __restrict__ inv)
+{
+    __shared__ T As[W][W + 1];
+    __shared__ T Y[2][W][W + 1];  // Y[0]: inv(L) (unit lower), Y[1]: inv(U); Y[h][i][j] = row i, column j
+    const int blk = blockIdx.x, k0 = blk * W;
+    const int kb = W < n - k0 ? W : n - k0;
+    const T* a = lu + k0 + (size_t)k0 * ld;
+    for (int e = threadIdx.x; e < W * W; e += 2 * W) {
+        const int i = e % W, k = e / W;
+        const bool in = i < kb && k < kb;
+        As[i][k] = in ? a[i + (size_t)k * ld] : (i == k ? s_one(T{}) : s_zero(T{}));
+    }
+    const int h = threadIdx.x >= W ? 1 : 0;
+    const int j = threadIdx.x - h * W;
+    for (int i = 0; i < W; i++) Y[h][i][j] = i == j ? s_one(T{}) : s_zero(T{});
+    __syncthreads();
+    if (h == 0) {
+        for (int p = 0; p < W; p++) {
+            const T ny = s_neg(Y[0][p][j]);
+            for (int i = p + 1; i < W; i++) Y[0][i][j] = s_add(Y[0][i][j], s_mul(ny, As[i][p]));
+        }
+    } else {
+        for (int p = W - 1; p >= 0; p--) {
+            const T yp = s_div(Y[1][p][j], As[p][p]);
+            Y[1][p][j] = yp;
+            const T ny = s_neg(yp);
+            for (int i = 0; i < p; i++) Y[1][i][j] = s_add(Y[1][i][j], s_mul(ny, As[i][p]));
+        }
+    }
+    T* out = inv + (size_t)blk * 2 * W * W + (h ? W * W : 0) + (size_t)j * W;
+    for (int i = 0; i < W; i++) out[i] = Y[h][i][j];
+}
+
 // dst[i] = src[perm[i]] (gather) or dst[perm[i]] = src[i] (scatter).
 template <typename T>
 __global__ void permute_rows_kernel(int n, int ncols, const int* __restrict__ perm, const T* __restrict__ src, int lds,
@@ -1844,6 +1884,51 @@ void getrs_impl(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu,
     TNS_HIP(hipGetLastError());
 }
 
+// A X = B (op = none) against the LU factors, for many right-hand sides
+// of a well-conditioned A (the Pade denominator): the inverses of every
+// diagonal block of L and U come from one launch (the blocks are
+// independent), and each block step is then two products on the matrix
+// cores, X_k = inv(D_kk) B_k and the update of the blocks still to come,
+// where the substitution form spends a launch with one barrier per row
+// of the block. The products go between two buffers, so none is in place.
+// The rounding differs from the substitution's; the backward error of a
+// well-conditioned block's explicit inverse is of the same order.
+template <typename T>
+void getrs_blockinv_impl(Context* ctx, int n, int nrhs, const T* lu, int ldlu, const int* d_perm, T* b, int ldb)
+{
+    if (n <= 0 || nrhs <= 0) return;
+    constexpr int BS = tri_block<T>();
+    const int nblk = (n + BS - 1) / BS;
+    DevBuf<T> inv(ctx, (size_t)nblk * 2 * BS * BS);
+    hipLaunchKernelGGL((tri_inverse_lds_kernel<T, BS>), dim3(nblk), dim3(2 * BS), 0, ctx->stream, n, lu, ldlu,
+                       inv.get());
+    TNS_HIP(hipGetLastError());
+    DevBuf<T> t1(ctx, (size_t)n * nrhs), t2(ctx, (size_t)n * nrhs);
+    permute_rows<T>(ctx, n, nrhs, d_perm, b, ldb, t1, n, 0);
+    const T one = Ops<T>::one(), neg = Ops<T>::neg_one(), zero{};
+    // Forward with the unit lower triangle: t2_k = inv(L_kk) t1_k, then
+    // t1_{>k} -= L_{>k,k} t2_k.
+    for (int k = 0; k < nblk; k++) {
+        const int k0 = k * BS, kb = n - k0 < BS ? n - k0 : BS;
+        const T* il = inv.get() + (size_t)k * 2 * BS * BS;
+        Ops<T>::gemm(ctx, 0, kb, nrhs, kb, one, il, BS, t1.get() + k0, n, zero, t2.get() + k0, n);
+        if (k0 + kb < n)
+            Ops<T>::gemm(ctx, 0, n - k0 - kb, nrhs, kb, neg, lu + (k0 + kb) + (size_t)k0 * ldlu, ldlu, t2.get() + k0, n,
+                         one, t1.get() + k0 + kb, n);
+    }
+    // Backward with the upper triangle: t1_k = inv(U_kk) t2_k, then
+    // t2_{<k} -= U_{<k,k} t1_k.
+    for (int k = nblk - 1; k >= 0; k--) {
+        const int k0 = k * BS, kb = n - k0 < BS ? n - k0 : BS;
+        const T* iu = inv.get() + (size_t)k * 2 * BS * BS + BS * BS;
+        Ops<T>::gemm(ctx, 0, kb, nrhs, kb, one, iu, BS, t2.get() + k0, n, zero, t1.get() + k0, n);
+        if (k0 > 0)
+            Ops<T>::gemm(ctx, 0, k0, nrhs, kb, neg, lu + (size_t)k0 * ldlu, ldlu, t1.get() + k0, n, one, t2.get(), n);
+    }
+    copy2d<T>(ctx, n, nrhs, t1, n, b, ldb);
+    TNS_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
@@ -1887,6 +1972,16 @@ void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu
 void zgetrs(Context* ctx, int trans, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm, cplx* b, int ldb)
 {
     getrs_impl<zd>(ctx, trans, n, nrhs, lu, ldlu, d_perm, b, ldb);
+}
+
+void dgetrs_blockinv(Context* ctx, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb)
+{
+    getrs_blockinv_impl<double>(ctx, n, nrhs, lu, ldlu, d_perm, b, ldb);
+}
+
+void zgetrs_blockinv(Context* ctx, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm, cplx* b, int ldb)
+{
+    getrs_blockinv_impl<zd>(ctx, n, nrhs, lu, ldlu, d_perm, b, ldb);
 }
 
 void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb)

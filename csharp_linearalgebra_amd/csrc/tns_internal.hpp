@@ -162,6 +162,11 @@ void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu
             double* b, int ldb);
 void zgetrs(Context* ctx, int trans, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm,
             cplx* b, int ldb);
+// A X = B against the factors with the diagonal blocks applied through
+// their explicit inverses (for many right-hand sides of a well-conditioned
+// A; rounding differs from dgetrs/zgetrs).
+void dgetrs_blockinv(Context* ctx, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb);
+void zgetrs_blockinv(Context* ctx, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm, cplx* b, int ldb);
 // The KernelEntry triangular solves: kind as tns_tri_kind.
 void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb);
 // min/max |u_ii| into d_out[0], d_out[1] (and a NaN flag).
