@@ -1684,13 +1684,18 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
 // triangles, then the rows below it L21 = A21 U11^-1 and the block row
 // U12 = L11^-1 A12 as products with those inverses (in place: each product
 // is one tile wide in the dimension it overwrites, so every tile reads its
-// operand before it writes), and the trailing update, all on sgemm_lp. The
+// operand before it writes), and the trailing update, all on sgemm_lp.
+// The block row and the trailing update reach only the columns of the
+// current outer panel of LOCAL_OB columns; the columns beyond it take
+// their block row by block substitution and one product of depth LOCAL_OB
+// at the end of the outer step. The
 // pivot search sees 64 rows instead of the whole column, so no step waits
 // on a tall panel; it is for matrices whose randomized form needs no wider
 // search (the mixed-precision solve), where the refinement absorbs the
 // growth that remains. P A = L U with perm as in getrf_impl; inv keeps
 // every diagonal block's inverses for the solves; a zero pivot sets info.
 constexpr int LOCAL_NB = 64;
+constexpr int LOCAL_OB = 256;  // outer step of getrf_local_impl, a multiple of LOCAL_NB
 
 template <typename T>
 void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_perm, T* inv, int* d_info)
@@ -1705,21 +1710,51 @@ void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_per
     DevBuf<int> rc(ctx, (size_t)n);
     DevBuf<int> list(ctx, 4 * (size_t)NB + 4);
     hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, d_perm, rc.get());
-    const T one = Ops<T>::one(), zero{};
-    for (int jb = 0; jb < n; jb += NB) {
-        const int w = NB < n - jb ? NB : n - jb;
-        T* pa = a + jb + (size_t)jb * lda;
-        panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb, n - jb - w, 32);
-        hipLaunchKernelGGL((tri_inverse_kernel<T, NB>), dim3(1), dim3(2 * NB), 0, ctx->stream, n, a, lda, jb / NB, inv);
-        const int rest = n - jb - w;
-        if (rest <= 0) continue;
-        const T* inv_l = inv + (size_t)(jb / NB) * 2 * NB * NB;
-        const T* inv_u = inv_l + NB * NB;
-        // The two products with the inverses stay FP32 in both modes.
-        sgemm_lp(ctx, LP_FP32, rest, w, w, one, pa + w, lda, inv_u, NB, zero, pa + w, lda);
-        sgemm_lp(ctx, LP_FP32, w, rest, w, one, inv_l, NB, pa + (size_t)w * lda, lda, zero, pa + (size_t)w * lda, lda);
-        Ops<T>::gemm(ctx, 0, rest, rest, w, Ops<T>::neg_one(), pa + w, lda, pa + (size_t)w * lda, lda, one,
-                     pa + w + (size_t)w * lda, lda);
+    const T one = Ops<T>::one(), zero{}, neg = Ops<T>::neg_one();
+    // Outer steps of OB columns: inside one, each NB block step updates only
+    // the outer panel's columns; the columns beyond it get one product of
+    // depth OB per outer step (a quarter of the trailing matrix's reads and
+    // writes of depth-NB updates). Interchanges still move whole rows at
+    // once: they permute rows of the pending update's L and A alike.
+    constexpr int OB = LOCAL_OB;
+    for (int J = 0; J < n; J += OB) {
+        const int ow = OB < n - J ? OB : n - J;
+        for (int jb = J; jb < J + ow; jb += NB) {
+            const int w = NB < n - jb ? NB : n - jb;
+            T* pa = a + jb + (size_t)jb * lda;
+            panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb, n - jb - w,
+                     32);
+            hipLaunchKernelGGL((tri_inverse_kernel<T, NB>), dim3(1), dim3(2 * NB), 0, ctx->stream, n, a, lda, jb / NB,
+                               inv);
+            const int rest = n - jb - w;
+            if (rest <= 0) continue;
+            const T* inv_l = inv + (size_t)(jb / NB) * 2 * NB * NB;
+            const T* inv_u = inv_l + NB * NB;
+            // The products with the inverses stay FP32 in both modes.
+            sgemm_lp(ctx, LP_FP32, rest, w, w, one, pa + w, lda, inv_u, NB, zero, pa + w, lda);
+            const int inner = J + ow - (jb + w);  // the outer panel's columns still to come
+            if (inner <= 0) continue;
+            sgemm_lp(ctx, LP_FP32, w, inner, w, one, inv_l, NB, pa + (size_t)w * lda, lda, zero, pa + (size_t)w * lda,
+                     lda);
+            Ops<T>::gemm(ctx, 0, rest, inner, w, neg, pa + w, lda, pa + (size_t)w * lda, lda, one,
+                         pa + w + (size_t)w * lda, lda);
+        }
+        const int cols = n - (J + ow);  // beyond the outer panel
+        if (cols <= 0) continue;
+        // U12 of the outer panel by block forward substitution with the
+        // diagonal blocks' inverses: U_j = inv(L_jj) (A_j - L_j,<j U_<j).
+        T* a12 = a + J + (size_t)(J + ow) * lda;
+        for (int r0 = J; r0 < J + ow; r0 += NB) {
+            const int w = NB < J + ow - r0 ? NB : J + ow - r0;
+            T* uj = a + r0 + (size_t)(J + ow) * lda;
+            if (r0 > J)
+                Ops<T>::gemm(ctx, 0, w, cols, r0 - J, neg, a + r0 + (size_t)J * lda, lda, a12, lda, one, uj, lda);
+            const T* inv_l = inv + (size_t)(r0 / NB) * 2 * NB * NB;
+            sgemm_lp(ctx, LP_FP32, w, cols, w, one, inv_l, NB, uj, lda, zero, uj, lda);
+        }
+        const int below = n - (J + ow);
+        Ops<T>::gemm(ctx, 0, below, cols, ow, neg, a + (J + ow) + (size_t)J * lda, lda, a12, lda, one,
+                     a + (J + ow) + (size_t)(J + ow) * lda, lda);
     }
     hipLaunchKernelGGL(finish_info_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info);
     TNS_HIP(hipGetLastError());
