@@ -1697,6 +1697,79 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
 constexpr int LOCAL_NB = 64;
 constexpr int LOCAL_OB = 256;  // outer step of getrf_local_impl, a multiple of LOCAL_NB
 
+// The LOCAL_NB x LOCAL_NB diagonal block of getrf_local_impl factored by
+// one wavefront: lane i holds row i of the block in registers throughout,
+// and the column loop is unrolled, so every register index is a constant.
+// As in getf2_reg_body, interchanges move no data (each lane carries its
+// row's current position; the pivot takes position j, the row at j takes
+// the pivot's), the pivot is the largest |x| with ties to the lower
+// position, a zero pivot leaves the rows in place, and the multipliers and
+// updates are formed the same way in the same order; the pivot row reaches
+// the other lanes by readlane instead of LDS, and no barrier is needed.
+// The rows are stored to their final positions at the end; piv and perm
+// (from row jb) and the move list (block-relative) are as getf2_reg writes
+// them, and info is min-reduced with the first zero-pivot column.
+__global__ void __launch_bounds__(64) local_block_lu_kernel(float* __restrict__ a, int lda, int jb,
+                                                            int* __restrict__ piv, int* __restrict__ perm,
+                                                            int* __restrict__ list, int* __restrict__ info)
+{
+    constexpr int W = LOCAL_NB;
+    const int lane = threadIdx.x;
+    float* blk = a + jb + (size_t)jb * lda;
+    float v[W];
+#pragma unroll
+    for (int c = 0; c < W; c++) v[c] = blk[lane + (size_t)c * lda];
+    const int p0 = perm[jb + lane];
+    int pos = lane;
+    bool done = false;
+    int first_zero = -1;
+    int my_piv = 0;  // lane j keeps piv[jb + j]
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        double bv = done ? -2.0 : pivot_key(v[j], pos, j);
+        int bi = done ? INT_MAX : pos;
+        wave_best_dpp(bv, bi);
+        const int pl = __ffsll((unsigned long long)__ballot(!done && pos == bi)) - 1;  // the pivot's lane
+        const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[j]), pl));
+        const bool zero_pivot = pv == 0.0f;
+        if (lane == j) my_piv = jb + bi;
+        if (zero_pivot && first_zero < 0) first_zero = jb + j;
+        const int pe = zero_pivot ? j : bi;
+        const bool is_p = !done && pos == pe;
+        const bool is_j = !done && pos == j && !is_p;
+        done = done || is_p;
+        pos = is_p ? j : (is_j ? pe : pos);
+        // Multipliers: x * (1/pivot) when |pivot| >= the safe minimum,
+        // x / pivot below it, x itself for a zero pivot.
+        float l = v[j];
+        if (!zero_pivot) {
+            if (fabsf(pv) >= (float)safe_min<float>()) l = l * (1.0f / pv);
+            else l = l / pv;
+        }
+        if (!done) v[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < W; c++) {
+            const float u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[c]), pl));
+            if (u != 0.0f && !done) v[c] = v[c] + (-u) * l;
+        }
+    }
+    __syncthreads();  // every lane's perm entry read before any is written
+#pragma unroll
+    for (int c = 0; c < W; c++) blk[pos + (size_t)c * lda] = v[c];
+    perm[jb + pos] = p0;
+    piv[jb + lane] = my_piv;
+    const unsigned long long moved = __ballot(pos != lane);
+    if (pos != lane) {
+        const int at = __popcll(moved & ((1ull << lane) - 1ull));
+        list[1 + 2 * at] = pos;
+        list[2 + 2 * at] = lane;
+    }
+    if (lane == 0) {
+        list[0] = __popcll(moved);
+        if (first_zero >= 0) atomicMin(info, first_zero);
+    }
+}
+
 template <typename T>
 void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_perm, T* inv, int* d_info)
 {
@@ -1725,8 +1798,16 @@ void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_per
         for (int jb = J; jb < J + ow; jb += NB) {
             const int w = NB < n - jb ? NB : n - jb;
             T* pa = a + jb + (size_t)jb * lda;
-            panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb, n - jb - w,
-                     32);
+            if (w == NB) {
+                // One wavefront for the whole diagonal block, then its
+                // interchanges across the rest of the rows.
+                hipLaunchKernelGGL(local_block_lu_kernel, dim3(1), dim3(64), 0, ctx->stream, a, lda, jb, d_piv,
+                                   d_perm, list.get(), d_info);
+                apply_moves<T>(ctx, a + jb, lda, jb, jb + w, n, list, 2 * w);
+            } else {
+                panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb,
+                         n - jb - w, 32);
+            }
             hipLaunchKernelGGL((tri_inverse_kernel<T, NB>), dim3(1), dim3(2 * NB), 0, ctx->stream, n, a, lda, jb / NB,
                                inv);
             const int rest = n - jb - w;

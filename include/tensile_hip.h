@@ -237,7 +237,7 @@ TNS_API int tns_zexpmv_op(tns_context* ctx, int n, tns_zapply_fn apply, void* us
  * max_j ||b_j - A x_j||inf / (||A||inf ||x_j||inf + ||b_j||inf) <= tol, at
  * most max_iter low-precision solves in all. A cycle that fails to halve the
  * error, residuals that shrink too slowly for the remaining steps to cost
- * less than about 2.5 factorizations, or an exactly zero low-precision pivot
+ * less than about 3.5 factorizations, or an exactly zero low-precision pivot
  * switch to the FP64 LU with partial pivoting (fell_back = 1). A and B are
  * not modified. TNS_ESTATE if the FP64 LU also meets a zero pivot. */
 typedef struct {
