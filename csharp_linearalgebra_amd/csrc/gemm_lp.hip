@@ -72,7 +72,9 @@ __device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned
     lo = to_bf16(x - __uint_as_float((unsigned)hi << 16));
 }
 
-template <int MODE>
+// VEC: A, B and their leading dimensions 16-byte aligned, so a slab is
+// loaded four floats at a time (A down a column, B along k).
+template <int MODE, bool VEC>
 __global__ void __launch_bounds__(LNT, 2)
 lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int lda, const float* __restrict__ B,
               int ldb, float beta, float* __restrict__ C, int ldc, int tiles_m, int tiles_n, int kc, long long cs)
@@ -106,6 +108,51 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
     float ra[LA], rb[LA];
 
     auto gload = [&](int k0) {
+        if constexpr (VEC) {
+            // Thread t: A rows 4(t%16).. at depth t/16 (+16), B depths
+            // 4(t%8).. of column t/8 (+32); ra/rb hold them in that order.
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int e4 = tid + h * LNT;
+                {
+                    const int i4 = (e4 % (LBM / 4)) * 4, kk = e4 / (LBM / 4);
+                    const int gi = m0 + i4, gk = k0 + kk;
+                    if (gi + 3 < m && gk < k) {
+                        const float4 v = *reinterpret_cast<const float4*>(A + gi + (size_t)gk * lda);
+                        ra[4 * h] = v.x;
+                        ra[4 * h + 1] = v.y;
+                        ra[4 * h + 2] = v.z;
+                        ra[4 * h + 3] = v.w;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int ci = gi + q < m ? gi + q : m - 1, ck = gk < k ? gk : k - 1;
+                            const float va = A[ci + (size_t)ck * lda];
+                            ra[4 * h + q] = (gi + q < m && gk < k) ? va : 0.f;
+                        }
+                    }
+                }
+                {
+                    const int kb4 = (e4 % (LBK / 4)) * 4, j = e4 / (LBK / 4);
+                    const int gkb = k0 + kb4, gj = n0 + j;
+                    if (gkb + 3 < k && gj < n) {
+                        const float4 v = *reinterpret_cast<const float4*>(B + gkb + (size_t)gj * ldb);
+                        rb[4 * h] = v.x;
+                        rb[4 * h + 1] = v.y;
+                        rb[4 * h + 2] = v.z;
+                        rb[4 * h + 3] = v.w;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int ckb = gkb + q < k ? gkb + q : k - 1, cj = gj < n ? gj : n - 1;
+                            const float vb = B[ckb + (size_t)cj * ldb];
+                            rb[4 * h + q] = (gkb + q < k && gj < n) ? vb : 0.f;
+                        }
+                    }
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < LA; r++) {
             const int e = tid + r * LNT;
@@ -126,9 +173,20 @@ lpgemm_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, int
     auto sstore = [&]() {
 #pragma unroll
         for (int r = 0; r < LA; r++) {
-            const int e = tid + r * LNT;
-            const int i = e % LBM, kk = e / LBM;
-            const int kb = e % LBK, j = e / LBK;
+            int i, kk, kb, j;
+            if constexpr (VEC) {
+                const int e4 = tid + (r / 4) * LNT, q = r % 4;
+                i = (e4 % (LBM / 4)) * 4 + q;
+                kk = e4 / (LBM / 4);
+                kb = (e4 % (LBK / 4)) * 4 + q;
+                j = e4 / (LBK / 4);
+            } else {
+                const int e = tid + r * LNT;
+                i = e % LBM;
+                kk = e / LBM;
+                kb = e % LBK;
+                j = e / LBK;
+            }
             if constexpr (MODE == LP_FP32) {
                 As[i * P::KP + kk] = __float_as_uint(ra[r]);
                 Bs[j * P::KP + kb] = __float_as_uint(rb[r]);
@@ -307,9 +365,14 @@ void sgemm_lp(Context* ctx, int mode, int m, int n, int k, float alpha, const fl
     }
     const int tm = (m + LBM - 1) / LBM, tn = (n + LBN - 1) / LBN;
     const int tiles = tm * tn;
-    auto kernel = mode == LP_BF16 ? lpgemm_kernel<LP_BF16>
-                  : mode == LP_BF16X3 ? lpgemm_kernel<LP_BF16X3>
-                                      : lpgemm_kernel<LP_FP32>;
+    const bool vec = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0 && lda % 4 == 0 &&
+                     ldb % 4 == 0;
+    auto kernel = vec ? (mode == LP_BF16 ? lpgemm_kernel<LP_BF16, true>
+                         : mode == LP_BF16X3 ? lpgemm_kernel<LP_BF16X3, true>
+                                             : lpgemm_kernel<LP_FP32, true>)
+                      : (mode == LP_BF16 ? lpgemm_kernel<LP_BF16, false>
+                         : mode == LP_BF16X3 ? lpgemm_kernel<LP_BF16X3, false>
+                                             : lpgemm_kernel<LP_FP32, false>);
     if (tiles <= 256 && k >= 256) {
         // Few tiles over a long k (the thin products of the triangular
         // solves): split k into parts written to scratch, then summed.
