@@ -1951,7 +1951,7 @@ template <typename T>
 void getrs_local_impl(Context* ctx, int N, int nrhs, const T* lu, int ldlu, const int* d_perm, const T* inv, T* b,
                       int ldb)
 {
-    constexpr int W = LOCAL_NB, TC = 2, SUPER = 512;
+    constexpr int W = LOCAL_NB, TC = 1, SUPER = 512;
     if (N <= 0 || nrhs <= 0) return;
     if (N % W) fail(TNS_EINVAL, "the local LU's order is a multiple of its block");
     DevBuf<T> bx(ctx, (size_t)N * nrhs);
