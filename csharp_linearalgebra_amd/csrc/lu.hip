@@ -1704,8 +1704,8 @@ constexpr int LOCAL_OB = 256;  // outer step of getrf_local_impl, a multiple of 
 // row's current position; the pivot takes position j, the row at j takes
 // the pivot's), the pivot is the largest |x| with ties to the lower
 // position, a zero pivot leaves the rows in place, and the multipliers and
-// updates are formed the same way in the same order; the pivot row reaches
-// the other lanes by readlane instead of LDS, and no barrier is needed.
+// updates are formed the same way; the pivot row reaches the other lanes
+// by readlane instead of LDS, and no barrier is needed.
 // The rows are stored to their final positions at the end; piv and perm
 // (from row jb) and the move list (block-relative) are as getf2_reg writes
 // them, and info is min-reduced with the first zero-pivot column.
@@ -1747,10 +1747,16 @@ __global__ void __launch_bounds__(64) local_block_lu_kernel(float* __restrict__ 
             else l = l / pv;
         }
         if (!done) v[j] = l;
+        // Rows already pivoted (this step's pivot row included) take a zero
+        // multiplier, which leaves them unchanged, so the update needs no
+        // per-element select; unlike the register panel kernel, zero
+        // entries of the pivot row are not skipped (x + (-0) l = x for the
+        // finite l of a nonzero pivot).
+        const float lm = done ? 0.0f : l;
 #pragma unroll
         for (int c = j + 1; c < W; c++) {
             const float u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[c]), pl));
-            if (u != 0.0f && !done) v[c] = v[c] + (-u) * l;
+            v[c] = v[c] + (-u) * lm;
         }
     }
     __syncthreads();  // every lane's perm entry read before any is written
