@@ -4,7 +4,7 @@ golden values (NormEstimateTests.cs, MatrixExponentialTests.cs,
 MatrixExponentialActionTests.cs, ComplexExponentialTests.cs).
 
 Tolerances: Expm / Expmv relative 1-norm error within 10x the oracle's own
-error against the extended-precision reference (north-star bound), with a
+error against the extended-precision reference, with a
 floor of 64 eps for cases where the oracle's error is zero to rounding."""
 import os
 

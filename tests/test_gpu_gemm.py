@@ -1,8 +1,8 @@
 """FP64-MFMA GEMM and 4M complex GEMM against the oracle, on the shapes and
 scalar cases of the reference's GEMM contracts (GemmTests.cs:31-129,
 ComplexMultiplyTests.cs:40-315). Bound: elementwise
-|C - C_ref| <= (k+2) eps (|alpha| |A||B| + |beta||C|), the north-star GEMM
-tolerance with room for the oracle's own rounding."""
+|C - C_ref| <= (k+2) eps (|alpha| |A||B| + |beta||C|): the k eps (|A||B|)
+GEMM bound with room for the oracle's own rounding."""
 import numpy as np
 import pytest
 

@@ -1,6 +1,5 @@
 """Gradients through F = exp(tA) B (the adjoint of Expmv): checked against
-central finite differences of the device forward (h = 1e-6, the north-star
-check) and against the exact derivative from the 2n x 2n block exponential
+central finite differences of the device forward (h = 1e-6) and against the exact derivative from the 2n x 2n block exponential
 (Frechet derivative: exp([[X, E], [0, X]]) has L(X, E) in its corner),
 computed by the oracle's Expm (MatrixExponential.cs:181-270 restated).
 Loss L = ||C F - Y||_F^2, G = dL/dF = 2 C^H (C F - Y)."""

@@ -1,7 +1,7 @@
 """Mixed-precision solve: FP32 (and BF16-product) LU with FP64 iterative
 refinement, falling back to the FP64 LU. The reference has no such path;
-the result is pinned by its backward error (the north-star bound 1e-15,
-the ComplexLuTests.cs:418-439 form) and by the FP64 solve of the oracle
+the result is pinned by its backward error (at most 1e-15, in the
+ComplexLuTests.cs:418-439 form) and by the FP64 solve of the oracle
 (Lu.cs:189-378) it must agree with to the conditioning."""
 import numpy as np
 import pytest
