@@ -199,3 +199,27 @@ def C_int():
     import ctypes
 
     return ctypes.c_int()
+
+
+@pytest.mark.parametrize("precision", [32, 16])
+def test_order_8192_outer_steps(ws, precision):
+    # n = 8192 takes the block-local LU's 256-column outer steps; the
+    # matrix is generated and the residual checked on the device.
+    import torch
+
+    from csharp_linearalgebra_amd import DeviceMatrix
+
+    n = 8192
+    a, b = W.conditioned_device(n, 1e2, 4, seed=2)
+    at, bt = a.T.contiguous(), b.T.contiguous()  # column-major storage
+    dA, dB = DeviceMatrix.from_torch(at), DeviceMatrix.from_torch(bt)
+    dX = DeviceMatrix.from_torch(torch.empty_like(bt))
+    torch.cuda.synchronize()  # the library's stream does not wait on torch's
+    _, info = ws.solve_mixed(dA, dB, precision=precision, out=dX)
+    ws.synchronize()
+    assert not info.fell_back, info
+    assert info.backward_error <= 1e-15, info
+    x = dX.owner.T  # the (cols, rows) tensor behind the column-major matrix
+    r = b - a @ x
+    den = a.abs().sum(1).max() * x.abs().max(0).values + b.abs().max(0).values
+    assert float((r.abs().max(0).values / den).max()) <= 2e-15
