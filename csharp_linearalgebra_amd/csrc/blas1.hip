@@ -194,8 +194,19 @@ __global__ void __launch_bounds__(256) panel_finish_kernel(int n, int t, int ksp
     size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
     if (e >= (size_t)n * t) return;
     int c = (int)(e / n), i = (int)(e % n);
-    T s = part[(size_t)c * n + i];
-    for (int q = 1; q < ksplit; q++) s = s_add(s, part[((size_t)q * t + c) * n + i]);
+    const T* p = part + (size_t)c * n + i;
+    const size_t qs = (size_t)t * n;  // from one k-chunk's partials to the next
+    T s = p[0];
+    int q = 1;
+    // Eight chunks' loads in flight at once, summed in chunk order.
+    for (; q + 7 < ksplit; q += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = p[(size_t)(q + u) * qs];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s = s_add(s, v[u]);
+    }
+    for (; q < ksplit; q++) s = s_add(s, p[(size_t)q * qs]);
     y[i + (size_t)c * ldy] = s;
 }
 
