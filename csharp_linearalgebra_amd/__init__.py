@@ -13,7 +13,9 @@ from .tensile import (  # noqa: F401
     ArgumentError,
     ArgumentOutOfRangeError,
     BatchedSystems,
+    DenseDeviceOperator,
     DeviceError,
+    DeviceLinearOperator,
     DeviceMatrix,
     ExpmDiagnostics,
     ExpmvDiagnostics,
@@ -21,6 +23,7 @@ from .tensile import (  # noqa: F401
     ILinearOperator,
     InvalidOperationError,
     LuDecomposition,
+    LuPhaseTimings,
     MixedInfo,
     NormEstimateResult,
     ObjectDisposedError,

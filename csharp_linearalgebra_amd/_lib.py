@@ -43,6 +43,11 @@ class MixedInfo(C.Structure):
                 ("fallback_ms", C.c_double), ("first_error", C.c_double)]
 
 
+class LuPhases(C.Structure):
+    _fields_ = [("panel_ms", C.c_double), ("swaps_ms", C.c_double), ("trsm_ms", C.c_double),
+                ("gemm_ms", C.c_double), ("total_ms", C.c_double), ("block_steps", C.c_int)]
+
+
 class ExpmvDiag(C.Structure):
     _fields_ = [("degree", C.c_int), ("scaling", C.c_int), ("applications", C.c_int), ("early_exits", C.c_int)]
 
@@ -67,6 +72,8 @@ SIGNATURES = {
     "tns_zgemm": (_I, [_P, _I, _I, _I, Complex, _P, _I, _P, _I, Complex, _P, _I]),
     "tns_dgetrf": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo)]),
     "tns_zgetrf": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo)]),
+    "tns_dgetrf_timed": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo), C.POINTER(LuPhases)]),
+    "tns_zgetrf_timed": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo), C.POINTER(LuPhases)]),
     "tns_dgetrs": (_I, [_P, _I, _I, _I, _P, _I, _P, _I, _P, _I]),
     "tns_zgetrs": (_I, [_P, _I, _I, _I, _P, _I, _P, _I, _P, _I]),
     "tns_dtrsm": (_I, [_P, _I, _I, _I, _P, _I, _P, _I]),

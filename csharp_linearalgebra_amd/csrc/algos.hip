@@ -597,6 +597,7 @@ void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out)
     int info = 0;
     TNS_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TNS_HIP(hipStreamSynchronize(ctx->stream));
+    if (info == LU_ABORTED) fail(TNS_EDEVICE, "the Pade denominator's LU panel exchange timed out");
     if (info >= 0)
         fail(TNS_ESTATE, "The Pade denominator is exactly singular at column " + std::to_string(info) +
                              ", which indicates a non-finite input rather than a property of the algorithm.");

@@ -410,6 +410,8 @@ void BatchPlan::solve(int* singular_columns)
         std::vector<int> h(count);
         TNS_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(int) * count, hipMemcpyDeviceToHost, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
+        for (int v : h)
+            if (v == LU_ABORTED) fail(TNS_EDEVICE, "a system's LU panel exchange timed out");
         std::copy(h.begin(), h.end(), singular_columns);
     }
 }
@@ -483,6 +485,8 @@ void BatchPlan::rcond(double* out, int t)
     TNS_HIP(hipMemcpyAsync(sing.data(), info.get(), sizeof(int) * count, hipMemcpyDeviceToHost, ctx->stream));
     TNS_HIP(hipMemcpyAsync(an.data(), anorm.get(), sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
     TNS_HIP(hipStreamSynchronize(ctx->stream));
+    for (int v : sing)
+        if (v == LU_ABORTED) fail(TNS_EDEVICE, "a system's LU panel exchange timed out");
 
     const size_t total = rc_off[count];
     std::vector<std::unique_ptr<NormEstRun<zd>>> runs(count);

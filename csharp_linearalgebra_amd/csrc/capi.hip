@@ -2,6 +2,7 @@
 // Tensile's seam (Operand.cs:80-96, KernelEntry.cs:246-269), staging of
 // host operands through the device, the context and its device pool.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -14,9 +15,17 @@
 #include "blas1.hpp"
 #include "mixed.hpp"
 
+struct tns_batch;
+
 struct tns_context {
     tns::Context impl;
-    bool disposed = false;
+    // Read without the lock for a fast refusal, then again under it: a call
+    // that was waiting on the lock while the context was destroyed must not
+    // run its body (Workspace.cs:243-253 checks disposal inside the lock).
+    std::atomic<bool> disposed{false};
+    // Live batches: their plans hold pool blocks and graphs, released with
+    // the context if the caller destroys the context first.
+    std::set<tns_batch*> batches;
 };
 
 struct tns_batch {
@@ -94,11 +103,18 @@ void Pool::trim()
     free_.clear();
 }
 
-Pool::~Pool()
+// Frees every block, lent or not; a handle that later gives one back finds
+// it unknown and drops it.
+void Pool::release_all()
 {
-    trim();
+    (void)hipDeviceSynchronize();
     for (auto& kv : sizes_) (void)hipFree(kv.first);
+    sizes_.clear();
+    free_.clear();
+    held_ = 0;
 }
+
+Pool::~Pool() { release_all(); }
 
 bool is_device_pointer(const void* p)
 {
@@ -153,17 +169,30 @@ char* stage(Context* ctx, size_t bytes)
 }
 
 // Host (ld) <-> device (packed, ld = rows) through the staging buffer, in
-// chunks of whole columns. Each chunk waits for its copy before the buffer
-// is reused, so the host pointer may be freed as soon as the call returns.
+// chunks of whole columns, or, for a column longer than the buffer, in
+// pieces of rows. Each chunk waits for its copy before the buffer is reused,
+// so the host pointer may be freed as soon as the call returns.
 template <typename T>
 void host_to_device(Context* ctx, T* d, const T* h, int rows, int cols, int ld)
 {
     const size_t col_bytes = sizeof(T) * (size_t)rows;
     char* buf = stage(ctx, col_bytes * cols);
-    const int per = (int)std::max<size_t>(1, ctx->h_stage_bytes / col_bytes);
+    const size_t cap = ctx->h_stage_bytes;
+    if (col_bytes > cap) {
+        const size_t piece = cap / sizeof(T);
+        for (int c = 0; c < cols; c++)
+            for (size_t r0 = 0; r0 < (size_t)rows; r0 += piece) {
+                const size_t nr = std::min(piece, (size_t)rows - r0);
+                std::memcpy(buf, h + (size_t)c * ld + r0, sizeof(T) * nr);
+                TNS_HIP(hipMemcpyAsync(d + (size_t)c * rows + r0, buf, sizeof(T) * nr, hipMemcpyHostToDevice,
+                                       ctx->stream));
+                TNS_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        return;
+    }
+    const int per = (int)(cap / col_bytes);
     for (int c0 = 0; c0 < cols; c0 += per) {
         const int nc = std::min(per, cols - c0);
-        if (ctx->h_stage_bytes < col_bytes * nc) buf = stage(ctx, col_bytes * nc);
         for (int c = 0; c < nc; c++) std::memcpy(buf + col_bytes * c, h + (size_t)(c0 + c) * ld, col_bytes);
         TNS_HIP(hipMemcpyAsync(d + (size_t)c0 * rows, buf, col_bytes * nc, hipMemcpyHostToDevice, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
@@ -175,10 +204,22 @@ void device_to_host(Context* ctx, T* h, const T* d, int rows, int cols, int ld)
 {
     const size_t col_bytes = sizeof(T) * (size_t)rows;
     char* buf = stage(ctx, col_bytes * cols);
-    const int per = (int)std::max<size_t>(1, ctx->h_stage_bytes / col_bytes);
+    const size_t cap = ctx->h_stage_bytes;
+    if (col_bytes > cap) {
+        const size_t piece = cap / sizeof(T);
+        for (int c = 0; c < cols; c++)
+            for (size_t r0 = 0; r0 < (size_t)rows; r0 += piece) {
+                const size_t nr = std::min(piece, (size_t)rows - r0);
+                TNS_HIP(hipMemcpyAsync(buf, d + (size_t)c * rows + r0, sizeof(T) * nr, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                TNS_HIP(hipStreamSynchronize(ctx->stream));
+                std::memcpy(h + (size_t)c * ld + r0, buf, sizeof(T) * nr);
+            }
+        return;
+    }
+    const int per = (int)(cap / col_bytes);
     for (int c0 = 0; c0 < cols; c0 += per) {
         const int nc = std::min(per, cols - c0);
-        if (ctx->h_stage_bytes < col_bytes * nc) buf = stage(ctx, col_bytes * nc);
         TNS_HIP(hipMemcpyAsync(buf, d + (size_t)c0 * rows, col_bytes * nc, hipMemcpyDeviceToHost, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
         for (int c = 0; c < nc; c++) std::memcpy(h + (size_t)(c0 + c) * ld, buf + col_bytes * c, col_bytes);
@@ -234,8 +275,9 @@ template <typename F>
 int guarded(tns_context* ctx, F&& body)
 {
     if (!ctx) return TNS_EINVAL;
-    if (ctx->disposed) return TNS_EDISPOSED;
+    if (ctx->disposed.load()) return TNS_EDISPOSED;
     std::lock_guard<std::mutex> lock(ctx->impl.gate);
+    if (ctx->disposed.load()) return TNS_EDISPOSED;
     try {
         TNS_HIP(hipSetDevice(ctx->impl.device));
         body(&ctx->impl);
@@ -289,29 +331,46 @@ inline zd c2(tns_complex z) { return make_zd(z.re, z.im); }
 // ---- generic bodies -----------------------------------------------------
 
 template <typename T>
-void getrf_body(Context* ctx, int m, int n, T* a, int lda, int nb, int* pivots, tns_lu_info* info)
+void getrf_body(Context* ctx, int m, int n, T* a, int lda, int nb, int* pivots, tns_lu_info* info,
+                tns_lu_phases* phases = nullptr)
 {
     check_operand(a, m, n, lda, "A");
     const int limit = std::min(m, n);
     if (limit > 0 && !pivots) fail(TNS_EINVAL, "pivots: null");
     View<T> va = bring(ctx, a, m, n, lda, true);
     DevBuf<int> piv(ctx, (size_t)std::max(limit, 1)), perm(ctx, (size_t)std::max(m, 1));
-    if constexpr (sizeof(T) == 8) tns::dgetrf(ctx, m, n, (double*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
-    else tns::zgetrf(ctx, m, n, (zd*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int);
+    tns::LuPhases split, *ps = phases ? &split : nullptr;
+    if constexpr (sizeof(T) == 8)
+        tns::dgetrf(ctx, m, n, (double*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int, ps);
+    else
+        tns::zgetrf(ctx, m, n, (zd*)va.d, va.ld, nb, piv.get(), perm.get(), ctx->d_int, ps);
     if (limit > 0) {
         if constexpr (sizeof(T) == 8) tns::dpivot_range(ctx, limit, (const double*)va.d, va.ld, ctx->d_scal);
         else tns::zpivot_range(ctx, limit, (const zd*)va.d, va.ld, ctx->d_scal);
+    }
+    int sing = -1;
+    double range[2] = {0.0, 0.0};
+    TNS_HIP(hipMemcpyAsync(&sing, ctx->d_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (limit > 0) TNS_HIP(hipMemcpyAsync(range, ctx->d_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    tns::sync(ctx);
+    // Checked before any output leaves the device.
+    if (sing == tns::LU_ABORTED) fail(TNS_EDEVICE, "an LU panel's cross-workgroup exchange timed out");
+    if (limit > 0) {
         if (tns::is_device_pointer(pivots))
             TNS_HIP(hipMemcpyAsync(pivots, piv.get(), sizeof(int) * limit, hipMemcpyDeviceToDevice, ctx->stream));
         else
             TNS_HIP(hipMemcpyAsync(pivots, piv.get(), sizeof(int) * limit, hipMemcpyDeviceToHost, ctx->stream));
     }
     send_back(ctx, va, a, m, n, lda);
-    int sing = -1;
-    double range[2] = {0.0, 0.0};
-    TNS_HIP(hipMemcpyAsync(&sing, ctx->d_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    if (limit > 0) TNS_HIP(hipMemcpyAsync(range, ctx->d_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     tns::sync(ctx);
+    if (phases) {
+        phases->panel_ms += split.panel_ms;
+        phases->swaps_ms += split.swaps_ms;
+        phases->trsm_ms += split.trsm_ms;
+        phases->gemm_ms += split.gemm_ms;
+        phases->total_ms += split.total_ms;
+        phases->block_steps += split.steps;
+    }
     if (info) {
         info->singular_column = sing;
         info->smallest_pivot = range[0];
@@ -619,12 +678,17 @@ TNS_API int tns_context_create(int device, tns_context** out)
 TNS_API int tns_context_destroy(tns_context* ctx)
 {
     if (!ctx) return TNS_EINVAL;
-    if (ctx->disposed) return TNS_OK;
+    if (ctx->disposed.load()) return TNS_OK;
     std::lock_guard<std::mutex> lock(ctx->impl.gate);
+    if (ctx->disposed.load()) return TNS_OK;
     tns::Context& c = ctx->impl;
     (void)hipSetDevice(c.device);
     (void)hipStreamSynchronize(c.stream);
-    c.pool.trim();
+    // Batches still alive give their blocks and graphs back now; their
+    // handles stay valid and refuse further calls with TNS_EDISPOSED.
+    for (tns_batch* b : ctx->batches) b->plan.reset();
+    ctx->batches.clear();
+    c.pool.release_all();
     if (c.h_stage) (void)hipHostFree(c.h_stage);
     c.h_stage = nullptr;
     c.h_stage_bytes = 0;
@@ -641,7 +705,7 @@ TNS_API int tns_context_destroy(tns_context* ctx)
 TNS_API const char* tns_last_error(const tns_context* ctx)
 {
     if (!ctx) return "null context";
-    if (ctx->disposed) return "the context has been destroyed";
+    if (ctx->disposed.load()) return "the context has been destroyed";
     return ctx->impl.last_error.c_str();
 }
 
@@ -721,6 +785,18 @@ TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, 
                        tns_lu_info* info)
 {
     return guarded(ctx, [&](Context* c) { getrf_body<zd>(c, m, n, (zd*)a, lda, nb, pivots, info); });
+}
+
+TNS_API int tns_dgetrf_timed(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots,
+                             tns_lu_info* info, tns_lu_phases* timings)
+{
+    return guarded(ctx, [&](Context* c) { getrf_body<double>(c, m, n, a, lda, nb, pivots, info, timings); });
+}
+
+TNS_API int tns_zgetrf_timed(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
+                             tns_lu_info* info, tns_lu_phases* timings)
+{
+    return guarded(ctx, [&](Context* c) { getrf_body<zd>(c, m, n, (zd*)a, lda, nb, pivots, info, timings); });
 }
 
 TNS_API int tns_dgetrs(tns_context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* pivots,
@@ -903,6 +979,7 @@ TNS_API int tns_zbatch_create(tns_context* ctx, int count, const int* n, tns_com
         batch->ctx = ctx;
         batch->plan = tns::make_batch(c, count, n, reinterpret_cast<zd* const*>(a), lda,
                                       reinterpret_cast<zd* const*>(b), ldb, nrhs, tune);
+        ctx->batches.insert(batch.get());
         *out = batch.release();
     });
 }
@@ -931,23 +1008,28 @@ TNS_API int tns_zbatch_pivots(tns_batch* batch, int index, int* pivots)
 TNS_API int tns_zbatch_info(tns_batch* batch, int* block, int* gemm_config, double* tuned_ms)
 {
     if (!batch) return TNS_EINVAL;
-    if (block) *block = batch->plan->block;
-    if (gemm_config) *gemm_config = batch->plan->gemm_cfg;
-    if (tuned_ms) *tuned_ms = batch->plan->tuned_ms;
-    return TNS_OK;
+    return guarded(batch->ctx, [&](Context*) {
+        if (block) *block = batch->plan->block;
+        if (gemm_config) *gemm_config = batch->plan->gemm_cfg;
+        if (tuned_ms) *tuned_ms = batch->plan->tuned_ms;
+    });
 }
 
+// The plan is released under the context's lock. After tns_context_destroy
+// it is already gone (the context released it with its pool), and only the
+// handle is freed here.
 TNS_API int tns_zbatch_destroy(tns_batch* batch)
 {
     if (!batch) return TNS_EINVAL;
-    if (batch->ctx && !batch->ctx->disposed) {
-        std::lock_guard<std::mutex> lock(batch->ctx->impl.gate);
-        (void)hipSetDevice(batch->ctx->impl.device);
-        (void)hipStreamSynchronize(batch->ctx->impl.stream);
-        batch->plan.reset();
-    } else if (batch->plan) {
-        batch->plan->ctx = nullptr;  // the context's pool is gone: drop, do not return
-        batch->plan.reset();
+    tns_context* ctx = batch->ctx;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->impl.gate);
+        if (!ctx->disposed.load()) {
+            (void)hipSetDevice(ctx->impl.device);
+            (void)hipStreamSynchronize(ctx->impl.stream);
+            batch->plan.reset();
+            ctx->batches.erase(batch);
+        }
     }
     delete batch;
     return TNS_OK;

@@ -29,7 +29,11 @@
 //    Pade denominator's n) use blocked substitution: each diagonal block
 //    solved by workgroups of 32 columns with the block in LDS, the
 //    off-diagonal update as a GEMM.
+#include <algorithm>
 #include <climits>
+#include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "grouped.hpp"
 #include "tns_internal.hpp"
@@ -427,6 +431,409 @@ __global__ void __launch_bounds__(NT) getf2_grouped_kernel(const PanelRec<T>* __
 {
     const PanelRec<T> r = recs[blockIdx.x];
     getf2_reg_body<T, NB, R, NT>(r.m, r.w, r.a, r.lda, r.piv, r.off, r.perm, r.list, 2 * r.w, r.info);
+}
+
+// ---------------------------------------------------------------------------
+// Panel factorization spread over P workgroups (the whole panel width in one
+// launch, no column chunks). Workgroup g holds panel rows [g*512, g*512+512)
+// in registers, one row per thread, across every panel column, and does the
+// work of getf2_reg_body for its rows with the same pivot rule, interchange
+// bookkeeping and update order. Per column each workgroup finds its best row
+// (DPP within a wavefront, LDS across wavefronts, one barrier) and, when
+// P > 1, joins an all-gather through device memory: it publishes its best
+// key (value and row position) and, speculatively, that row's values; one
+// wavefront polls the P keys, picks the winner by the same rule (largest
+// key, lowest row on ties) and fetches the winner's row into LDS for a
+// second barrier. Every published 64-bit word carries 32 bits of payload and
+// the column's sequence number (global column + 1) in its other half, so a
+// reader knows a word is current without any fence, and each relaxed atomic
+// word is written and read whole. Records alternate between two buffers by
+// column parity: a workgroup can be at most one column ahead of any other
+// (it needs every key of a column to leave it). The working workgroups are
+// blockIdx.x = 8g, which round-robin dispatch places on one XCD (one shared
+// L2); the others exit at once. Every wait gives up after SPREAD_SPIN polls
+// or when another workgroup has raised the abort word (xbuf[0]), so the
+// kernel always ends; the caller turns a raised abort word into an error.
+// P = 1 keeps everything in LDS. Rows are stored to their final positions
+// at the end; moved rows are appended to `list` (count in list[0], zero at
+// entry) by atomic reservation.
+// ---------------------------------------------------------------------------
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), each
+// call inlined: a loop whose index is a constant in every iteration.
+template <typename F, int... J>
+__device__ __forceinline__ void static_for_seq(F& f, std::integer_sequence<int, J...>)
+{
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F& f)
+{
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+// f(integral_constant<int, j>) for a run-time j in [LO, HI), by a binary
+// search of uniform branches: the callee sees j as a constant.
+template <int LO, int HI, typename F>
+__device__ __forceinline__ void dispatch(int j, F& f)
+{
+    if constexpr (HI - LO == 1) {
+        f(std::integral_constant<int, LO>{});
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        if (j < MID) dispatch<LO, MID>(j, f);
+        else dispatch<MID, HI>(j, f);
+    }
+}
+
+constexpr int LU_OUTER_FROM = 8192;      // order from which getrf_impl takes outer steps
+constexpr int SPREAD_NT = 512;           // threads per workgroup, one panel row each
+constexpr int SPREAD_MAXP = 32;          // workgroups: panels of up to 16384 rows
+constexpr int SPREAD_ROW_WORDS = 128;    // exchange words of one row: 64 doubles' halves
+constexpr int SPREAD_REC = 3 + SPREAD_ROW_WORDS;  // key (value lo, hi; row) + row
+constexpr int SPREAD_SPIN = 1 << 22;     // polls before a wait gives up
+// Exchange buffer length in 64-bit words: the abort word, then two parities
+// of SPREAD_MAXP records.
+constexpr size_t SPREAD_XWORDS = 1 + 2 * (size_t)SPREAD_MAXP * SPREAD_REC;
+// One panel's move list in ints: the count, then (destination, source) for
+// each moved row, at most two per column of a 64-column panel.
+constexpr int SPREAD_LIST = 1 + 2 * (2 * 64);
+
+__device__ __forceinline__ void xstore(unsigned long long* p, unsigned seq, unsigned payload)
+{
+    __hip_atomic_store(p, ((unsigned long long)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long xload(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until `count` consecutive words at p carry sequence number seq;
+// false if it gave up (too many polls, or the abort word is raised).
+template <int N>
+__device__ __forceinline__ bool xwait(const unsigned long long* p, unsigned seq, const unsigned long long* abort_word,
+                                      unsigned (&payload)[N])
+{
+    for (int it = 0; it < SPREAD_SPIN; ++it) {
+        bool all = true;
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const unsigned long long x = xload(p + q);
+            payload[q] = (unsigned)x;
+            all &= (unsigned)(x >> 32) == seq;
+        }
+        if (all) return true;
+        if ((it & 63) == 63 && xload(abort_word) != 0) return false;
+    }
+    return false;
+}
+
+// Largest key across the wavefront, ties to the lowest index, in every
+// lane: the maximum by DPP moves (as wave_best_dpp, on the key alone), then
+// the lanes holding it by one ballot; only a tie between lanes needs the
+// index comparison. *lane_of receives the winning lane. Keys are never NaN.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_max(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const long long idv = __double_as_longlong(-2.0);
+    const int lo = __builtin_amdgcn_update_dpp((int)idv, (int)b, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(idv >> 32), (int)(b >> 32), CTRL, ROWS, 0xF, false);
+    return fmax(x, __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ void wave_best_fast(double& bv, int& bi, int& lane_of)
+{
+    double mx = bv;
+    mx = dpp_max<0xB1>(mx);
+    mx = dpp_max<0x4E>(mx);
+    mx = dpp_max<0x124>(mx);
+    mx = dpp_max<0x128>(mx);
+    mx = dpp_max<0x142, 0xA>(mx);
+    mx = dpp_max<0x143, 0xC>(mx);
+    {
+        const long long b = __double_as_longlong(mx);
+        const int lo = __builtin_amdgcn_readlane((int)b, 63);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+        mx = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+    const unsigned long long hit = __ballot(bv == mx);
+    if (__popcll(hit) == 1) {
+        lane_of = __ffsll((long long)hit) - 1;
+        bi = __builtin_amdgcn_readlane(bi, lane_of);
+    } else {
+        double tv = bv == mx ? bv : -2.0;
+        int ti = bv == mx ? bi : INT_MAX;
+        wave_best_dpp(tv, ti);
+        lane_of = __ffsll((long long)__ballot(bv == mx && bi == ti)) - 1;
+        bi = ti;
+    }
+    bv = mx;
+}
+
+// v[c] -= prow[c] * l for c in [C1, NB): prow (LDS) read in chunks of 32
+// bytes, each chunk's reads issued before the previous chunk's arithmetic,
+// so the LDS latency is paid once per chunk pair instead of once per read.
+template <int C1, typename T, int NB>
+__device__ __forceinline__ void update_tail(T (&v)[NB], const T* __restrict__ prow, T l)
+{
+    constexpr int K = 32 / (int)sizeof(T);
+    constexpr int NCH = C1 < NB ? (NB - C1 + K - 1) / K : 0;
+    if constexpr (NCH > 0) {
+        T u[2][K];
+#pragma unroll
+        for (int q = 0; q < K; q++) u[0][q] = C1 + q < NB ? prow[C1 + q < NB ? C1 + q : 0] : s_zero(T{});
+        auto chunk = [&](auto kc) __attribute__((always_inline)) {
+            constexpr int KI = decltype(kc)::value;
+            if constexpr (KI + 1 < NCH) {
+#pragma unroll
+                for (int q = 0; q < K; q++) {
+                    const int c = C1 + (KI + 1) * K + q;
+                    u[(KI + 1) & 1][q] = c < NB ? prow[c < NB ? c : 0] : s_zero(T{});
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const int c = C1 + KI * K + q;
+                if (c < NB) v[c < NB ? c : 0] = s_add(v[c < NB ? c : 0], s_mul(s_neg(u[KI & 1][q]), l));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        static_for<NCH>(chunk);
+    }
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(SPREAD_NT)
+getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
+                    int* __restrict__ list, int P, unsigned long long* __restrict__ xbuf, int* __restrict__ info)
+{
+    static_assert(NB * sizeof(T) == 4 * SPREAD_ROW_WORDS, "a panel row is 128 exchange words");
+    static_assert(NB <= 64 && NB % 8 == 0, "one ballot covers the row; columns go in groups of 8");
+    constexpr int NT = SPREAD_NT, NW = NT / 64, GW = 8, NG = NB / GW;
+    constexpr int WPE = (int)sizeof(T) / 4;  // exchange words per element
+    int g = 0;
+    if (P > 1) {
+        if (blockIdx.x & 7) return;
+        g = blockIdx.x >> 3;
+    }
+    __shared__ double red_v[2][NW];
+    __shared__ int red_i[2][NW];
+    __shared__ T cand[2][NW][NB];  // each wavefront's best row
+    __shared__ T xrow[2][NB];      // the winning row, from the exchange
+    __shared__ int xp[2];          // its position
+    __shared__ int xabort;
+    __shared__ int spiv[NB];
+    __shared__ int moves;
+    __shared__ int base;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row = g * NT + tid;
+    const bool in = row < m;
+    T v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) v[c] = load_if(a + row + (size_t)c * lda, in && c < w);
+    const int p0 = in ? perm[row] : 0;
+    T nx = v[0];
+    int pos = row;
+    bool done = !in;
+    int first_zero = -1;
+    if (tid == 0) {
+        moves = 0;
+        xabort = 0;
+    }
+    const unsigned long long* abort_word = xbuf;
+    const int jl = m < w ? m : w;
+    int steps = 0;
+    bool stop = false;
+
+    // Columns in groups of GW: the group index G is a constant, the column
+    // inside it (jj) a run-time index. Columns of later groups are all right
+    // of j, so the candidate row and the update reach them with constant
+    // indices and no test; only the current group's eight columns need a
+    // per-column selection. nx is the row's column j, for the pivot search.
+    auto group = [&](auto gc) __attribute__((always_inline)) {
+        constexpr int G = decltype(gc)::value;
+        constexpr int C0 = G * GW;
+#pragma unroll 1
+        for (int jj = 0; jj < GW; ++jj) {
+            const int j = C0 + jj;
+            if (stop || j >= jl) {
+                stop = true;
+                break;
+            }
+            const int buf = j & 1;
+            double bv = done ? -2.0 : pivot_key(nx, pos, j);
+            int bi = done ? INT_MAX : pos;
+            int who;
+            wave_best_fast(bv, bi, who);
+            if (lane == 0) {
+                red_v[buf][wave] = bv;
+                red_i[buf][wave] = bi;
+            }
+            if (lane == who && !done) {
+#pragma unroll
+                for (int c = C0; c < NB; c++) cand[buf][wave][c] = v[c];
+            }
+            __syncthreads();
+            // Workgroup best: lane q reads wavefront q's record.
+            double kv = lane < NW ? red_v[buf][lane < NW ? lane : 0] : -2.0;
+            int ki = lane < NW ? red_i[buf][lane < NW ? lane : 0] : INT_MAX;
+            int wq;
+            wave_best_fast(kv, ki, wq);
+            bv = kv;
+            bi = ki;
+            int p = bi;
+            const T* prow = cand[buf][wq];
+            if (P > 1) {
+                const unsigned seq = (unsigned)(off + j + 1);
+                unsigned long long* recs = xbuf + 1 + (size_t)buf * SPREAD_MAXP * SPREAD_REC;
+                unsigned long long* mine = recs + (size_t)g * SPREAD_REC;
+                const unsigned* words = reinterpret_cast<const unsigned*>(prow);
+                // Only the columns from j on are read from a fetched row.
+                if (tid >= j * WPE && tid < SPREAD_ROW_WORDS) xstore(mine + 3 + tid, seq, words[tid]);
+                else if (tid >= SPREAD_ROW_WORDS && tid < SPREAD_ROW_WORDS + 3) {
+                    const int q = tid - SPREAD_ROW_WORDS;
+                    const unsigned long long kb = bits_of(bv);
+                    xstore(mine + q, seq, q == 0 ? (unsigned)kb : q == 1 ? (unsigned)(kb >> 32) : (unsigned)bi);
+                }
+                if (wave == 0) {
+                    double gv = -2.0;
+                    int gi = INT_MAX;
+                    bool ok = true;
+                    if (lane < P) {
+                        unsigned kw[3];
+                        ok = xwait<3>(recs + (size_t)lane * SPREAD_REC, seq, abort_word, kw);
+                        gv = double_of(((unsigned long long)kw[1] << 32) | kw[0]);
+                        gi = (int)kw[2];
+                    }
+                    int gw;
+                    wave_best_fast(gv, gi, gw);
+                    unsigned rw[2] = {0u, 0u};
+                    const bool need = 2 * lane + 1 >= j * WPE;
+                    if (__all(ok) && need)
+                        ok = xwait<2>(recs + (size_t)gw * SPREAD_REC + 3 + 2 * lane, seq, abort_word, rw);
+                    if (need)
+                        reinterpret_cast<unsigned long long*>(xrow[buf])[lane] =
+                            ((unsigned long long)rw[1] << 32) | rw[0];
+                    if (lane == 0) xp[buf] = gi;
+                    if (!__all(ok) && lane == 0) {
+                        xabort = 1;
+                        __hip_atomic_store(xbuf, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+                __syncthreads();
+                if (xabort) {
+                    stop = true;
+                    break;
+                }
+                p = xp[buf];
+                prow = xrow[buf];
+            }
+
+            const T pv = prow[j];
+            const bool zero_pivot = s_iszero(pv);
+            if (tid == 0) spiv[j] = off + p;
+            if (zero_pivot && first_zero < 0) first_zero = off + j;
+            // The pivot's row retires as row j of U; the row at position j
+            // takes the pivot's place. A zero pivot leaves the rows in place.
+            const int pe = zero_pivot ? j : p;
+            {
+                const bool is_p = !done & (pos == pe);
+                const bool is_j = !done & (pos == j) & !is_p;
+                done = done | is_p;
+                pos = is_p ? j : (is_j ? pe : pos);
+            }
+            steps = j + 1;
+            if (!done) {
+                // Multiplier: x * (1/pivot) when |pivot| >= the safe minimum,
+                // x / pivot below it, x itself for a zero pivot.
+                T l = nx;
+                if (!zero_pivot) {
+                    bool safe = s_pivmax(pv) >= safe_min<T>();
+                    if (!safe) safe = s_abs(pv) >= safe_min<T>();
+                    l = safe ? s_mul(l, s_div(s_one(T{}), pv)) : s_div(l, pv);
+                }
+                // Column j takes the multiplier and every column right of it
+                // is updated. For a finite multiplier a zero pivot-row entry
+                // leaves the value as it is, as the reference's skip does; a
+                // non-finite one times zero is NaN, so there the skip is
+                // followed exactly (the nonzero entries by one ballot).
+                unsigned long long keep = ~0ull;
+                if (!s_isfinite(l)) {
+                    const bool nz = lane < NB && lane < w && lane > j && !s_iszero(prow[lane < NB ? lane : 0]);
+                    keep = __ballot(nz);
+                }
+                const bool plain = keep == ~0ull;
+#pragma unroll
+                for (int q = 0; q < GW; q++) {
+                    const int c = C0 + q;
+                    const T nv = s_add(v[c], s_mul(s_neg(prow[c]), l));
+                    const bool upd = q > jj && (plain || ((keep >> c) & 1ull) != 0);
+                    v[c] = s_sel(q == jj, l, s_sel(upd, nv, v[c]));
+                }
+                if (plain) {
+                    update_tail<C0 + GW>(v, prow, l);
+                } else {
+#pragma unroll
+                    for (int c = C0 + GW; c < NB; c++)
+                        v[c] = s_sel(((keep >> c) & 1ull) != 0, s_add(v[c], s_mul(s_neg(prow[c]), l)), v[c]);
+                }
+            }
+            // nx moves on to column j + 1.
+            T nn = C0 + GW < NB ? v[C0 + GW < NB ? C0 + GW : 0] : nx;
+#pragma unroll
+            for (int q = 1; q < GW; q++) nn = s_sel(q == jj + 1, v[C0 + q], nn);
+            nx = nn;
+        }
+    };
+    static_for<NG>(group);
+
+    // Rows to their final positions. Every workgroup loaded its rows and
+    // payloads before column 0, and none gets past column 0 before all have
+    // joined it, so no store here can overtake another workgroup's load.
+    int at = -1;
+    if (in) {
+        const int dst = pos;
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            if (c < w) a[dst + (size_t)c * lda] = v[c];
+        perm[dst] = p0;
+        if (dst != row) at = atomicAdd(&moves, 1);
+    }
+    __syncthreads();
+    if (tid == 0) base = moves > 0 ? atomicAdd(list, moves) : 0;
+    __syncthreads();
+    if (at >= 0 && base + at < (SPREAD_LIST - 1) / 2) {
+        list[1 + 2 * (base + at)] = pos;
+        list[2 + 2 * (base + at)] = row;
+    }
+    if (g == 0 && tid == 0) {
+        for (int q = 0; q < steps; ++q) piv[q] = spiv[q];
+        if (first_zero >= 0) atomicMin(info, first_zero);
+    }
+}
+
+// Zeroes the spread panel's exchange words and move-list counters and sets
+// up the row payloads (d_perm = identity, rc = identity when given).
+__global__ void spread_init_kernel(int m, int* __restrict__ perm, int* __restrict__ rc,
+                                   unsigned long long* __restrict__ xbuf, int xwords, int* __restrict__ lists,
+                                   int nlist, int list_stride)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        perm[i] = i;
+        if (rc) rc[i] = i;
+    }
+    if (i < xwords) xbuf[i] = 0ull;
+    if (i < nlist) lists[(size_t)i * list_stride] = 0;
+}
+
+// Sets info to -2 when a spread panel gave up waiting (the abort word).
+__global__ void spread_check_kernel(const unsigned long long* __restrict__ xbuf, int* __restrict__ info)
+{
+    if (xbuf[0] != 0ull) *info = -2;
 }
 
 // Panel factorization straight from global memory, for panels taller than
@@ -1375,6 +1782,46 @@ __global__ void __launch_bounds__(256) pivot_range_kernel(int n, const T* __rest
 // ---------------------------------------------------------------------------
 // Host-side drivers.
 // ---------------------------------------------------------------------------
+// The LU phase split (LuPhaseTimings, Lu.cs:92-164): an event is recorded on
+// the stream after each phase, and the interval it closes is charged to that
+// phase's bucket; `total` is measured from the first event to the last, so
+// time no phase accounts for stays visible as the difference.
+struct PhaseClock {
+    enum { NONE = -1, PANEL = 0, SWAPS = 1, TRSM = 2, GEMM = 3 };
+    Context* ctx;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> bucket;
+    explicit PhaseClock(Context* c) : ctx(c) {}
+    PhaseClock(const PhaseClock&) = delete;
+    PhaseClock& operator=(const PhaseClock&) = delete;
+    ~PhaseClock()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void mark(int b)
+    {
+        hipEvent_t e = nullptr;
+        TNS_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        bucket.push_back(b);
+        TNS_HIP(hipEventRecord(e, ctx->stream));
+    }
+    // Adds this factorization's split to *out (pooling, as LuPhaseTimings.Add).
+    void read(LuPhases* out)
+    {
+        TNS_HIP(hipEventSynchronize(ev.back()));
+        double* dst[4] = {&out->panel_ms, &out->swaps_ms, &out->trsm_ms, &out->gemm_ms};
+        for (size_t i = 1; i < ev.size(); i++) {
+            float ms = 0.f;
+            TNS_HIP(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+            if (bucket[i] >= 0) *dst[bucket[i]] += ms;
+        }
+        float all = 0.f;
+        TNS_HIP(hipEventElapsedTime(&all, ev.front(), ev.back()));
+        out->total_ms += all;
+    }
+};
+
 template <typename T>
 struct Ops;
 template <>
@@ -1645,36 +2092,147 @@ void panel(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* pe
     }
 }
 
+// Outer step of getrf_impl: a multiple of nb. Measured on MI355X (FP64 LU,
+// nb = 64): at order 8192 four panels per outer step, the trailing products
+// of depth 256 instead of 64, 56.2 -> 53.5 ms (the products 16.9 -> 10.3 ms,
+// the block-row solves 4.9 -> 9.1 ms); at 4096 slower (20.0 -> 22.1 ms), so
+// below 8192 one.
 template <typename T>
-void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+int lu_outer_block(int m, int n, int nb)
 {
+    const int limit = m < n ? m : n;
+    if (sizeof(T) >= 8 && limit >= LU_OUTER_FROM && nb <= 64) return 4 * nb;
+    return nb;
+}
+
+// Widest panel the spread kernel takes for T (0: it does not take T).
+template <typename T>
+constexpr int spread_nb()
+{
+    return sizeof(T) == 8 ? 64 : sizeof(T) == 16 ? 32 : 0;
+}
+
+// Factors the m x w panel at a with getf2_spread_kernel and moves its
+// interchanges to the `left` columns before it and `right` after it.
+template <typename T>
+void panel_spread(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, int* perm, int* info, int* list,
+                  unsigned long long* xbuf, int left, int right, PhaseClock* clk)
+{
+    constexpr int NB = spread_nb<T>();
+    if constexpr (NB > 0) {
+        const int P = (m + SPREAD_NT - 1) / SPREAD_NT;
+        hipLaunchKernelGGL((getf2_spread_kernel<T, NB>), dim3(P > 1 ? 8 * P : 1), dim3(SPREAD_NT), 0, ctx->stream, m,
+                           w, a, lda, piv, off, perm, list, P, xbuf, info);
+        if (clk) clk->mark(PhaseClock::PANEL);
+        const int jl = m < w ? m : w;
+        apply_moves<T>(ctx, a - (size_t)left * lda, lda, left, left + w, left + w + right, list, 2 * jl);
+        if (clk) clk->mark(PhaseClock::SWAPS);
+    }
+}
+
+// work: scratch carved by the caller (lu_work_ints(m, n, nb) ints, 8-byte
+// aligned), or null to take it from the pool.
+template <typename T>
+void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info,
+                LuPhases* phases = nullptr, int* work = nullptr)
+{
+    PhaseClock clock(ctx), *clk = phases ? &clock : nullptr;
+    if (clk) clk->mark(PhaseClock::NONE);
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info, INT_MAX);
     const int limit = m < n ? m : n;
+    bool spread_used = false;
+    unsigned long long* xbuf = nullptr;
+    DevBuf<int> pooled;
     if (limit > 0) {
         if (nb < 1) nb = limit < 1024 ? 32 : 64;
         const int maxw = nb < limit ? nb : n;
-        DevBuf<int> rc(ctx, (size_t)m);
-        DevBuf<int> list(ctx, 4 * (size_t)maxw + 4);
-        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, d_perm, rc.get());
-        if (nb >= limit) {
-            // Small problems are all panel (Lu.cs:244-249).
-            panel<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, rc, d_info, list, 0, 0);
+        const bool whole = nb >= limit;  // small problems are all panel (Lu.cs:244-249)
+        constexpr int SNB = spread_nb<T>();
+        // Panels of the spread kernel's width inside outer steps of the block
+        // size when the block is wider than the kernel takes (complex at
+        // nb = 64); the pivots are the same whatever the panel width.
+        const int ib = (SNB > 0 && !whole && nb > SNB && m <= SPREAD_NT * SPREAD_MAXP) ? SNB : nb;
+        const int steps = whole ? 1 : (limit + ib - 1) / ib;
+        const bool spread = SNB > 0 && (whole ? maxw : ib) <= SNB && m <= SPREAD_NT * SPREAD_MAXP;
+        spread_used = spread;
+        const size_t need = lu_work_ints(m, n, nb);
+        if (!work) {
+            pooled = DevBuf<int>(ctx, need);
+            work = pooled.get();
+        }
+        // Layout: the exchange words first (8-byte aligned), then the
+        // per-panel move lists, the chunked panels' list, the row ids.
+        xbuf = reinterpret_cast<unsigned long long*>(work);
+        int* lists = work + 2 * SPREAD_XWORDS;
+        const int lstride = SPREAD_LIST;
+        int* list = lists + (size_t)steps * lstride;
+        int* rc = list + 4 * (size_t)maxw + 4;
+        const int xwords = spread ? (int)SPREAD_XWORDS : 0;
+        const int init = std::max(std::max(m, xwords), spread ? steps : 0);
+        hipLaunchKernelGGL(spread_init_kernel, dim3((init + 255) / 256), dim3(256), 0, ctx->stream, m, d_perm, rc,
+                           xbuf, xwords, lists, spread ? steps : 0, lstride);
+        if (clk) clk->mark(PhaseClock::NONE);
+        if (whole) {
+            if (spread) {
+                panel_spread<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, d_info, lists, xbuf, 0, 0, clk);
+            } else {
+                panel<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, rc, d_info, list, 0, 0);
+                if (clk) clk->mark(PhaseClock::PANEL);
+            }
+            if (phases) phases->steps += 1;
         } else {
-            for (int jb = 0; jb < limit; jb += nb) {
-                const int w = nb < limit - jb ? nb : limit - jb;
-                T* pa = a + jb + (size_t)jb * lda;
-                panel<T>(ctx, m - jb, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb,
-                         n - jb - w);
-                if (jb + w >= n) continue;
-                trsm_llnu<T>(ctx, w, n - jb - w, pa, lda, pa + (size_t)w * lda, lda);
-                if (jb + w >= m) continue;
-                Ops<T>::gemm(ctx, 0, m - jb - w, n - jb - w, w, Ops<T>::neg_one(), pa + w, lda, pa + (size_t)w * lda,
-                             lda, Ops<T>::one(), pa + w + (size_t)w * lda, lda);
+            // Outer steps of ob columns: inside one, each panel's U12 solve
+            // and trailing update reach only the outer step's columns; the
+            // columns beyond take their block row by one solve with the outer
+            // step's L11 and one product of depth ob, so the trailing matrix
+            // is read and written once per outer step instead of once per
+            // panel. Every panel sees the same updated columns as with
+            // ob = nb, so the pivots are the same.
+            const int ob = lu_outer_block<T>(m, n, nb);
+            int step = 0;
+            for (int jo = 0; jo < limit; jo += ob) {
+                const int wo = ob < limit - jo ? ob : limit - jo;
+                const int end = jo + wo < n ? jo + wo : n;  // columns the inner steps update
+                for (int jb = jo; jb < jo + wo; jb += ib, ++step) {
+                    const int w = ib < jo + wo - jb ? ib : jo + wo - jb;
+                    T* pa = a + jb + (size_t)jb * lda;
+                    if (spread) {
+                        panel_spread<T>(ctx, m - jb, w, pa, lda, d_piv + jb, jb, d_perm + jb, d_info,
+                                        lists + (size_t)step * lstride, xbuf, jb, n - jb - w, clk);
+                    } else {
+                        panel<T>(ctx, m - jb, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc + jb, d_info, list, jb,
+                                 n - jb - w);
+                        if (clk) clk->mark(PhaseClock::PANEL);
+                    }
+                    if (phases) phases->steps += 1;
+                    const int right = end - jb - w;
+                    if (right <= 0) continue;
+                    trsm_llnu<T>(ctx, w, right, pa, lda, pa + (size_t)w * lda, lda);
+                    if (clk) clk->mark(PhaseClock::TRSM);
+                    if (jb + w >= m) continue;
+                    Ops<T>::gemm(ctx, 0, m - jb - w, right, w, Ops<T>::neg_one(), pa + w, lda, pa + (size_t)w * lda,
+                                 lda, Ops<T>::one(), pa + w + (size_t)w * lda, lda);
+                    if (clk) clk->mark(PhaseClock::GEMM);
+                }
+                const int beyond = n - end;
+                if (beyond <= 0) continue;
+                T* po = a + jo + (size_t)jo * lda;
+                trsm_llnu<T>(ctx, wo, beyond, po, lda, po + (size_t)wo * lda, lda);
+                if (clk) clk->mark(PhaseClock::TRSM);
+                if (jo + wo >= m) continue;
+                Ops<T>::gemm(ctx, 0, m - jo - wo, beyond, wo, Ops<T>::neg_one(), po + wo, lda, po + (size_t)wo * lda,
+                             lda, Ops<T>::one(), po + wo + (size_t)wo * lda, lda);
+                if (clk) clk->mark(PhaseClock::GEMM);
             }
         }
     }
     hipLaunchKernelGGL(finish_info_kernel, dim3(1), dim3(1), 0, ctx->stream, d_info);
+    if (spread_used) hipLaunchKernelGGL(spread_check_kernel, dim3(1), dim3(1), 0, ctx->stream, xbuf, d_info);
     TNS_HIP(hipGetLastError());
+    if (clk) {
+        clk->mark(PhaseClock::NONE);
+        clk->read(phases);
+    }
 }
 
 // LU with the pivots of each block step chosen inside its NB x NB
@@ -2056,14 +2614,26 @@ void getrs_blockinv_impl(Context* ctx, int n, int nrhs, const T* lu, int ldlu, c
 
 }  // namespace
 
-void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+size_t lu_work_ints(int m, int n, int nb)
 {
-    getrf_impl<double>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+    const int limit = m < n ? m : n;
+    if (nb < 1) nb = limit < 1024 ? 32 : 64;
+    const int maxw = nb < limit ? nb : n;
+    // Panels are at least 32 columns wide (the complex spread kernel's width).
+    const int steps = nb >= limit ? 1 : (limit + 31) / 32 + (limit + nb - 1) / nb;
+    return (size_t)m + 4 * (size_t)maxw + 4 + (size_t)steps * SPREAD_LIST + 2 * SPREAD_XWORDS;
 }
 
-void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)
+void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info,
+            LuPhases* phases, int* work)
 {
-    getrf_impl<zd>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info);
+    getrf_impl<double>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info, phases, work);
+}
+
+void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info,
+            LuPhases* phases, int* work)
+{
+    getrf_impl<zd>(ctx, m, n, a, lda, nb, d_piv, d_perm, d_info, phases, work);
 }
 
 void sgetrf(Context* ctx, int m, int n, float* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info)

@@ -632,6 +632,7 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
         int hinfo = -1;
         TNS_HIP(hipMemcpyAsync(&hinfo, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
+        if (hinfo == LU_ABORTED) fail(TNS_EDEVICE, "the FP64 fallback's panel exchange timed out");
         if (hinfo >= 0) fail(TNS_ESTATE, "the matrix is singular: the FP64 fallback met a zero pivot");
         dcopy2d(ctx, n, t, b, ldb, x, ldx);
         dgetrs(ctx, 0, n, t, f.get(), n, perm.get(), x, ldx);

@@ -43,6 +43,7 @@ public:
     void* take(size_t bytes);
     void give(void* p, size_t bytes);
     void trim();
+    void release_all();
     size_t held() const { return held_; }
     size_t limit() const { return limit_; }
     void set_limit(size_t l) { limit_ = l; }
@@ -104,9 +105,6 @@ struct Context {
     size_t h_stage_bytes = 0;
     std::string last_error;
     std::mutex gate;            // the Workspace lock (Workspace.cs:136-139)
-    bool timing = false;
-    double phase_ms[8] = {0};   // LU phase split when timing is on
-    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 template <typename T>
@@ -152,11 +150,26 @@ void zgemm_ex(Context* ctx, int ta, int m, int n, int k, cplx alpha, const cplx*
 void gemm_force_config(int real_cfg, int cplx_cfg);
 int gemm_config_count(bool complex_);
 
+// The LU phase split (LuPhaseTimings, Lu.cs:92-164), in milliseconds of
+// device time; a factorization adds its split to the fields.
+struct LuPhases {
+    double panel_ms = 0, swaps_ms = 0, trsm_ms = 0, gemm_ms = 0, total_ms = 0;
+    int steps = 0;
+};
+
 // LU (lu.hip). d_piv: min(m,n) LAPACK-order interchanges (0-based);
 // d_perm: m entries, row i of P*A is row d_perm[i] of A; d_info: the first
-// exactly-zero pivot column or -1.
-void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info);
-void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info);
+// exactly-zero pivot column, -1, or LU_ABORTED if a panel's cross-workgroup
+// exchange gave up waiting (a device fault, never a property of A).
+// phases, if given, receives the phase split (the call then waits for the
+// factorization). work, if given, is device scratch of lu_work_ints(m, n,
+// nb) ints, 8-byte aligned, used instead of pool memory.
+constexpr int LU_ABORTED = -2;
+size_t lu_work_ints(int m, int n, int nb);
+void dgetrf(Context* ctx, int m, int n, double* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info,
+            LuPhases* phases = nullptr, int* work = nullptr);
+void zgetrf(Context* ctx, int m, int n, cplx* a, int lda, int nb, int* d_piv, int* d_perm, int* d_info,
+            LuPhases* phases = nullptr, int* work = nullptr);
 // Solves against the factors; trans 1 = A^T (real) / A^H (complex).
 void dgetrs(Context* ctx, int trans, int n, int nrhs, const double* lu, int ldlu, const int* d_perm,
             double* b, int ldb);

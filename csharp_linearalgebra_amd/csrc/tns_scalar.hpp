@@ -96,6 +96,11 @@ __host__ __device__ inline double s_pivmax(zd a)
     return (a.x != a.x || a.y != a.y) ? a.x + a.y : fmax(fabs(a.x), fabs(a.y));
 }
 
+// Both parts finite.
+__host__ __device__ inline bool s_isfinite(double a) { return __builtin_isfinite(a); }
+__host__ __device__ inline bool s_isfinite(zd a) { return __builtin_isfinite(a.x) && __builtin_isfinite(a.y); }
+__host__ __device__ inline bool s_isfinite(float a) { return __builtin_isfinite(a); }
+
 // c ? a : b, component by component (a conditional expression over
 // struct lvalues would select an address; see load_if).
 __host__ __device__ inline double s_sel(bool c, double a, double b) { return c ? a : b; }

@@ -209,8 +209,10 @@ class Workspace:
         return c
 
     # ---- LU (Workspace.FactorLu, Workspace.cs:208-233) -------------------------
-    def factor_lu(self, a, block_size: int = 0) -> "LuDecomposition":
-        """Factor a copy of A (MatrixOperations.FactorLu clones, MatrixOperations.cs:76-84)."""
+    def factor_lu(self, a, block_size: int = 0, timings: "LuPhaseTimings" = None) -> "LuDecomposition":
+        """Factor a copy of A (MatrixOperations.FactorLu clones, MatrixOperations.cs:76-84).
+        With `timings`, the factorization's phase split is added to it
+        (Workspace.FactorLu's LuPhaseTimings argument, Workspace.cs:208-233)."""
         cplx = _is_complex(a)
         f = np.array(a, dtype=np.complex128 if cplx else np.float64, order="F", copy=True)
         if f.ndim != 2:
@@ -219,11 +221,33 @@ class Workspace:
         anorm = self.one_norm(f) if f.size else 0.0
         piv = np.zeros(max(1, min(m, n)), dtype=np.int32)
         info = L.LuInfo()
-        fn = self._lib.tns_zgetrf if cplx else self._lib.tns_dgetrf
-        self.check(fn(self._h, m, n, f.ctypes.data if f.size else None, max(1, m), int(block_size),
-                      piv.ctypes.data, C.byref(info)))
+        args = (self._h, m, n, f.ctypes.data if f.size else None, max(1, m), int(block_size), piv.ctypes.data,
+                C.byref(info))
+        if timings is None:
+            fn = self._lib.tns_zgetrf if cplx else self._lib.tns_dgetrf
+            self.check(fn(*args))
+        else:
+            fn = self._lib.tns_zgetrf_timed if cplx else self._lib.tns_dgetrf_timed
+            self.check(fn(*args, C.byref(timings._raw)))
         return LuDecomposition(self, f, piv[: min(m, n)].copy(), info.singular_column, info.smallest_pivot,
                                info.largest_pivot, anorm)
+
+    def factor_lu_device(self, a: "DeviceMatrix", pivots: "DeviceMatrix" = None, block_size: int = 0,
+                         timings: "LuPhaseTimings" = None):
+        """Factor a device matrix in place (KernelEntry.FactorLu on device
+        memory). Returns (host pivots, singular column)."""
+        cplx = a.complex_
+        m, n = a.rows, a.cols
+        piv = np.zeros(max(1, min(m, n)), dtype=np.int32)
+        info = L.LuInfo()
+        args = (self._h, m, n, a.ptr, a.ld, int(block_size), piv.ctypes.data, C.byref(info))
+        if timings is None:
+            fn = self._lib.tns_zgetrf if cplx else self._lib.tns_dgetrf
+            self.check(fn(*args))
+        else:
+            fn = self._lib.tns_zgetrf_timed if cplx else self._lib.tns_dgetrf_timed
+            self.check(fn(*args, C.byref(timings._raw)))
+        return piv[: min(m, n)].copy(), info.singular_column
 
     # ---- norms (MatrixOperations.cs:142-174) ----------------------------------------
     def _norm(self, a, kind):
@@ -383,6 +407,40 @@ class Workspace:
         fn = self._lib.tns_zexpmv_op if cplx else self._lib.tns_dexpmv_op
         self.check(fn(self._h, n, fnp, None, 0, B.cols, B.ptr, B.ld, float(t), float(one_norm_bound),
                       F.ctypes.data if F.size else None, max(1, n), C.byref(d)))
+        return F, ExpmvDiagnostics(d.degree, d.scaling, d.applications, d.early_exits)
+
+    def expmv_device_operator(self, op: "DeviceLinearOperator", b, t: float, one_norm_bound: float, out=None):
+        """The matrix-free action over a device-resident operator
+        (MatrixExponentialAction.cs:203-230 with ILinearOperator<T>,
+        LinearOperators.cs:37-75, applied to panels in device memory): every
+        application hands op.apply_device two DeviceMatrix panels and the
+        context's stream; the operator's work must be complete, or ordered on
+        that stream, when it returns. B and the result may be host or device
+        matrices."""
+        cplx = bool(op.complex_)
+        B = _Operand(b, cplx)
+        n = op.order
+        if B.rows != n:
+            raise ArgumentError(f"B has {B.rows} rows, expected the operator's order {n}.")
+        F = out if out is not None else _host_out(B.rows, B.cols, cplx)
+        Fo = _Operand(F, cplx) if isinstance(F, DeviceMatrix) else None
+        fptr, fld = (Fo.ptr, Fo.ld) if Fo else (F.ctypes.data if F.size else None, max(1, n))
+        stream = self.stream()
+
+        def cb(_user, nn, tt, x, ldx, y, ldy):
+            try:
+                op.apply_device(DeviceMatrix(int(x), nn, tt, ldx, cplx), DeviceMatrix(int(y), nn, tt, ldy, cplx),
+                                stream)
+                return 0
+            except BaseException as e:  # surfaced by check() as OperatorError
+                self._pending = e
+                return 1
+
+        fnp = L.ZAPPLY(cb) if cplx else L.DAPPLY(cb)
+        d = L.ExpmvDiag()
+        fn = self._lib.tns_zexpmv_op if cplx else self._lib.tns_dexpmv_op
+        self.check(fn(self._h, n, fnp, None, 1, B.cols, B.ptr, B.ld, float(t), float(one_norm_bound), fptr, fld,
+                      C.byref(d)))
         return F, ExpmvDiagnostics(d.degree, d.scaling, d.applications, d.early_exits)
 
 
@@ -609,6 +667,67 @@ class ILinearOperator:
         raise NotImplementedError
 
 
+class DeviceLinearOperator:
+    """ILinearOperator<T> whose panels live in device memory
+    (LinearOperators.cs:37-75): apply_device(x, y, stream) writes Op*x into
+    y, both DeviceMatrix panels of `order` rows; its work must be complete,
+    or ordered on `stream` (the calling context's), when it returns.
+    Subclasses set `order` and `complex_`."""
+
+    order: int = 0
+    complex_: bool = False
+
+    def apply_device(self, x: DeviceMatrix, y: DeviceMatrix, stream: int):  # pragma: no cover - interface
+        raise NotImplementedError
+
+
+class DenseDeviceOperator(DeviceLinearOperator):
+    """A device matrix as a device-resident operator: y := A x by this
+    library's product on a second workspace (the calling context is busy
+    with the action), synchronized before returning."""
+
+    def __init__(self, a: DeviceMatrix, workspace: "Workspace"):
+        if a.rows != a.cols:
+            raise ArgumentError("the operator must be square")
+        self.a = a
+        self.order = a.rows
+        self.complex_ = a.complex_
+        self._ws = workspace
+
+    def apply_device(self, x, y, stream):
+        self._ws.multiply(self.a, x, y)
+        self._ws.synchronize()
+
+
+class LuPhaseTimings:
+    """LuPhaseTimings (Lu.cs:92-164): device milliseconds charged to the
+    panel factorizations, the row interchanges, the U12 triangular solves and
+    the trailing GEMMs, the whole factorization's total, and the block steps;
+    a factorization passed this object adds its split (pooling, as Add)."""
+
+    def __init__(self):
+        self._raw = L.LuPhases()
+
+    panel = property(lambda self: self._raw.panel_ms)
+    swaps = property(lambda self: self._raw.swaps_ms)
+    triangular = property(lambda self: self._raw.trsm_ms)
+    gemm = property(lambda self: self._raw.gemm_ms)
+    total = property(lambda self: self._raw.total_ms)
+    block_steps = property(lambda self: self._raw.block_steps)
+
+    @property
+    def unattributed(self) -> float:
+        return max(0.0, self.total - (self.panel + self.swaps + self.triangular + self.gemm))
+
+    def share(self, ms: float) -> float:
+        """A phase's share of the total, as a percentage."""
+        return 0.0 if self.total == 0 else 100.0 * ms / self.total
+
+    def add(self, other: "LuPhaseTimings"):
+        for f in ("panel_ms", "swaps_ms", "trsm_ms", "gemm_ms", "total_ms", "block_steps"):
+            setattr(self._raw, f, getattr(self._raw, f) + getattr(other._raw, f))
+
+
 # ---- fluent functions over the shared workspace (MatrixOperations.cs) ---------------------
 def _ws(workspace):
     return workspace if workspace is not None else Workspace.shared()
@@ -622,8 +741,8 @@ def multiply_into(a, b, destination, alpha=1.0, beta=0.0, workspace=None):
     return _ws(workspace).multiply(a, b, destination, alpha, beta)
 
 
-def factor_lu(a, block_size=0, workspace=None) -> LuDecomposition:
-    return _ws(workspace).factor_lu(a, block_size)
+def factor_lu(a, block_size=0, workspace=None, timings=None) -> LuDecomposition:
+    return _ws(workspace).factor_lu(a, block_size, timings)
 
 
 def solve(a, b, workspace=None):

@@ -21,7 +21,10 @@
  *   - padding rows between rows and ld are never written (GemmTests.cs:107-129);
  *   - on any failure the destination is left untouched (invariant I5,
  *     ComplexKernels.cs:103-106) and the status says why; no C++ exception
- *     crosses this boundary.
+ *     crosses this boundary. The exception is an in-place call on device
+ *     memory (tns_dgetrf / tns_zgetrf on a device matrix): the factorization
+ *     overwrites the matrix as it runs, so a failure reported at its end
+ *     (TNS_EDEVICE) leaves the matrix partly factored.
  *
  * A context is not thread-safe; the C# Workspace keeps its lock around it
  * (Workspace.cs:136-139).
@@ -133,6 +136,28 @@ TNS_API int tns_dgetrf(tns_context* ctx, int m, int n, double* a, int lda, int n
                        tns_lu_info* info);
 TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
                        tns_lu_info* info);
+
+/* ---- the same with the phase split: KernelEntry.FactorLu's optional
+ * LuPhaseTimings (KernelEntry.cs:59-67, Lu.cs:92-164, hooked in at
+ * Lu.cs:251-305). The device time of the panel factorizations, the row
+ * interchanges either side of the panels, the U12 triangular solves and the
+ * trailing GEMMs is ADDED to *timings (pooling over runs, as
+ * LuPhaseTimings.Add); total_ms spans the whole factorization, so
+ * total_ms - (panel + swaps + trsm + gemm) is the time no phase accounts
+ * for. block_steps counts the panels. Events are recorded between phases,
+ * so the call waits for the factorization. timings may be null. */
+typedef struct {
+    double panel_ms;
+    double swaps_ms;
+    double trsm_ms;
+    double gemm_ms;
+    double total_ms;
+    int block_steps;
+} tns_lu_phases;
+TNS_API int tns_dgetrf_timed(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots,
+                             tns_lu_info* info, tns_lu_phases* timings);
+TNS_API int tns_zgetrf_timed(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
+                             tns_lu_info* info, tns_lu_phases* timings);
 
 /* ---- LU solves: KernelEntry.SolveLu / SolveLuTransposed (KernelEntry.cs:70-95),
  * Lu.Solve / SolveTransposed (Lu.cs:336-378); complex BlockedLu.Solve /
@@ -290,7 +315,9 @@ TNS_API int tns_zexpmv_grad(tns_context* ctx, int n, int nb, const tns_complex* 
  * OneNormOfA, LuDecomposition.cs:37-80), and Condition.ReciprocalOne
  * (NormEstimate.cs:432-466) for rcond. All matrices must be device memory;
  * the batch records the pointers, so they must stay valid until
- * tns_zbatch_destroy. tune: 0 defaults, 1 the context's remembered choice for
+ * tns_zbatch_destroy. A batch belongs to its context: tns_context_destroy
+ * releases the device memory of every batch still alive, whose handle then
+ * refuses calls with TNS_EDISPOSED until tns_zbatch_destroy frees it. tune: 0 defaults, 1 the context's remembered choice for
  * this batch shape (tuning on scratch copies if there is none), 2 tune
  * again. tns_zbatch_solve overwrites every A_i with its LU factors and every
  * B_i with X_i; refill them before solving again. singular_columns (host,
@@ -304,7 +331,7 @@ TNS_API int tns_zbatch_create(tns_context* ctx, int count, const int* n, tns_com
 TNS_API int tns_zbatch_solve(tns_batch* batch, int* singular_columns);
 TNS_API int tns_zbatch_rcond(tns_batch* batch, int columns, double* rcond);
 TNS_API int tns_zbatch_pivots(tns_batch* batch, int index, int* pivots);
-/* What the autotuner chose: the LU panel width (16 or 32), the tile of the
+/* What the autotuner chose: the LU panel width (32 or 64), the tile of the
  * grouped 4M trailing update (1: 64 x 64, 2: 64 x 32), and the best graph
  * time it measured (0 if it did not run). No counterpart in the reference. */
 TNS_API int tns_zbatch_info(tns_batch* batch, int* block, int* gemm_config, double* tuned_ms);
