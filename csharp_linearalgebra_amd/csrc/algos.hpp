@@ -10,6 +10,7 @@
 #include <functional>
 #include <vector>
 
+#include "dotnet_bcl.hpp"
 #include "tns_internal.hpp"
 
 namespace tns {
@@ -26,53 +27,6 @@ struct NormEst {
     double value = 0.0;
     int iterations = 0;
     int products = 0;
-};
-
-// ---------------------------------------------------------------------------
-// System.Random(seed): Knuth's subtractive generator (TAOCP vol. 2, 3.6), in
-// the form .NET keeps for seeded instances; Next(k) = (int)(Sample() * k).
-// ---------------------------------------------------------------------------
-class SeededRandom {
-public:
-    explicit SeededRandom(int seed)
-    {
-        const int mbig = INT_MAX;
-        int subtraction = (seed == INT_MIN) ? INT_MAX : std::abs(seed);
-        int mj = 161803398 - subtraction;
-        s_[55] = mj;
-        int mk = 1, ii = 0;
-        for (int i = 1; i < 55; i++) {
-            if ((ii += 21) >= 55) ii -= 55;
-            s_[ii] = mk;
-            mk = mj - mk;
-            if (mk < 0) mk += mbig;
-            mj = s_[ii];
-        }
-        for (int k = 1; k < 5; k++)
-            for (int i = 1; i < 56; i++) {
-                int n = i + 30;
-                if (n >= 55) n -= 55;
-                s_[i] -= s_[1 + n];
-                if (s_[i] < 0) s_[i] += mbig;
-            }
-        a_ = 0;
-        b_ = 21;
-    }
-    double sample()
-    {
-        if (++a_ >= 56) a_ = 1;
-        if (++b_ >= 56) b_ = 1;
-        int v = s_[a_] - s_[b_];
-        if (v == INT_MAX) v--;
-        if (v < 0) v += INT_MAX;
-        s_[a_] = v;
-        return v * (1.0 / INT_MAX);
-    }
-    int next(int max_value) { return (int)(sample() * max_value); }
-
-private:
-    int s_[56] = {0};
-    int a_ = 0, b_ = 0;
 };
 
 // NormEstimate.Estimate (NormEstimate.cs:104-274) as a resumable run.

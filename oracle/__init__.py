@@ -110,6 +110,7 @@ def _declare(L):
     L.or_zexpmv_op.restype = i
     L.or_dexpm_taylor_ld.argtypes = [i, p, i, p, i]
     L.or_zexpm_taylor_ld.argtypes = [i, p, i, p, i]
+    L.or_zexpmv_taylor_ld.argtypes = [i, p, i, d, i, p, i, p, i]
     L.or_cabs.argtypes = [Cplx]
     L.or_cabs.restype = d
     L.or_cdiv.argtypes = [Cplx, Cplx]
@@ -366,6 +367,16 @@ def expm_taylor_ld(a):
     fn = lib().or_zexpm_taylor_ld if cplx else lib().or_dexpm_taylor_ld
     fn(n, _ptr(a), max(1, n), _ptr(e), max(1, n))
     return e
+
+
+def expmv_taylor_ld(a, b, t):
+    """exp(tA) B with every operation in x87 extended precision."""
+    a = _f(a, np.complex128)
+    b = _f(np.asarray(b).reshape(a.shape[0], -1), np.complex128)
+    n, nb = b.shape
+    f = np.zeros((n, nb), dtype=np.complex128, order="F")
+    lib().or_zexpmv_taylor_ld(n, _ptr(a), max(1, n), float(t), nb, _ptr(b), max(1, n), _ptr(f), max(1, n))
+    return f
 
 
 def cabs(z):

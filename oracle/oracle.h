@@ -131,6 +131,10 @@ int or_zexpmv_op(int n, or_zapply apply, void* ctx, int nb, const ocplx* b, int 
  * (the MatrixExponentialTests.cs:454-480 oracle, at higher precision). */
 void or_dexpm_taylor_ld(int n, const double* a, int lda, double* e, int lde);
 void or_zexpm_taylor_ld(int n, const ocplx* a, int lda, ocplx* e, int lde);
+/* exp(tA)B in extended precision by scaled Taylor sums on the columns
+ * (n^2 work per term and column), for error statements at large n. */
+void or_zexpmv_taylor_ld(int n, const ocplx* a, int lda, double t, int nb, const ocplx* b, int ldb, ocplx* f,
+                         int ldf);
 
 #ifdef __cplusplus
 }

@@ -287,6 +287,78 @@ static void taylor_ld(int n, const ldc* a, ldc* e)
     free(b); free(term); free(next);
 }
 
+/* exp(tA)B in extended precision, column by column: tA/s with
+ * s = ceil(||tA||_1) (so ||tA/s||_1 <= 1), each of the s steps a Taylor sum
+ * of x <- sum_k (tA/s)^k x / k! until a term is below 1e-22 of the sum in
+ * max-norm. The yardstick for the error of exp(A)V and exp(tA)B at orders
+ * where the full Taylor squaring above is too slow: only n^2 operations per
+ * term and column. */
+void or_zexpmv_taylor_ld(int n, const ocplx* a, int lda, double t, int nb, const ocplx* b, int ldb, ocplx* f,
+                         int ldf)
+{
+    size_t nn = (size_t)n * n;
+    ldc* x = (ldc*)malloc(sizeof(ldc) * (nn ? nn : 1));
+    long double norm = 0.0L;
+    for (int j = 0; j < n; j++) {
+        long double s = 0.0L;
+        for (int i = 0; i < n; i++) {
+            long double re = a[(size_t)j * lda + i].re, im = a[(size_t)j * lda + i].im;
+            s += sqrtl(re * re + im * im);
+        }
+        if (s > norm) norm = s;
+    }
+    norm *= fabsl((long double)t);
+    int steps = (int)ceill(norm);
+    if (steps < 1) steps = 1;
+    const long double f_scale = (long double)t / steps;
+    for (size_t i = 0; i < nn; i++) {
+        x[i].re = a[i % n + (i / n) * (size_t)lda].re * f_scale;
+        x[i].im = a[i % n + (i / n) * (size_t)lda].im * f_scale;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < nb; c++) {
+        ldc* v = (ldc*)malloc(sizeof(ldc) * (n ? n : 1));
+        ldc* term = (ldc*)malloc(sizeof(ldc) * (n ? n : 1));
+        ldc* next = (ldc*)malloc(sizeof(ldc) * (n ? n : 1));
+        for (int i = 0; i < n; i++) {
+            v[i].re = b[(size_t)c * ldb + i].re;
+            v[i].im = b[(size_t)c * ldb + i].im;
+        }
+        for (int q = 0; q < steps; q++) {
+            memcpy(term, v, sizeof(ldc) * n);
+            for (int k = 1; k < 200; k++) {
+                for (int i = 0; i < n; i++) { next[i].re = 0.0L; next[i].im = 0.0L; }
+                for (int p = 0; p < n; p++) {
+                    const ldc y = term[p];
+                    const ldc* col = x + (size_t)p * n;
+                    for (int i = 0; i < n; i++) {
+                        next[i].re += col[i].re * y.re - col[i].im * y.im;
+                        next[i].im += col[i].re * y.im + col[i].im * y.re;
+                    }
+                }
+                long double tmax = 0.0L, smax = 0.0L;
+                for (int i = 0; i < n; i++) {
+                    term[i].re = next[i].re / k;
+                    term[i].im = next[i].im / k;
+                    v[i].re += term[i].re;
+                    v[i].im += term[i].im;
+                    long double tm = fabsl(term[i].re) + fabsl(term[i].im);
+                    long double sm = fabsl(v[i].re) + fabsl(v[i].im);
+                    if (tm > tmax) tmax = tm;
+                    if (sm > smax) smax = sm;
+                }
+                if (tmax <= 1e-22L * smax) break;
+            }
+        }
+        for (int i = 0; i < n; i++) {
+            f[(size_t)c * ldf + i].re = (double)v[i].re;
+            f[(size_t)c * ldf + i].im = (double)v[i].im;
+        }
+        free(v); free(term); free(next);
+    }
+    free(x);
+}
+
 void or_dexpm_taylor_ld(int n, const double* a, int lda, double* e, int lde)
 {
     size_t nn = (size_t)n * n;

@@ -1,0 +1,200 @@
+"""Failure and boundary contracts of the C ABI on the device:
+
+  * a context destroyed before the call refuses every entry point with
+    TNS_EDISPOSED and leaves every output as it was (the Workspace's
+    ObjectDisposedException before any write: ComplexLuTests.cs:312-323,
+    GemmDispatch.cs:107);
+  * the device-memory ceiling (tns_context_set_limit, the analogue of
+    TensileLimits.MaxElements, TensileLimits.cs:59-65) refuses with
+    TNS_ENOMEM before the destination is touched, and the same call succeeds
+    once the ceiling is raised;
+  * host operands whose single column is longer than the 64 MiB page-locked
+    staging buffer travel in pieces;
+  * a device-resident ILinearOperator (LinearOperators.cs:37-75) drives the
+    matrix-free Expmv (MatrixExponentialAction.cs:203-230) with the oracle's
+    result and diagnostics.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import csharp_linearalgebra_amd as T
+import oracle as O
+from csharp_linearalgebra_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def addr(a):
+    return a.ctypes.data
+
+
+def test_a_destroyed_context_refuses_every_entry_point(ws):
+    lib = L.load()
+    w = T.Workspace(0)
+    h = w.handle
+    assert lib.tns_context_destroy(h) == L.TNS_OK
+    assert lib.tns_context_destroy(h) == L.TNS_OK  # idempotent
+    n = 8
+    r = np.random.default_rng(0)
+    mats = {k: np.asfortranarray(r.uniform(-1, 1, (n, n))) for k in "abcde"}
+    zmats = {k: np.asfortranarray(r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n))) for k in "abcde"}
+    fl = np.asfortranarray(r.uniform(-1, 1, (n, n)).astype(np.float32))
+    piv = np.arange(n, dtype=np.int32)
+    before = {k: v.copy() for k, v in mats.items()}
+    zbefore = {k: v.copy() for k, v in zmats.items()}
+    fbefore = fl.copy()
+    info, phases, ne = L.LuInfo(), L.LuPhases(), L.NormEst()
+    ed, evd, mi, out = L.ExpmDiag(), L.ExpmvDiag(), L.MixedInfo(), C.c_double(-7.0)
+    one, zero = L.Complex(1, 0), L.Complex(0, 0)
+    a, b, c, d, e = (addr(mats[k]) for k in "abcde")
+    za, zb, zc, zd, ze = (addr(zmats[k]) for k in "abcde")
+    cb = L.DAPPLY(lambda *args: 0)
+    calls = {
+        "tns_context_set_limit": (h, 1 << 20),
+        "tns_context_stream": (h, C.byref(C.c_void_p())),
+        "tns_synchronize": (h,),
+        "tns_dgemm": (h, n, n, n, 1.0, a, n, b, n, 0.0, c, n),
+        "tns_zgemm": (h, n, n, n, one, za, n, zb, n, zero, zc, n),
+        "tns_dgetrf": (h, n, n, a, n, 0, addr(piv), C.byref(info)),
+        "tns_zgetrf": (h, n, n, za, n, 0, addr(piv), C.byref(info)),
+        "tns_dgetrf_timed": (h, n, n, a, n, 0, addr(piv), C.byref(info), C.byref(phases)),
+        "tns_zgetrf_timed": (h, n, n, za, n, 0, addr(piv), C.byref(info), C.byref(phases)),
+        "tns_dgetrs": (h, 0, n, n, a, n, addr(piv), -1, b, n),
+        "tns_zgetrs": (h, 0, n, n, za, n, addr(piv), -1, zb, n),
+        "tns_dtrsm": (h, 1, n, n, a, n, b, n),
+        "tns_dpanel": (h, 0, n, 2, a, n, b, n, c, n),
+        "tns_zpanel": (h, 0, n, 2, za, n, zb, n, zc, n),
+        "tns_dnorm": (h, b"1", n, n, a, n, C.byref(out)),
+        "tns_znorm": (h, b"1", n, n, za, n, C.byref(out)),
+        "tns_dnormest_power": (h, n, a, n, 1, 2, 5, 1, C.byref(ne)),
+        "tns_znormest_power": (h, n, za, n, 1, 2, 5, 1, C.byref(ne)),
+        "tns_dgecon": (h, n, a, n, addr(piv), -1, 1.0, 2, C.byref(out)),
+        "tns_zgecon": (h, n, za, n, addr(piv), -1, 1.0, 2, C.byref(out)),
+        "tns_dexpm": (h, n, a, n, c, n, C.byref(ed)),
+        "tns_zexpm": (h, n, za, n, zc, n, C.byref(ed)),
+        "tns_dexpmv": (h, n, 2, a, n, b, n, 1.0, c, n, C.byref(evd)),
+        "tns_zexpmv": (h, n, 2, za, n, zb, n, 1.0, zc, n, C.byref(evd)),
+        "tns_dexpmv_op": (h, n, cb, None, 0, 2, b, n, 1.0, 1.0, c, n, C.byref(evd)),
+        "tns_zexpmv_op": (h, n, cb, None, 0, 2, zb, n, 1.0, 1.0, zc, n, C.byref(evd)),
+        "tns_dgesv_mixed": (h, n, 2, a, n, b, n, c, n, 32, 1e-15, 30, C.byref(mi)),
+        "tns_sgemm": (h, 32, n, n, n, 1.0, addr(fl), n, addr(fl), n, 0.0, addr(fl), n),
+        "tns_sgetrf": (h, 32, n, addr(fl), n, addr(piv), C.byref(C.c_int())),
+        "tns_sgetrs": (h, 32, n, 2, addr(fl), n, addr(piv), addr(fl), n),
+        "tns_dexpmv_grad": (h, n, 2, a, n, b, n, 1.0, c, n, d, n, e, n, C.byref(evd)),
+        "tns_zexpmv_grad": (h, n, 2, za, n, zb, n, 1.0, zc, n, zd, n, ze, n, C.byref(evd)),
+    }
+    for name, args in calls.items():
+        assert getattr(lib, name)(*args) == L.TNS_EDISPOSED, name
+    nn = np.array([n], dtype=np.int32)
+    pa, pb = (C.c_void_p * 1)(za), (C.c_void_p * 1)(zb)
+    assert lib.tns_zbatch_create(h, 1, addr(nn), C.cast(pa, C.c_void_p), addr(nn), C.cast(pb, C.c_void_p),
+                                 addr(nn), 2, 0, C.byref(C.c_void_p())) == L.TNS_EDISPOSED
+    for k in "abcde":
+        assert np.array_equal(mats[k], before[k], equal_nan=True), k
+        assert np.array_equal(zmats[k], zbefore[k], equal_nan=True), k
+    assert np.array_equal(fl, fbefore) and np.array_equal(piv, np.arange(n))
+    assert out.value == -7.0
+    assert b"destroyed" in lib.tns_last_error(h)
+    covered = set(calls) | {"tns_zbatch_create", "tns_context_create", "tns_context_destroy", "tns_last_error",
+                            "tns_version", "tns_gemm_set_config", "tns_gemm_config_count"}
+    batch_calls = {n_ for n_ in L.SIGNATURES if n_.startswith("tns_zbatch_")}
+    assert set(L.SIGNATURES) - covered - batch_calls == set()
+
+
+def test_a_batch_outlives_its_context_safely(ws):
+    import torch
+
+    lib = L.load()
+    w = T.Workspace(0)
+    n = 70
+    r = np.random.default_rng(2)
+    z = r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    ta = torch.from_numpy(np.ascontiguousarray(z.T)).cuda()
+    tb = torch.from_numpy(np.ascontiguousarray(np.ones((2, n), dtype=complex))).cuda()
+    torch.cuda.synchronize()
+    batch = w.batched_systems([T.DeviceMatrix.from_torch(ta)], [T.DeviceMatrix.from_torch(tb)], tune=0)
+    batch.solve()
+    lib.tns_context_destroy(w.handle)
+    with pytest.raises(T.ObjectDisposedError):
+        batch.solve()
+    with pytest.raises(T.ObjectDisposedError):
+        batch.reciprocal_condition()
+    with pytest.raises(T.ObjectDisposedError):
+        _ = batch.config
+    batch.close()  # frees only the handle: the context released the plan
+
+
+def test_the_memory_ceiling_refuses_before_touching_the_destination(ws):
+    lib = L.load()
+    w = T.Workspace(0)
+    n = 300
+    r = np.random.default_rng(4)
+    a = np.asfortranarray(r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n)))
+    sentinel = np.full((n, n), 3.25 + 1j, order="F")
+    w.set_limit(64 << 10)
+    with pytest.raises(T.AllocationLimitError):
+        w.expm(a, out=sentinel)
+    assert np.all(sentinel == 3.25 + 1j)
+    f = np.asfortranarray(r.uniform(-1, 1, (n, n)))
+    f0 = f.copy()
+    piv = np.full(n, -5, dtype=np.int32)
+    info = L.LuInfo()
+    rc = lib.tns_dgetrf(w.handle, n, n, addr(f), n, 0, addr(piv), C.byref(info))
+    assert rc == L.TNS_ENOMEM
+    assert np.array_equal(f, f0) and np.all(piv == -5)
+    w.set_limit(1 << 30)
+    e, _ = w.expm(a, out=sentinel)
+    ref, _ = O.expm(a)
+    assert np.abs(e - ref).max() <= 1e-12 * np.abs(ref).max()
+    w.dispose()
+
+
+def test_host_columns_longer_than_the_staging_buffer(ws):
+    m = 9_000_000  # 72 MB in one column, past the 64 MiB staging buffer
+    r = np.random.default_rng(8)
+    a = r.uniform(-1, 1, (m, 1))
+    c = np.full((m, 1), np.nan, order="F")
+    ws.multiply(a, np.array([[2.0]]), c)
+    assert np.array_equal(c, 2.0 * a)
+    z = np.asfortranarray(a[: m // 2] + 1j * a[m // 2:])  # 72 MB complex column
+    zc = np.zeros_like(z)
+    ws.multiply(z, np.array([[1j]]), zc)
+    assert np.array_equal(zc, 1j * z)
+
+
+def test_a_device_resident_operator_drives_the_matrix_free_action(ws):
+    import torch
+
+    n = 160
+    r = np.random.default_rng(9)
+    a = (r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))) * (3.0 / n)
+    b = r.standard_normal((n, 3)) + 1j * r.standard_normal((n, 3))
+    bound = float(np.abs(a).sum(0).max())
+    helper = T.Workspace(0)
+    da = T.DeviceMatrix.from_torch(torch.from_numpy(np.ascontiguousarray(a.T)).cuda())
+    torch.cuda.synchronize()
+    op = T.DenseDeviceOperator(da, helper)
+    f, diag = ws.expmv_device_operator(op, b, 0.7, bound)
+    ref, rdiag = O.expmv_op(n, lambda x: a @ x, b, 0.7, bound, complex_=True)
+    assert (diag.degree, diag.scaling, diag.applications, diag.early_exits) == rdiag
+    assert np.abs(f - ref).max() <= 1e-12 * np.abs(ref).max()
+    # B and F in device memory too.
+    db = T.DeviceMatrix.from_torch(torch.from_numpy(np.ascontiguousarray(b.T)).cuda())
+    df = T.DeviceMatrix.from_torch(torch.zeros((3, n), dtype=torch.complex128, device="cuda"))
+    torch.cuda.synchronize()  # torch's stream, then the library's
+    ws.expmv_device_operator(op, db, 0.7, bound, out=df)
+    ws.synchronize()
+    assert np.abs(df.owner.cpu().numpy().T - ref).max() <= 1e-12 * np.abs(ref).max()
+
+    class Failing(T.DeviceLinearOperator):
+        order, complex_ = n, True
+
+        def apply_device(self, x, y, stream):
+            raise KeyError("boom")
+
+    with pytest.raises(T.OperatorError) as err:
+        ws.expmv_device_operator(Failing(), b, 0.7, bound)
+    assert isinstance(err.value.__cause__, KeyError)
+    helper.dispose()

@@ -1,0 +1,278 @@
+"""The device paths the benchmark's configurations reach, at their own sizes,
+against the oracle (the CPU restatement of the reference's algorithms):
+
+  * config 2: complex Expm and Expmv of order 1000 against an
+    extended-precision reference (tests/golden/expm_n1000_extended.npz), with
+    the bar "within 10x the reference's own error": the oracle's error against
+    the same reference, measured in the same test;
+  * LU solves with few right-hand sides at orders >= 2048 (the chip-wide
+    triangular solves), tall panels (4097..16384 rows: panels spread over
+    several workgroups), the outer-blocked FP64 LU at 8192 and the complex LU
+    with 32-column panels inside 64-column steps, with pivots equal to the
+    oracle's;
+  * config 3 at n = 4096: the mixed-precision solve at every condition number
+    of the benchmark, FP32 and BF16, fallback included, by backward error;
+  * config 5 at n = 256: the Expmv gradients against the 2n x 2n block
+    exponential and a central difference at h = 1e-6;
+  * the estimator's contracts from the reference's own tests
+    (NormEstimateTests.cs:102-118, ComplexNormEstimateTests.cs:103-154) on
+    the device, with the reference's matrices rebuilt from the System.Random
+    restatement.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import csharp_linearalgebra_amd as T
+import oracle as O
+from csharp_linearalgebra_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(float).eps
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def to_device(x):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(np.asfortranarray(x).T)).cuda()
+    # The library runs on its own stream: the data must be there first.
+    torch.cuda.synchronize()
+    return T.DeviceMatrix.from_torch(t)
+
+
+def from_device(d):
+    return np.asfortranarray(d.owner.cpu().numpy().T)
+
+
+def one_norm_rel(x, ref):
+    return float(np.abs(x - ref).sum(0).max() / np.abs(ref).sum(0).max())
+
+
+def berr(a, x, b):
+    r = b - a @ x
+    den = np.abs(a).sum(1).max() * np.abs(x).max(0) + np.abs(b).max(0)
+    return float((np.abs(r).max(0) / den).max())
+
+
+# ---- config 2 against the extended-precision reference ------------------------------
+@pytest.fixture(scope="module")
+def config2():
+    d = np.load(os.path.join(HERE, "golden", "expm_n1000_extended.npz"))
+    a, b = W.expm_operator(1000, 50.0, 8, seed=0)
+    check = [a.sum(), (a * a).sum(), b.sum(), np.abs(a).sum(0).max()]
+    assert np.array_equal(d["checksum"], np.array(check)), "the operator generator changed; rerun make_extended.py"
+    return a, b, d["expm_v"], d["expmv_b"]
+
+
+def test_config2_expm_within_ten_times_the_reference_error(ws, config2):
+    a, b, ref_v, _ = config2
+    e, diag = ws.expm(a)
+    eo, do = O.expm(a)
+    assert (diag.degree, diag.squarings) == do
+    dev = one_norm_rel(e @ b[:, :4], ref_v)
+    own = one_norm_rel(eo @ b[:, :4], ref_v)
+    assert dev <= 10 * own, (dev, own)
+    assert dev <= 1e-12
+
+
+def test_config2_expmv_within_ten_times_the_reference_error(ws, config2):
+    a, b, _, ref_b = config2
+    f, diag = ws.expmv(a, b, 0.5)
+    fo, do = O.expmv(a, b, 0.5)
+    assert (diag.degree, diag.scaling, diag.applications) == do[:3]
+    dev = one_norm_rel(f, ref_b)
+    own = one_norm_rel(fo, ref_b)
+    assert dev <= 10 * own, (dev, own)
+    assert dev <= 1e-13
+
+
+# ---- LU at the sizes of configs 1, 3 and 4 --------------------------------------------
+@pytest.mark.parametrize("n,nrhs", [(2100, 1), (2100, 5), (4096, 1), (4096, 32), (8192, 1), (8192, 32)])
+def test_dgetrs_with_few_rhs_at_large_order(ws, n, nrhs):
+    r = np.random.default_rng(n + nrhs)
+    a = r.uniform(-1, 1, (n, n))
+    b = r.uniform(-1, 1, (n, nrhs))
+    d = to_device(a)
+    piv, sing = ws.factor_lu_device(d)
+    assert sing == -1
+    f = from_device(d)
+    lu = T.LuDecomposition(ws, f, piv, sing, 0.0, 1.0, 0.0)
+    x = lu.solve(b)
+    ref = O.getrs(O.LU(f, piv, sing, 0.0, 0.0, 0.0), b)
+    assert berr(a, x, b) <= n * EPS
+    assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max()
+    xt = lu.solve_adjoint(b)
+    assert berr(a.T, xt, b) <= n * EPS
+
+
+@pytest.mark.parametrize("n", [2100, 4096])
+def test_zgetrs_with_few_rhs_at_large_order(ws, n):
+    r = np.random.default_rng(n)
+    a = r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n))
+    b = r.uniform(-1, 1, (n, 4)) + 1j * r.uniform(-1, 1, (n, 4))
+    lu = ws.factor_lu(a)
+    x = lu.solve(b)
+    assert berr(a, x, b) <= n * EPS
+    assert berr(a.conj().T, lu.solve_adjoint(b), b) <= n * EPS
+
+
+@pytest.mark.parametrize("kind", range(6))
+def test_triangular_solves_at_large_order(ws, kind):
+    n, nrhs = 2100, 5
+    r = np.random.default_rng(kind)
+    a = r.uniform(-1, 1, (n, n)) / n + np.eye(n) * (1.0 + r.uniform(0, 1, n))
+    b = r.uniform(-1, 1, (n, nrhs))
+    x = ws.solve_triangular(kind, a, b)
+    ref = O.trsv(kind, a, b)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("m,n", [(4500, 4500), (6000, 400), (16384, 64), (8192, 8192)])
+def test_dgetrf_tall_panels_pivots_equal_the_oracle(ws, m, n):
+    r = np.random.default_rng(m + n)
+    a = r.uniform(-1, 1, (m, n))
+    d = to_device(a)
+    piv, sing = ws.factor_lu_device(d)
+    ref = O.getrf(a)
+    assert np.array_equal(piv, ref.pivots)
+    f = from_device(d)
+    k = min(m, n)
+    assert np.max(np.abs(f - ref.factors)) <= 1e-9 * max(1.0, np.max(np.abs(ref.factors)))
+    if m == n:
+        x = T.LuDecomposition(ws, f, piv, sing, 0.0, 1.0, 0.0).solve(np.ones(n))
+        assert berr(a, x.reshape(-1, 1), np.ones((n, 1))) <= n * EPS
+    assert sing == -1 and k == len(piv)
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_zgetrf_pivots_equal_the_oracle(ws, n):
+    r = np.random.default_rng(n + 1)
+    a = r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n))
+    lu = ws.factor_lu(a)
+    ref = O.getrf(a)
+    assert np.array_equal(lu.pivots, ref.pivots)
+    assert np.max(np.abs(lu.factors - ref.factors)) <= 1e-9 * max(1.0, np.max(np.abs(ref.factors)))
+
+
+def test_lu_phase_split(ws):
+    n = 2048
+    a = np.random.default_rng(5).uniform(-1, 1, (n, n))
+    tm = T.LuPhaseTimings()
+    timed = ws.factor_lu(a, timings=tm)
+    plain = ws.factor_lu(a)
+    assert np.array_equal(timed.factors, plain.factors) and np.array_equal(timed.pivots, plain.pivots)
+    assert tm.block_steps == n // 64
+    parts = [tm.panel, tm.swaps, tm.triangular, tm.gemm]
+    assert all(p > 0 for p in parts) and sum(parts) <= tm.total * (1 + 1e-6)
+    again = T.LuPhaseTimings()
+    ws.factor_lu(a, timings=again)
+    first = tm.total
+    tm.add(again)
+    assert tm.block_steps == 2 * (n // 64) and tm.total == pytest.approx(first + again.total)
+    z = a + 1j * a[::-1]
+    tz = T.LuPhaseTimings()
+    ws.factor_lu(z, timings=tz)
+    assert tz.block_steps == n // 32 and tz.panel > 0
+
+
+# ---- config 3 at n = 4096 -----------------------------------------------------------
+@pytest.mark.parametrize("kappa", [1e2, 1e6, 1e10])
+@pytest.mark.parametrize("precision", [32, 16])
+def test_config3_n4096_reaches_fp64_backward_error(ws, kappa, precision):
+    a, b = W.conditioned_device(4096, kappa, 32, seed=0)
+    da = T.DeviceMatrix.from_torch(a.T.contiguous())
+    db = T.DeviceMatrix.from_torch(b.T.contiguous())
+    import torch
+
+    x = torch.zeros_like(b.T).contiguous()
+    dx = T.DeviceMatrix.from_torch(x)
+    torch.cuda.synchronize()  # A and B are built on torch's stream; the library runs on its own
+    _, info = ws.solve_mixed(da, db, precision=precision, out=dx)
+    ws.synchronize()
+    assert info.backward_error <= 1e-15, info
+    ah, bh, xh = a.cpu().numpy(), b.cpu().numpy(), x.T.cpu().numpy()
+    assert berr(ah, xh, bh) <= 2e-15
+    if kappa >= 1e10:
+        assert info.fell_back, info  # kappa * u_fp32 >> 1: refinement cannot contract
+    if kappa <= 1e2:
+        assert not info.fell_back, info
+
+
+# ---- config 5 at n = 256 --------------------------------------------------------------
+def _loss(ws, a, b, c, y, t):
+    f, _ = ws.expmv(a, b, t)
+    r = c @ f - y
+    return float(np.sum(np.abs(r) ** 2)), f
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_config5_n256_gradients(ws, complex_):
+    n, t = 256, 1.0
+    a, b, c, y = W.adjoint_problem(n, complex_, target=20.0, seed=256)
+    _, f = _loss(ws, a, b, c, y, t)
+    g = 2.0 * c.conj().T @ (c @ f - y)
+    ga, gb = ws.expmv_backward(a, b, t, g)
+    ah = a.conj().T
+    blk = np.zeros((2 * n, 2 * n), dtype=np.result_type(a, g))
+    blk[:n, :n] = t * ah
+    blk[n:, n:] = t * ah
+    blk[:n, n:] = g @ b.conj().T
+    e, _ = O.expm(blk)
+    ra, rb = t * e[:n, n:], e[:n, :n] @ g
+    assert np.abs(ga - ra).max() <= 1e-10 * np.abs(ra).max()
+    assert np.abs(gb - rb).max() <= 1e-10 * np.abs(rb).max()
+    # A central difference along one direction (h = 1e-6).
+    r = np.random.default_rng(7)
+    d = r.standard_normal(a.shape) + (1j * r.standard_normal(a.shape) if complex_ else 0)
+    d /= np.linalg.norm(d)
+    h = 1e-6
+    lp, _ = _loss(ws, a + h * d, b, c, y, t)
+    lm, _ = _loss(ws, a - h * d, b, c, y, t)
+    fd = (lp - lm) / (2 * h)
+    an = float(np.real(np.sum(np.conj(ga) * d)))
+    assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (fd, an)
+
+
+# ---- the estimator's contracts from the reference's tests -----------------------------
+def _dotnet_uniform(n, seed, shift=0.5):
+    """NormEstimateTests.Random(n, seed): System.Random(seed).NextDouble() - 0.5, column by column."""
+    rnd = O.Random(seed)
+    a = np.zeros((n, n), order="F")
+    for j in range(n):
+        for i in range(n):
+            a[i, j] = rnd.sample() - shift
+    return a
+
+
+def test_different_seeds_stay_lower_bounds(ws):
+    # NormEstimateTests.DifferentSeedsStayLowerBounds (NormEstimateTests.cs:102-118).
+    a = _dotnet_uniform(48, 101)
+    truth = np.abs(a).sum(0).max()
+    for seed in range(25):
+        got = ws.estimate_one_norm(a, columns=2, max_iterations=5, seed=seed)
+        ref = O.normest_power(a, 1, 2, 5, seed)
+        assert got.value <= truth * (1 + 1e-12), seed
+        assert got.value >= truth * 0.4, seed
+        assert got.value == pytest.approx(ref[0], rel=1e-12) and (got.iterations, got.products) == ref[1:]
+
+
+def test_a_unit_phase_on_the_rows_changes_nothing(ws):
+    # ComplexNormEstimateTests.AUnitPhaseOnTheRowsChangesNothing (ComplexNormEstimateTests.cs:133-154),
+    # with ComplexLuTests.Random's matrices (NextDouble() * 2 - 1 per part).
+    n = 30
+    for seed in range(6):
+        rnd = O.Random(seed)
+        a = np.zeros((n, n), dtype=complex, order="F")
+        for j in range(n):
+            for i in range(n):
+                re = rnd.sample() * 2.0 - 1.0
+                a[i, j] = re + 1j * (rnd.sample() * 2.0 - 1.0)
+        ph = O.Random(seed + 100)
+        phases = np.array([np.exp(1j * ph.sample() * 2 * np.pi) for _ in range(n)])
+        rotated = phases[:, None] * a
+        before = ws.estimate_one_norm(a, columns=2)
+        after = ws.estimate_one_norm(rotated, columns=2)
+        assert after.value == pytest.approx(before.value, rel=1e-12, abs=0)
+        assert after.iterations == before.iterations
