@@ -73,6 +73,7 @@ SIGNATURES = {
     "tns_dgetrf": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo)]),
     "tns_zgetrf": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo)]),
     "tns_dgetrf_timed": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo), C.POINTER(LuPhases)]),
+    "tns_dgetrf_noalloc": (_I, [_P, _I, _I, _P, _I, _I, _P, _P]),
     "tns_zgetrf_timed": (_I, [_P, _I, _I, _P, _I, _I, _P, C.POINTER(LuInfo), C.POINTER(LuPhases)]),
     "tns_dgetrs": (_I, [_P, _I, _I, _I, _P, _I, _P, _I, _P, _I]),
     "tns_zgetrs": (_I, [_P, _I, _I, _I, _P, _I, _P, _I, _P, _I]),

@@ -33,6 +33,36 @@ struct tns_batch {
     std::unique_ptr<tns::BatchPlan> plan;
 };
 
+// Device scratch of tns_dgetrf_noalloc: the row payloads and the panels'
+// exchange words and move lists, for callers that cannot allocate. One copy
+// per device, held until the process ends; every call on a device shares
+// it, so calls that overlap on one device race. Host code never uses this
+// name as an address: tns_context_create looks up each device's copy with
+// hipGetSymbolAddress and keeps that pointer.
+constexpr size_t kNoAllocScratchInts = size_t(1) << 18;  // 1 MiB
+static __device__ __align__(256) int scratch[kNoAllocScratchInts];
+
+namespace {
+std::mutex g_scratch_mutex;
+int* g_scratch_ptr[256] = {nullptr};
+
+// This device's copy of `scratch` (the device is current).
+int* device_scratch(int device)
+{
+    if (device < 0 || device >= 256) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[device] = static_cast<int*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 namespace tns {
 
 [[noreturn]] void fail(int status, const std::string& message) { throw Error{status, message}; }
@@ -660,6 +690,7 @@ TNS_API int tns_context_create(int device, tns_context** out)
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0) c.pool.set_limit(total_b / 4 * 3);
         else (void)hipGetLastError();
     }
+    if (e == hipSuccess) c.noalloc_scratch = device_scratch(device);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         if (c.d_scal) (void)hipFree(c.d_scal);
@@ -785,6 +816,29 @@ TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, 
                        tns_lu_info* info)
 {
     return guarded(ctx, [&](Context* c) { getrf_body<zd>(c, m, n, (zd*)a, lda, nb, pivots, info); });
+}
+
+TNS_API int tns_dgetrf_noalloc(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots, int* info)
+{
+    return guarded(ctx, [&](Context* c) {
+        check_shape(m, n, lda, "A");
+        if (nb > 64) fail(TNS_EINVAL, "block size above 64: the no-allocation LU takes nb <= 64 (or < 1 for the default)");
+        const size_t need = 8 + (size_t)(m > 0 ? m : 1) + tns::lu_work_ints(m, n, nb);
+        if (need > kNoAllocScratchInts)
+            fail(TNS_ERANGE, "this factorization needs " + std::to_string(need * sizeof(int)) +
+                                 " bytes of scratch; the no-allocation entry point holds " +
+                                 std::to_string(kNoAllocScratchInts * sizeof(int)));
+        if (!info || !tns::is_device_pointer(info)) fail(TNS_EINVAL, "info must be device memory");
+        const int limit = std::min(m, n);
+        if (limit > 0 && (!a || !pivots || !tns::is_device_pointer(a) || !tns::is_device_pointer(pivots)))
+            fail(TNS_EINVAL, "A and pivots must be device memory");
+        if (!c->noalloc_scratch) fail(TNS_EDEVICE, "the device scratch could not be located");
+        // Layout: the LU's work area (8-byte aligned, at the start), then
+        // the row payloads.
+        int* work = c->noalloc_scratch;
+        int* perm = work + (tns::lu_work_ints(m, n, nb) + 1) / 2 * 2;
+        tns::dgetrf(c, m, n, a, lda, nb, pivots, perm, info, nullptr, work);
+    });
 }
 
 TNS_API int tns_dgetrf_timed(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots,

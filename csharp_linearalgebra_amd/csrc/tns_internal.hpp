@@ -105,6 +105,8 @@ struct Context {
     size_t h_stage_bytes = 0;
     std::string last_error;
     std::mutex gate;            // the Workspace lock (Workspace.cs:136-139)
+    // This device's copy of the no-allocation LU's file-scope scratch.
+    int* noalloc_scratch = nullptr;
 };
 
 template <typename T>

@@ -137,6 +137,19 @@ TNS_API int tns_dgetrf(tns_context* ctx, int m, int n, double* a, int lda, int n
 TNS_API int tns_zgetrf(tns_context* ctx, int m, int n, tns_complex* a, int lda, int nb, int* pivots,
                        tns_lu_info* info);
 
+/* ---- LU that allocates nothing. No counterpart in the reference: it is
+ * tns_dgetrf (the same kernels, pivots and factors) for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). A, pivots (min(m,n) ints) and info (one
+ * int: the first zero-pivot column, -1, or -2 if a panel's cross-workgroup
+ * exchange timed out) are all device memory; the call only enqueues work on
+ * the context's stream and returns. Its scratch (1 MiB per device, held until
+ * the process ends) is shared by every call on a device, so calls that
+ * overlap on one device race. nb <= 64 (< 1: the default); a size whose
+ * scratch would exceed 1 MiB (about min(m,n) > 16384 at the default block
+ * size) is refused with TNS_ERANGE. */
+TNS_API int tns_dgetrf_noalloc(tns_context* ctx, int m, int n, double* a, int lda, int nb, int* pivots, int* info);
+
 /* ---- the same with the phase split: KernelEntry.FactorLu's optional
  * LuPhaseTimings (KernelEntry.cs:59-67, Lu.cs:92-164, hooked in at
  * Lu.cs:251-305). The device time of the panel factorizations, the row

@@ -60,6 +60,7 @@ def test_a_destroyed_context_refuses_every_entry_point(ws):
         "tns_dgetrf": (h, n, n, a, n, 0, addr(piv), C.byref(info)),
         "tns_zgetrf": (h, n, n, za, n, 0, addr(piv), C.byref(info)),
         "tns_dgetrf_timed": (h, n, n, a, n, 0, addr(piv), C.byref(info), C.byref(phases)),
+        "tns_dgetrf_noalloc": (h, n, n, a, n, 0, addr(piv), addr(piv)),
         "tns_zgetrf_timed": (h, n, n, za, n, 0, addr(piv), C.byref(info), C.byref(phases)),
         "tns_dgetrs": (h, 0, n, n, a, n, addr(piv), -1, b, n),
         "tns_zgetrs": (h, 0, n, n, za, n, addr(piv), -1, zb, n),
