@@ -6,6 +6,10 @@
 // the operator products it drives run on the device. The exponential's and
 // the action's branch decisions read back a handful of scalars; their
 // products, norms and element-wise passes are device kernels.
+#include <algorithm>
+#include <memory>
+#include <numeric>
+
 #include "algos.hpp"
 
 #include <climits>
@@ -335,6 +339,76 @@ NormEst normest(Context* ctx, const DevOp<T>& op, int columns, int max_iteration
     return run.result();
 }
 
+// NormEstimate runs over several powers of one dense operator, in lockstep:
+// the same probes, tests and results as one normest over
+// power_operator(p) per power (NormEstimate.cs:104-274, each with its own
+// generator from `seed`), but each round uploads every active run's panel
+// once, forms all the runs' products together and downloads once. Runs are
+// ordered by power, highest first, so the runs still multiplying at step k
+// of a round are a prefix of the stacked panel: step k is one panel product
+// over that prefix, and a run of power p finds its product in the buffer
+// step p wrote. All runs start wanting A x and alternate in step, so a round
+// is all products or all adjoint products.
+template <typename T>
+std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, const std::vector<int>& powers,
+                                    int columns, int max_iterations, int seed)
+{
+    const int R = (int)powers.size();
+    std::vector<NormEst> out(R);
+    if (R == 0 || n <= 0) return out;
+    std::vector<std::unique_ptr<NormEstRun<T>>> runs(R);
+    for (int r = 0; r < R; r++) runs[r] = std::make_unique<NormEstRun<T>>(n, columns, max_iterations, seed);
+    std::vector<int> ord(R);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return powers[x] > powers[y]; });
+    const int t = runs[0]->columns();
+    const size_t panel = (size_t)n * t;
+    DevBuf<T> dx(ctx, panel * R), dy0(ctx, panel * R), dy1(ctx, panel * R);
+    std::vector<T> hx(panel * R), hy0(panel * R), hy1(panel * R);
+    std::vector<int> act;
+    while (true) {
+        act.clear();
+        for (int r : ord)
+            if (runs[r]->want() != 0) act.push_back(r);
+        if (act.empty()) break;
+        const int kind = runs[act[0]]->want();
+        for (int r : act)
+            if (runs[r]->want() != kind) fail(TNS_ESTATE, "norm estimates out of step");
+        const int A = (int)act.size();
+        for (int i = 0; i < A; i++)
+            std::copy(runs[act[i]]->input().begin(), runs[act[i]]->input().end(), hx.begin() + panel * i);
+        TNS_HIP(hipMemcpyAsync(dx.get(), hx.data(), sizeof(T) * panel * A, hipMemcpyHostToDevice, ctx->stream));
+        const T* src = dx.get();
+        T* bufs[2] = {dy0.get(), dy1.get()};
+        for (int k = 1;; k++) {
+            int w = 0;
+            while (w < A && powers[act[w]] >= k) w++;
+            if (w == 0) break;
+            T* dst = bufs[k & 1];
+            // In slices the narrow-panel kernels take (16 real, 8 complex
+            // columns): each reads A once at HBM speed, where one wide
+            // product would go to the GEMM with a handful of tiles.
+            constexpr int SLICE = sizeof(T) == 8 ? 16 : 8;
+            for (int c0 = 0; c0 < w * t; c0 += SLICE) {
+                const int cw = SLICE < w * t - c0 ? SLICE : w * t - c0;
+                panel_t<T>(ctx, kind == 1 ? 0 : 1, n, cw, a, lda, src + (size_t)c0 * n, n, dst + (size_t)c0 * n, n);
+            }
+            src = dst;
+        }
+        TNS_HIP(hipMemcpyAsync(hy0.data(), dy0.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
+        TNS_HIP(hipMemcpyAsync(hy1.data(), dy1.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
+        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < A; i++) {
+            NormEstRun<T>& run = *runs[act[i]];
+            const std::vector<T>& hy = (powers[act[i]] & 1) ? hy1 : hy0;
+            std::copy(hy.begin() + panel * i, hy.begin() + panel * (i + 1), run.output().begin());
+            run.resume();
+        }
+    }
+    for (int r = 0; r < R; r++) out[r] = runs[r]->result();
+    return out;
+}
+
 template <typename T>
 DevOp<T> power_operator(Context* ctx, int n, const T* a, int lda, int power)
 {
@@ -448,12 +522,6 @@ int ell(Context* ctx, int n, const T* a, int m)
     double alpha = apn / (anorm * abs_coef_recip(m));
     int v = ceil_int(std::log2(alpha / kUnitRoundoff) / (2 * m));
     return v > 0 ? v : 0;
-}
-
-template <typename T>
-double power_norm_estimate(Context* ctx, int n, const T* a, int p)
-{
-    return normest<T>(ctx, power_operator<T>(ctx, n, a, n, p), 2, 5, 20260915).value;
 }
 
 template <typename T>
@@ -575,7 +643,11 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
         degree = 5;
         low_degree<T>(ctx, kPade5, 5, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
     } else {
-        const double d8 = std::pow(power_norm_estimate<T>(ctx, n, a.p(), 8), 1.0 / 8.0);
+        // d8 and d10 in one lockstep run (d10 is read only past the degree 7
+        // and 9 tests, as the reference computes it; estimating it with d8
+        // changes no value, each estimate having its own probes).
+        const std::vector<NormEst> d810 = normest_powers<T>(ctx, n, a.p(), n, {8, 10}, 2, 5, 20260915);
+        const double d8 = std::pow(d810[0].value, 1.0 / 8.0);
         const double eta3 = std::fmax(d6, d8);
         if (eta3 < 9.504178996162932e-1 && ell<T>(ctx, n, a.p(), 7) == 0) {
             degree = 7;
@@ -586,7 +658,7 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
             product<T>(ctx, n, a4.p(), a4.p(), a8.p());
             low_degree<T>(ctx, kPade9, 9, n, a.p(), a2.p(), a4.p(), a6.p(), a8.p(), x.p());
         } else {
-            const double d10 = std::pow(power_norm_estimate<T>(ctx, n, a.p(), 10), 1.0 / 10.0);
+            const double d10 = std::pow(d810[1].value, 1.0 / 10.0);
             const double eta4 = std::fmax(d8, d10);
             const double eta5 = std::fmin(eta3, eta4);
             s = eta5 == 0.0 ? 0 : ceil_int(std::log2(eta5 / 4.25));
@@ -634,6 +706,26 @@ const double kThetas[] = {2.29e-16, 2.58e-8, 1.39e-5, 3.40e-4, 2.40e-3, 9.07e-3,
                           3.08,     3.31,    3.54,    4.7,     6.0,     7.2,     8.5,     9.9};
 constexpr int kMaxDegree = 55, kMaxPower = 8, kEll = 2, kCount = 35;
 
+// Condition 3.13 (MatrixExponentialAction.cs:404-410): the 1-norm alone
+// selects the parameters, no d(p) is needed.
+bool norm_bound_suffices(double scaled_norm, int columns)
+{
+    const double a = 2.0 * kEll * kMaxPower * (kMaxPower + 3);
+    const double b = kThetas[kCount - 1] / (columns * (double)kMaxDegree);
+    return scaled_norm <= a * b;
+}
+
+// d(p) = ||A^p||^(1/p) for p = 2 .. kMaxPower + 1, every one SelectParameters
+// reads when the 1-norm does not suffice, estimated in one lockstep run.
+template <typename T>
+void power_norms(Context* ctx, int n, const T* a, double* dcache)
+{
+    std::vector<int> powers;
+    for (int p = 2; p <= kMaxPower + 1; p++) powers.push_back(p);
+    const std::vector<NormEst> est = normest_powers<T>(ctx, n, a, n, powers, kEll, 5, 20260915);
+    for (size_t i = 0; i < powers.size(); i++) dcache[powers[i]] = std::pow(est[i].value, 1.0 / powers[i]);
+}
+
 // SelectParameters (MatrixExponentialAction.cs:347-398); alpha(p) returns
 // max(d(p), d(p+1)) or is empty for the norm-only selection.
 void select_parameters(double scaled_norm, const std::function<double(int)>* alpha, double scale, int columns,
@@ -649,12 +741,7 @@ void select_parameters(double scaled_norm, const std::function<double(int)>* alp
         best_degree = kDegrees[index];
         best_scaling = scaling;
     };
-    bool norm_only = alpha == nullptr;
-    if (!norm_only) {
-        double a = 2.0 * kEll * kMaxPower * (kMaxPower + 3);
-        double b = kThetas[kCount - 1] / (columns * (double)kMaxDegree);
-        norm_only = scaled_norm <= a * b;
-    }
+    bool norm_only = alpha == nullptr || norm_bound_suffices(scaled_norm, columns);
     if (norm_only) {
         for (int i = 0; i < kCount; i++) consider(i, scaled_norm);
     } else {
@@ -805,6 +892,7 @@ void expmv(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb
         return dcache[p];
     };
     std::function<double(int)> alpha = [&](int p) { return std::fmax(dpow(p), dpow(p + 1)); };
+    if (std::fabs(t) * anorm1 != 0.0 && !norm_bound_suffices(std::fabs(t) * anorm1, nb)) power_norms<T>(ctx, n, sp, dcache);
     DevOp<T> op = power_operator<T>(ctx, n, sp, n, 1);
     run<T>(ctx, op, sp, n, nb, b, ldb, t, mu, anorm1, &alpha, f, ldf, diag);
 }
@@ -952,6 +1040,7 @@ void expmv_grad(Context* ctx, int n, int nb, const T* a, int lda, const T* b, in
     // gradient keeps the first one, which adds nothing to F there but
     // carries the derivative (t conj(eta) G B^H) the zero-term form drops.
     int m = 1, s = 1;
+    if (nb > 0 && scale * anorm1 != 0.0 && !norm_bound_suffices(scale * anorm1, nb)) power_norms<T>(ctx, n, sp, dcache);
     if (nb > 0 && scale * anorm1 != 0.0) select_parameters(scale * anorm1, &alpha, scale, nb, m, s);
     if (diag) {
         diag->degree = m;

@@ -322,11 +322,23 @@ __global__ void shift_diag_kernel(int n, T* __restrict__ a, int lda, T delta)
 
 // Sequential diagonal sum, one thread: the reference's Trace order.
 template <typename T>
-__global__ void trace_kernel(int n, const T* __restrict__ a, int lda, T* __restrict__ out)
+__global__ void __launch_bounds__(1024) trace_kernel(int n, const T* __restrict__ a, int lda, T* __restrict__ out)
 {
+    // The diagonal is read by the whole workgroup a slab at a time, then
+    // summed by one thread in index order (the reference's sequential sum).
+    __shared__ T diag[1024];
     T s = s_zero(T{});
-    for (int i = 0; i < n; i++) s = s_add(s, a[i + (size_t)i * lda]);
-    out[0] = s;
+    for (int i0 = 0; i0 < n; i0 += 1024) {
+        const int i = i0 + (int)threadIdx.x;
+        if (i < n) diag[threadIdx.x] = a[i + (size_t)i * lda];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int cnt = n - i0 < 1024 ? n - i0 : 1024;
+            for (int q = 0; q < cnt; q++) s = s_add(s, diag[q]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s;
 }
 
 template <typename T>
@@ -631,7 +643,7 @@ template void shift_diag<zd>(Context*, int, zd*, int, zd);
 template <typename T>
 void trace(Context* ctx, int n, const T* a, int lda, T* d_out)
 {
-    hipLaunchKernelGGL((trace_kernel<T>), dim3(1), dim3(1), 0, ctx->stream, n, a, lda, d_out);
+    hipLaunchKernelGGL((trace_kernel<T>), dim3(1), dim3(1024), 0, ctx->stream, n, a, lda, d_out);
 }
 template void trace<double>(Context*, int, const double*, int, double*);
 template void trace<zd>(Context*, int, const zd*, int, zd*);

@@ -1238,8 +1238,13 @@ __device__ __forceinline__ zd readlane_t<zd>(zd v, int lane)
     return make_zd(readlane_t(v.x, lane), readlane_t(v.y, lane));
 }
 
-template <typename T, bool LOWER, int TR, bool UNIT>
-__global__ void __launch_bounds__(256)
+// NT threads per column and UL entries of A loaded per row before they are
+// applied: 512 and 16 (8 complex) from order 256 on, where the latency of
+// those loads, paid once per group of UL, sets the time (512: wavefront 0
+// holds a whole diagonal block in registers, 128 of them, which 1024
+// threads per workgroup would not leave it).
+template <typename T, bool LOWER, int TR, bool UNIT, int NT = 256, int UL = (sizeof(T) == 16 ? 4 : 8)>
+__global__ void __launch_bounds__(NT)
 trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb)
 {
     constexpr int BS = tri_block<T>();
@@ -1248,12 +1253,13 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
     __shared__ T part[BS];
     T* xg = B + (size_t)blockIdx.x * ldb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = NT / 64;
     auto at = [&](int i, int k) -> T {
         T v = TR ? A[k + (size_t)i * lda] : A[i + (size_t)k * lda];
         return TR == 2 ? s_conj(v) : v;
     };
     constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
-    for (int i = tid; i < n; i += 256) xs[i] = xg[i];
+    for (int i = tid; i < n; i += NT) xs[i] = xg[i];
     __syncthreads();
 
     const int nblk = (n + BS - 1) / BS;
@@ -1266,7 +1272,7 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
             // (k0 + i) of A per block row i.
             const int lo = FORWARD ? 0 : k0 + kb;
             const int hi = FORWARD ? k0 : n;
-            for (int i = wave; i < kb; i += 4) {
+            for (int i = wave; i < kb; i += NW) {
                 T s0 = s_zero(T{}), s1 = s_zero(T{});
                 int k = lo + lane;
 #pragma unroll 4
@@ -1306,11 +1312,11 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
             // Right-looking: push this block into the rows still to come, in
             // the reference's axpy order, two rows at a time, the next U
             // entries of each row loaded together before they are applied.
-            constexpr int U = sizeof(T) == 16 ? 4 : 8;
+            constexpr int U = UL;
             const int r0 = FORWARD ? k0 + kb : 0;
             const int r1 = FORWARD ? n : k0;
-            for (int i = r0 + tid; i < r1; i += 512) {
-                const int i2 = i + 256;
+            for (int i = r0 + tid; i < r1; i += 2 * NT) {
+                const int i2 = i + NT;
                 const bool two = i2 < r1;
                 T acc = xs[i];
                 T acc2 = load_if(xs + i2, two);
@@ -1340,7 +1346,7 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
             __syncthreads();
         }
     }
-    for (int i = tid; i < n; i += 256) xg[i] = xs[i];
+    for (int i = tid; i < n; i += NT) xg[i] = xs[i];
 }
 
 // One step of a few-right-hand-side solve op(A) X = B, A lower unit
@@ -1987,7 +1993,11 @@ void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb
     if (nrhs <= 32 || n <= 64) {
         const size_t xbytes = sizeof(T) * (size_t)n;
 
-        if (xbytes + sizeof(T) * tri_block<T>() <= (size_t)ctx->max_lds) {
+        if (xbytes + sizeof(T) * tri_block<T>() <= (size_t)ctx->max_lds && n >= 256) {
+            constexpr int UL = sizeof(T) == 16 ? 8 : 16;
+            hipLaunchKernelGGL((trsv_lds_kernel<T, LOWER, TR, UNIT, 512, UL>), dim3(nrhs), dim3(512), xbytes,
+                               ctx->stream, n, A, lda, B, ldb);
+        } else if (xbytes + sizeof(T) * tri_block<T>() <= (size_t)ctx->max_lds) {
             hipLaunchKernelGGL((trsv_lds_kernel<T, LOWER, TR, UNIT>), dim3(nrhs), dim3(256), xbytes, ctx->stream, n, A,
                                lda, B, ldb);
         } else {
