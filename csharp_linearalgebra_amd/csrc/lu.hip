@@ -438,25 +438,37 @@ __global__ void __launch_bounds__(NT) getf2_grouped_kernel(const PanelRec<T>* __
 // launch, no column chunks). Workgroup g holds panel rows [g*512, g*512+512)
 // in registers, one row per thread, across every panel column, and does the
 // work of getf2_reg_body for its rows with the same pivot rule, interchange
-// bookkeeping and update order. Per column each workgroup finds its best row
-// (DPP within a wavefront, LDS across wavefronts, one barrier) and, when
-// P > 1, joins an all-gather through device memory: it publishes its best
-// key (value and row position) and, speculatively, that row's values; one
-// wavefront polls the P keys, picks the winner by the same rule (largest
-// key, lowest row on ties) and fetches the winner's row into LDS for a
-// second barrier. Every published 64-bit word carries 32 bits of payload and
-// the column's sequence number (global column + 1) in its other half, so a
-// reader knows a word is current without any fence, and each relaxed atomic
-// word is written and read whole. Records alternate between two buffers by
-// column parity: a workgroup can be at most one column ahead of any other
-// (it needs every key of a column to leave it). The working workgroups are
-// blockIdx.x = 8g, which round-robin dispatch places on one XCD (one shared
-// L2); the others exit at once. Every wait gives up after SPREAD_SPIN polls
-// or when another workgroup has raised the abort word (xbuf[0]), so the
-// kernel always ends; the caller turns a raised abort word into an error.
-// P = 1 keeps everything in LDS. Rows are stored to their final positions
-// at the end; moved rows are appended to `list` (count in list[0], zero at
-// entry) by atomic reservation.
+// bookkeeping and update order. Per column:
+//  * each wavefront finds its best row by one argmax over order-preserving
+//    integer keys (DPP maximum of the high words; the low words, then the
+//    row positions, only on ties) and publishes it in LDS; one barrier;
+//  * every wavefront reduces the eight wavefront winners the same way, and
+//    the lane that holds the workgroup's best row writes that row to LDS;
+//    a second barrier;
+//  * when P > 1, the workgroups join an all-gather through device memory:
+//    each publishes its best key (integer key and row position) and,
+//    speculatively, that row's values; one wavefront polls the P keys, picks
+//    the winner by the same rule and fetches the winner's row into LDS for a
+//    third barrier.
+// Every published 64-bit word carries 32 bits of payload and the column's
+// sequence number (global column + 1) in its other half, so a reader knows
+// a word is current without any fence. Words are written whole by
+// workgroup-scope atomic stores, which go to the L2 of the writer's XCD, and
+// read whole by agent-scope atomic loads, which skip the reader's L1; the
+// working workgroups are blockIdx.x = 8g, which round-robin dispatch places
+// on one XCD, so every store and load meets in one L2 (a measured 0.27 us
+// one way, against 0.5 us for agent-scope stores, which write through to
+// memory). A word that does not arrive keeps its old sequence number, so a
+// placement that broke this could only make a wait give up, never return a
+// stale value. Records alternate between two buffers by column parity: a
+// workgroup can be at most one column ahead of any other (it needs every key
+// of a column to leave it). The others exit at once. Every wait gives up
+// after SPREAD_SPIN polls or when another workgroup has raised the abort
+// word (xbuf[0], written with agent scope), so the kernel always ends; the
+// caller turns a raised abort word into an error. P = 1 keeps everything in
+// LDS. Rows are stored to their final positions at the end; moved rows are
+// appended to `list` (count in list[0], zero at entry) by atomic
+// reservation.
 // ---------------------------------------------------------------------------
 // f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), each
 // call inlined: a loop whose index is a constant in every iteration.
@@ -489,7 +501,7 @@ constexpr int LU_OUTER_FROM = 8192;      // order from which getrf_impl takes ou
 constexpr int SPREAD_NT = 512;           // threads per workgroup, one panel row each
 constexpr int SPREAD_MAXP = 32;          // workgroups: panels of up to 16384 rows
 constexpr int SPREAD_ROW_WORDS = 128;    // exchange words of one row: 64 doubles' halves
-constexpr int SPREAD_REC = 3 + SPREAD_ROW_WORDS;  // key (value lo, hi; row) + row
+constexpr int SPREAD_REC = 3 + SPREAD_ROW_WORDS;  // key (lo, hi; row) + row
 constexpr int SPREAD_SPIN = 1 << 22;     // polls before a wait gives up
 // Exchange buffer length in 64-bit words: the abort word, then two parities
 // of SPREAD_MAXP records.
@@ -500,7 +512,7 @@ constexpr int SPREAD_LIST = 1 + 2 * (2 * 64);
 
 __device__ __forceinline__ void xstore(unsigned long long* p, unsigned seq, unsigned payload)
 {
-    __hip_atomic_store(p, ((unsigned long long)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, ((unsigned long long)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ unsigned long long xload(const unsigned long long* p)
@@ -528,47 +540,81 @@ __device__ __forceinline__ bool xwait(const unsigned long long* p, unsigned seq,
     return false;
 }
 
-// Largest key across the wavefront, ties to the lowest index, in every
-// lane: the maximum by DPP moves (as wave_best_dpp, on the key alone), then
-// the lanes holding it by one ballot; only a tie between lanes needs the
-// index comparison. *lane_of receives the winning lane. Keys are never NaN.
+// One DPP step of an unsigned maximum (minimum) across lanes; 0 (all ones)
+// is the identity, so lanes a row mask leaves out keep their value and the
+// compiler folds the move into the maximum.
 template <int CTRL, int ROWS = 0xF>
-__device__ __forceinline__ double dpp_max(double x)
+__device__ __forceinline__ unsigned dpp_umax(unsigned x)
 {
-    const long long b = __double_as_longlong(x);
-    const long long idv = __double_as_longlong(-2.0);
-    const int lo = __builtin_amdgcn_update_dpp((int)idv, (int)b, CTRL, ROWS, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(idv >> 32), (int)(b >> 32), CTRL, ROWS, 0xF, false);
-    return fmax(x, __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+    return max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ unsigned dpp_umin(unsigned x)
+{
+    return min(x, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)x, CTRL, ROWS, 0xF, false));
 }
 
-__device__ __forceinline__ void wave_best_fast(double& bv, int& bi, int& lane_of)
+// Maximum (minimum) over the wavefront as a uniform value: quad_perm
+// [1,0,3,2] and [2,3,0,1], row_ror 4 and 8 reduce each row of 16 lanes;
+// row_bcast 15 and 31 fold the rows into lane 63. ROW: only lanes 0..15
+// hold values (the others the identity), so the row reduction suffices.
+template <bool ROW>
+__device__ __forceinline__ unsigned wave_umax(unsigned x)
 {
-    double mx = bv;
-    mx = dpp_max<0xB1>(mx);
-    mx = dpp_max<0x4E>(mx);
-    mx = dpp_max<0x124>(mx);
-    mx = dpp_max<0x128>(mx);
-    mx = dpp_max<0x142, 0xA>(mx);
-    mx = dpp_max<0x143, 0xC>(mx);
-    {
-        const long long b = __double_as_longlong(mx);
-        const int lo = __builtin_amdgcn_readlane((int)b, 63);
-        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-        mx = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    x = dpp_umax<0xB1>(x);
+    x = dpp_umax<0x4E>(x);
+    x = dpp_umax<0x124>(x);
+    x = dpp_umax<0x128>(x);
+    if constexpr (ROW) return __builtin_amdgcn_readfirstlane(x);
+    x = dpp_umax<0x142, 0xA>(x);
+    x = dpp_umax<0x143, 0xC>(x);
+    return __builtin_amdgcn_readlane(x, 63);
+}
+template <bool ROW>
+__device__ __forceinline__ unsigned wave_umin(unsigned x)
+{
+    x = dpp_umin<0xB1>(x);
+    x = dpp_umin<0x4E>(x);
+    x = dpp_umin<0x124>(x);
+    x = dpp_umin<0x128>(x);
+    if constexpr (ROW) return __builtin_amdgcn_readfirstlane(x);
+    x = dpp_umin<0x142, 0xA>(x);
+    x = dpp_umin<0x143, 0xC>(x);
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
+// Order-preserving integer form of a pivot key: rows out of the search 0, a
+// NaN off the diagonal 1, a measure v >= 0 (or +inf) its bits + 2. Integer
+// order is then the order of pivot_key, ties included.
+__device__ __forceinline__ unsigned long long key_bits(double v, bool out)
+{
+    return out ? 0ull : (v < 0.0 ? 1ull : bits_of(v) + 2ull);
+}
+
+// The wavefront's largest key, ties to the lowest position: the high words
+// first, the low words only where high words tie, positions only where
+// whole keys tie. kmax and pmax receive the winner (pmax = INT_MAX when
+// every key is 0); returns its lane.
+template <bool ROW>
+__device__ __forceinline__ int wave_argmax_u(unsigned long long key, int pos, unsigned long long& kmax, int& pmax)
+{
+    const unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+    const unsigned mh = wave_umax<ROW>(hi);
+    bool c = hi == mh;
+    unsigned long long hit = __ballot(c);
+    if (__popcll(hit) != 1) {
+        const unsigned ml = wave_umax<ROW>(c ? lo : 0u);
+        c = c && lo == ml;
+        hit = __ballot(c);
+        if (__popcll(hit) != 1) {
+            const unsigned mp = wave_umin<ROW>(c ? (unsigned)pos : 0xFFFFFFFFu);
+            hit = __ballot(c && (unsigned)pos == mp);
+        }
     }
-    const unsigned long long hit = __ballot(bv == mx);
-    if (__popcll(hit) == 1) {
-        lane_of = __ffsll((long long)hit) - 1;
-        bi = __builtin_amdgcn_readlane(bi, lane_of);
-    } else {
-        double tv = bv == mx ? bv : -2.0;
-        int ti = bv == mx ? bi : INT_MAX;
-        wave_best_dpp(tv, ti);
-        lane_of = __ffsll((long long)__ballot(bv == mx && bi == ti)) - 1;
-        bi = ti;
-    }
-    bv = mx;
+    const int ln = __ffsll((long long)hit) - 1;
+    kmax = ((unsigned long long)mh << 32) | (unsigned)__builtin_amdgcn_readlane((int)lo, ln);
+    pmax = kmax == 0 ? INT_MAX : __builtin_amdgcn_readlane(pos, ln);
+    return ln;
 }
 
 // v[c] -= prow[c] * l for c in [C1, NB): prow (LDS) read in chunks of 32
@@ -618,11 +664,11 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
         if (blockIdx.x & 7) return;
         g = blockIdx.x >> 3;
     }
-    __shared__ double red_v[2][NW];
-    __shared__ int red_i[2][NW];
-    __shared__ T cand[2][NW][NB];  // each wavefront's best row
-    __shared__ T xrow[2][NB];      // the winning row, from the exchange
-    __shared__ int xp[2];          // its position
+    __shared__ unsigned long long red_k[2][NW];  // each wavefront's best key
+    __shared__ int red_i[2][NW];                 // and its row position
+    __shared__ T wrow[2][NB];                    // the workgroup's best row
+    __shared__ T xrow[2][NB];                    // the winning row, from the exchange
+    __shared__ int xp[2];                        // its position
     __shared__ int xabort;
     __shared__ int spiv[NB];
     __shared__ int moves;
@@ -650,9 +696,9 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
 
     // Columns in groups of GW: the group index G is a constant, the column
     // inside it (jj) a run-time index. Columns of later groups are all right
-    // of j, so the candidate row and the update reach them with constant
-    // indices and no test; only the current group's eight columns need a
-    // per-column selection. nx is the row's column j, for the pivot search.
+    // of j, so the pivot row and the update reach them with constant indices
+    // and no test; the current group's columns are reached through a switch
+    // on jj. nx is the row's column j, for the pivot search.
     auto group = [&](auto gc) __attribute__((always_inline)) {
         constexpr int G = decltype(gc)::value;
         constexpr int C0 = G * GW;
@@ -664,52 +710,51 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
                 break;
             }
             const int buf = j & 1;
-            double bv = done ? -2.0 : pivot_key(nx, pos, j);
-            int bi = done ? INT_MAX : pos;
-            int who;
-            wave_best_fast(bv, bi, who);
+            unsigned long long kw = 0ull;
+            int pw = INT_MAX;
+            if (__ballot(!done) != 0) wave_argmax_u<false>(key_bits(done ? 0.0 : pivot_key(nx, pos, j), done), pos, kw, pw);
             if (lane == 0) {
-                red_v[buf][wave] = bv;
-                red_i[buf][wave] = bi;
-            }
-            if (lane == who && !done) {
-#pragma unroll
-                for (int c = C0; c < NB; c++) cand[buf][wave][c] = v[c];
+                red_k[buf][wave] = kw;
+                red_i[buf][wave] = pw;
             }
             __syncthreads();
             // Workgroup best: lane q reads wavefront q's record.
-            double kv = lane < NW ? red_v[buf][lane < NW ? lane : 0] : -2.0;
-            int ki = lane < NW ? red_i[buf][lane < NW ? lane : 0] : INT_MAX;
-            int wq;
-            wave_best_fast(kv, ki, wq);
-            bv = kv;
-            bi = ki;
-            int p = bi;
-            const T* prow = cand[buf][wq];
+            unsigned long long kg;
+            int p;
+            wave_argmax_u<true>(lane < NW ? red_k[buf][lane < NW ? lane : 0] : 0ull,
+                                lane < NW ? red_i[buf][lane < NW ? lane : 0] : INT_MAX, kg, p);
+            // The lane holding that row writes it (from column C0 on).
+            if (!done && pos == p) {
+#pragma unroll
+                for (int c = C0; c < NB; c++) wrow[buf][c] = v[c];
+            }
+            __syncthreads();
+            const T* prow = wrow[buf];
             if (P > 1) {
                 const unsigned seq = (unsigned)(off + j + 1);
                 unsigned long long* recs = xbuf + 1 + (size_t)buf * SPREAD_MAXP * SPREAD_REC;
                 unsigned long long* mine = recs + (size_t)g * SPREAD_REC;
                 const unsigned* words = reinterpret_cast<const unsigned*>(prow);
                 // Only the columns from j on are read from a fetched row.
-                if (tid >= j * WPE && tid < SPREAD_ROW_WORDS) xstore(mine + 3 + tid, seq, words[tid]);
-                else if (tid >= SPREAD_ROW_WORDS && tid < SPREAD_ROW_WORDS + 3) {
+                if (tid >= j * WPE && tid < SPREAD_ROW_WORDS) {
+                    xstore(mine + 3 + tid, seq, words[tid]);
+                } else if (tid >= SPREAD_ROW_WORDS && tid < SPREAD_ROW_WORDS + 3) {
                     const int q = tid - SPREAD_ROW_WORDS;
-                    const unsigned long long kb = bits_of(bv);
-                    xstore(mine + q, seq, q == 0 ? (unsigned)kb : q == 1 ? (unsigned)(kb >> 32) : (unsigned)bi);
+                    xstore(mine + q, seq, q == 0 ? (unsigned)kg : q == 1 ? (unsigned)(kg >> 32) : (unsigned)p);
                 }
                 if (wave == 0) {
-                    double gv = -2.0;
+                    unsigned long long gk = 0ull;
                     int gi = INT_MAX;
                     bool ok = true;
                     if (lane < P) {
-                        unsigned kw[3];
-                        ok = xwait<3>(recs + (size_t)lane * SPREAD_REC, seq, abort_word, kw);
-                        gv = double_of(((unsigned long long)kw[1] << 32) | kw[0]);
-                        gi = (int)kw[2];
+                        unsigned kk[3];
+                        ok = xwait<3>(recs + (size_t)lane * SPREAD_REC, seq, abort_word, kk);
+                        gk = ((unsigned long long)kk[1] << 32) | kk[0];
+                        gi = (int)kk[2];
                     }
-                    int gw;
-                    wave_best_fast(gv, gi, gw);
+                    unsigned long long gkm;
+                    int gim;
+                    const int gw = wave_argmax_u<false>(gk, gi, gkm, gim);
                     unsigned rw[2] = {0u, 0u};
                     const bool need = 2 * lane + 1 >= j * WPE;
                     if (__all(ok) && need)
@@ -717,7 +762,7 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
                     if (need)
                         reinterpret_cast<unsigned long long*>(xrow[buf])[lane] =
                             ((unsigned long long)rw[1] << 32) | rw[0];
-                    if (lane == 0) xp[buf] = gi;
+                    if (lane == 0) xp[buf] = gim;
                     if (!__all(ok) && lane == 0) {
                         xabort = 1;
                         __hip_atomic_store(xbuf, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -765,17 +810,27 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
                     const bool nz = lane < NB && lane < w && lane > j && !s_iszero(prow[lane < NB ? lane : 0]);
                     keep = __ballot(nz);
                 }
-                const bool plain = keep == ~0ull;
+                if (keep == ~0ull) {
+                    // The group's columns right of j by a switch on jj (the
+                    // empty asm keeps the eight cases apart, so v stays in
+                    // registers), then the later groups.
+                    auto cols = [&](auto jc) __attribute__((always_inline)) {
+                        constexpr int JJ = decltype(jc)::value;
 #pragma unroll
-                for (int q = 0; q < GW; q++) {
-                    const int c = C0 + q;
-                    const T nv = s_add(v[c], s_mul(s_neg(prow[c]), l));
-                    const bool upd = q > jj && (plain || ((keep >> c) & 1ull) != 0);
-                    v[c] = s_sel(q == jj, l, s_sel(upd, nv, v[c]));
-                }
-                if (plain) {
+                        for (int q = JJ + 1; q < GW; q++) v[C0 + q] = s_add(v[C0 + q], s_mul(s_neg(prow[C0 + q]), l));
+                        v[C0 + JJ] = l;
+                        asm volatile("; column %0 of the group" ::"n"(JJ));
+                    };
+                    dispatch<0, GW>(jj, cols);
                     update_tail<C0 + GW>(v, prow, l);
                 } else {
+#pragma unroll
+                    for (int q = 0; q < GW; q++) {
+                        const int c = C0 + q;
+                        const T nv = s_add(v[c], s_mul(s_neg(prow[c]), l));
+                        const bool upd = q > jj && ((keep >> c) & 1ull) != 0;
+                        v[c] = s_sel(q == jj, l, s_sel(upd, nv, v[c]));
+                    }
 #pragma unroll
                     for (int c = C0 + GW; c < NB; c++)
                         v[c] = s_sel(((keep >> c) & 1ull) != 0, s_add(v[c], s_mul(s_neg(prow[c]), l)), v[c]);
