@@ -676,6 +676,10 @@ TNS_API int tns_context_create(int device, tns_context** out)
     c.device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_panel, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[1], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c.d_scal, 64 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c.d_int, 64 * sizeof(int));
     if (e == hipSuccess) {
@@ -696,6 +700,9 @@ TNS_API int tns_context_create(int device, tns_context** out)
         if (c.d_scal) (void)hipFree(c.d_scal);
         if (c.d_int) (void)hipFree(c.d_int);
         if (c.stream) (void)hipStreamDestroy(c.stream);
+        if (c.side) (void)hipStreamDestroy(c.side);
+        for (hipEvent_t ev : {c.ev_panel, c.ev_rest[0], c.ev_rest[1]})
+            if (ev) (void)hipEventDestroy(ev);
         delete ctx;
         return e == hipErrorOutOfMemory ? TNS_ENOMEM : TNS_EDEVICE;
     }
@@ -715,6 +722,7 @@ TNS_API int tns_context_destroy(tns_context* ctx)
     tns::Context& c = ctx->impl;
     (void)hipSetDevice(c.device);
     (void)hipStreamSynchronize(c.stream);
+    (void)hipStreamSynchronize(c.side);
     // Batches still alive give their blocks and graphs back now; their
     // handles stay valid and refuse further calls with TNS_EDISPOSED.
     for (tns_batch* b : ctx->batches) b->plan.reset();
@@ -726,6 +734,10 @@ TNS_API int tns_context_destroy(tns_context* ctx)
     (void)hipFree(c.d_scal);
     (void)hipFree(c.d_int);
     (void)hipStreamDestroy(c.stream);
+    (void)hipStreamDestroy(c.side);
+    for (hipEvent_t ev : {c.ev_panel, c.ev_rest[0], c.ev_rest[1]}) (void)hipEventDestroy(ev);
+    c.side = nullptr;
+    c.ev_panel = c.ev_rest[0] = c.ev_rest[1] = nullptr;
     c.d_scal = nullptr;
     c.d_int = nullptr;
     c.stream = nullptr;

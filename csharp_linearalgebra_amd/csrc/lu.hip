@@ -2195,6 +2195,102 @@ void panel_spread(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, 
     }
 }
 
+// Points ctx->stream at s for the guard's life: the helpers launch on
+// ctx->stream.
+struct OnStream {
+    Context* ctx;
+    hipStream_t saved;
+    OnStream(Context* c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
+    ~OnStream() { ctx->stream = saved; }
+    OnStream(const OnStream&) = delete;
+    OnStream& operator=(const OnStream&) = delete;
+};
+
+// getrf_impl's blocked loop with look-ahead (spread panels, panel width ib
+// inside outer steps of ob columns). Step k:
+//   on ctx->stream: factor panel k; once the side stream's step k - 1 is
+//     done, move its interchanges into the outer step's columns left of it
+//     and into the next panel's columns; give the next panel's columns
+//     step k's update (the U12 solve and the trailing product of depth ib;
+//     at the outer step's last panel, the outer step's solve and product of
+//     depth ob) - the next panel can start;
+//   on ctx->side, after those moves: move the interchanges into every other
+//     column (earlier outer steps, and past the next panel) and give the
+//     columns past the next panel their update.
+// Every column takes the same updates in the same order as in the loop
+// without look-ahead, and no update reads rows that another stream is
+// moving: the interchanges of step k reach the outer step's L columns only
+// after the side stream's step k - 1 (which reads them) is done, and the
+// side stream's outer update starts after them. So the pivots are the same
+// and every element sees the same arithmetic. The trailing updates of step
+// k run while panel k + 1 is factored.
+template <typename T>
+void getrf_lookahead(Context* ctx, int m, int n, T* a, int lda, int ib, int ob, int* d_piv, int* d_perm,
+                     int* d_info, int* lists, int lstride, unsigned long long* xbuf)
+{
+    constexpr int NB = spread_nb<T>();
+    if constexpr (NB > 0) {
+        const int limit = m < n ? m : n;
+        hipStream_t side = ctx->side;
+        int step = 0;
+        for (int jo = 0; jo < limit; jo += ob) {
+            const int wo = ob < limit - jo ? ob : limit - jo;
+            const int end = jo + wo < n ? jo + wo : n;  // columns the inner steps update
+            T* po = a + jo + (size_t)jo * lda;
+            for (int jb = jo; jb < jo + wo; jb += ib, ++step) {
+                const int w = ib < jo + wo - jb ? ib : jo + wo - jb;
+                T* pa = a + jb + (size_t)jb * lda;
+                int* list = lists + (size_t)step * lstride;
+                const int mp = m - jb;
+                const int P = (mp + SPREAD_NT - 1) / SPREAD_NT;
+                hipLaunchKernelGGL((getf2_spread_kernel<T, NB>), dim3(P > 1 ? 8 * P : 1), dim3(SPREAD_NT), 0,
+                                   ctx->stream, mp, w, pa, lda, d_piv + jb, jb, d_perm + jb, list, P, xbuf, d_info);
+                const int moves = 2 * (mp < w ? mp : w);
+                const bool last = jb + w >= jo + wo;  // the outer step's last panel
+                // The next panel's columns [c1, c2).
+                const int c1 = jb + w;
+                int c2 = c1;
+                if (c1 < limit) {
+                    const int room = last ? limit - c1 : jo + wo - c1;
+                    c2 = c1 + (ib < room ? ib : room);
+                }
+                if (step > 0) TNS_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_rest[(step - 1) & 1], 0));
+                // Columns [jo, jb) and [c1, c2), rows from jb.
+                apply_moves<T>(ctx, a + jb + (size_t)jo * lda, lda, jb - jo, c1 - jo, c2 - jo, list, moves);
+                TNS_HIP(hipEventRecord(ctx->ev_panel, ctx->stream));
+                // Step k's update of the columns [lo, hi): its own solve and
+                // product inside the outer step, the outer step's past it.
+                auto update = [&](int lo, int hi) {
+                    if (hi <= lo) return;
+                    if (!last) {
+                        trsm_llnu<T>(ctx, w, hi - lo, pa, lda, a + jb + (size_t)lo * lda, lda);
+                        if (jb + w < m)
+                            Ops<T>::gemm(ctx, 0, m - jb - w, hi - lo, w, Ops<T>::neg_one(), pa + w, lda,
+                                         a + jb + (size_t)lo * lda, lda, Ops<T>::one(), a + jb + w + (size_t)lo * lda,
+                                         lda);
+                    } else {
+                        trsm_llnu<T>(ctx, wo, hi - lo, po, lda, a + jo + (size_t)lo * lda, lda);
+                        if (jo + wo < m)
+                            Ops<T>::gemm(ctx, 0, m - jo - wo, hi - lo, wo, Ops<T>::neg_one(), po + wo, lda,
+                                         a + jo + (size_t)lo * lda, lda, Ops<T>::one(),
+                                         a + jo + wo + (size_t)lo * lda, lda);
+                    }
+                };
+                {
+                    // The rest, on the side stream.
+                    OnStream on(ctx, side);
+                    TNS_HIP(hipStreamWaitEvent(side, ctx->ev_panel, 0));
+                    apply_moves<T>(ctx, a + jb, lda, jo, c2, n, list, moves);
+                    update(c2, last ? n : end);
+                    TNS_HIP(hipEventRecord(ctx->ev_rest[step & 1], side));
+                }
+                update(c1, c2);  // the next panel's columns, on the main stream
+            }
+        }
+        if (step > 0) TNS_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_rest[(step - 1) & 1], 0));
+    }
+}
+
 // work: scratch carved by the caller (lu_work_ints(m, n, nb) ints, 8-byte
 // aligned), or null to take it from the pool.
 template <typename T>
@@ -2237,7 +2333,13 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
         hipLaunchKernelGGL(spread_init_kernel, dim3((init + 255) / 256), dim3(256), 0, ctx->stream, m, d_perm, rc,
                            xbuf, xwords, lists, spread ? steps : 0, lstride);
         if (clk) clk->mark(PhaseClock::NONE);
-        if (whole) {
+        const int ob = lu_outer_block<T>(m, n, nb);
+        // Look-ahead needs the second stream, no phase clock (its marks time
+        // one stream) and no graph capture.
+        const bool ahead = spread && !whole && steps > 1 && !clk && ctx->side && !ctx->hold;
+        if (ahead) {
+            getrf_lookahead<T>(ctx, m, n, a, lda, ib, ob, d_piv, d_perm, d_info, lists, lstride, xbuf);
+        } else if (whole) {
             if (spread) {
                 panel_spread<T>(ctx, m, n, a, lda, d_piv, 0, d_perm, d_info, lists, xbuf, 0, 0, clk);
             } else {
@@ -2253,7 +2355,6 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
             // is read and written once per outer step instead of once per
             // panel. Every panel sees the same updated columns as with
             // ob = nb, so the pivots are the same.
-            const int ob = lu_outer_block<T>(m, n, nb);
             int step = 0;
             for (int jo = 0; jo < limit; jo += ob) {
                 const int wo = ob < limit - jo ? ob : limit - jo;

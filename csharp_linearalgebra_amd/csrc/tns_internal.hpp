@@ -95,6 +95,13 @@ struct Context {
     // The batch autotuner's choices: batch shape -> (streams, 4M tile).
     std::map<std::string, std::pair<int, int>> batch_tuning;
     hipStream_t stream = nullptr;
+    // The LU's look-ahead: a second stream for the trailing updates while
+    // the next panel is factored on `stream`, and the events that order
+    // them (a panel and its interchanges on `stream` done; the trailing
+    // update of the even and of the odd steps done).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_panel = nullptr;
+    hipEvent_t ev_rest[2] = {nullptr, nullptr};
     Pool pool{size_t(64) << 30};
     // Small fixed device areas for reductions and flags.
     double* d_scal = nullptr;   // 64 doubles
