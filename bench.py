@@ -6,12 +6,17 @@ A = -D + iW with ||A||_1 = 50 (degree-13 Pade, s squarings, 4M products on
 FP64 matrix cores), inputs resident in device memory. `value` is its
 algorithmic rate in GFLOP/s: ((6 + s) * 8n^3 + (8/3)n^3 + 8n^3) per step.
 
-Also reported on the same line: the dominant kernel's roofline (the 4M
-complex GEMM at n = 1000, timed with HIP events on the library's stream),
-the exponential action exp(tA)B of the same operator (B 1000 x 8, t = 0.5),
-the FP64 LU + solve + rcond at n = 1024, the FP64 GEMM at n = 8192, and the
-CPU baseline: the oracle (a C restatement of Tensile's algorithms, OpenMP
-GEMM) running one exponential of the same workload on the host.
+Also reported on the same line: the step's device-time split by phase
+(HIP events between the phases of one timed Expm), the dominant kernel's
+roofline (the 4M complex GEMM at n = 1000, HIP events around each launch on
+the library's stream), the exponential action exp(tA)B of the same
+operator (B 1000 x 8, t = 0.5), the FP64 LU + solve + rcond at n = 1024
+(host operands and device-resident), the FP64 GEMM at n = 8192, the
+mixed-precision solves against the FP64 LU (and against an FP32
+partial-pivoting LU with FP64 refinement), the batched sweep, the Expmv
+gradients, and beside each a CPU baseline: the oracle (a C restatement of
+Tensile's algorithms, OpenMP GEMM) on a bounded sample of the same workload
+on the host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
   python bench.py --roofline-only   # the dominant kernel alone (20 launches)
@@ -76,6 +81,27 @@ def event_time(ws, fn, reps):
     return start.elapsed_time(stop) / reps / 1e3
 
 
+def launch_time(ws, fn, reps):
+    """Average device time of one fn() launch: HIP events recorded on the
+    library's stream right before and after each launch (what a kernel
+    trace reports as the kernel's duration, plus the event overhead)."""
+    import torch
+
+    stream = torch.cuda.ExternalStream(ws.stream())
+    fn()
+    ws.synchronize()
+    pairs = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        pairs.append((a, b))
+    ws.synchronize()
+    return sum(a.elapsed_time(b) for a, b in pairs) / reps / 1e3
+
+
 def roofline_only():
     """The dominant kernel alone: 20 launches of the 4M ZGEMM at n = 1000 on
     the benchmark's operator, then 20 device copies of 1 GiB (the counter
@@ -91,7 +117,7 @@ def roofline_only():
     dA = to_device(a)
     dC = empty_device(n, n, True)
     torch.cuda.synchronize()
-    s = event_time(ws, lambda: ws.multiply(dA, dA, dC), 20)
+    s = launch_time(ws, lambda: ws.multiply(dA, dA, dC), 20)
     src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     src.fill_(1.0)
@@ -146,6 +172,8 @@ def mixed_leg(ws, n, kappas, nrhs=32):
         den = a.abs().sum(1).max() * x64.T.abs().max(0).values + bt.T.abs().max(0).values
         berr64 = float((r.abs().max(0).values / den).max())
         row = {"kappa": kappa, "fp64_ms": round(t64 * 1e3, 2), "fp64_backward_error": berr64}
+        if kappa <= 1e2:
+            row["fp32_partial_pivoting_ir"] = fp32_pp_ir(ws, a, at, bt, n, nrhs, t64)
         for prec in (32, 16):
             dX = DeviceMatrix.from_torch(torch.empty_like(bt))
             ws.solve_mixed(dA, dB, precision=prec, out=dX)
@@ -169,7 +197,53 @@ def mixed_leg(ws, n, kappas, nrhs=32):
     return out
 
 
-def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
+def fp32_pp_ir(ws, a, at, bt, n, nrhs, t64, tol=1e-15, max_iter=30):
+    """The precision gain apart from the random butterfly transform: FP32 LU
+    with partial pivoting on the FP32 matrix cores (tns_sgetrf), then
+    classical iterative refinement - residual and update in FP64, each
+    correction through the FP32 factors (tns_sgetrs) - until the normwise
+    backward error of every column is at most tol."""
+    import ctypes as C
+
+    import torch
+
+    lib = ws._lib
+    piv = np.zeros(n, dtype=np.int32)
+    sc = C.c_int()
+    anorm = a.abs().sum(1).max()
+
+    def run():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f32 = at.float()
+        torch.cuda.synchronize()
+        ws.check(lib.tns_sgetrf(ws.handle, 32, n, f32.data_ptr(), n, piv.ctypes.data, C.byref(sc)))
+        tf = time.perf_counter()
+        x = torch.zeros_like(bt)
+        r = bt.clone()
+        it, berr = 0, float("inf")
+        while it < max_iter:
+            c = r.float().contiguous()
+            torch.cuda.synchronize()
+            ws.check(lib.tns_sgetrs(ws.handle, 32, n, nrhs, f32.data_ptr(), n, piv.ctypes.data, c.data_ptr(), n))
+            x += c.double()
+            it += 1
+            r = bt - (a @ x.T).T
+            den = anorm * x.abs().max(1).values + bt.abs().max(1).values
+            berr = float((r.abs().max(1).values / den).max())
+            if not berr > tol:
+                break
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        return t1 - t0, tf - t0, it, berr
+
+    run()
+    dt, tf, it, berr = min(run() for _ in range(2))
+    return {"ms": round(dt * 1e3, 2), "speedup": round(t64 / dt, 3), "factor_ms": round(tf * 1e3, 2),
+            "iterations": it, "backward_error": berr, "converged": berr <= tol}
+
+
+def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4, cpu_sample=False):
     """The frequency sweep: `count` complex systems Z(w) = R + i w L of order
     n ~ U{nmin..nmax} with 4 right-hand sides, factored and solved in one
     graph launch (autotuned), then rcond of every system."""
@@ -213,6 +287,22 @@ def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
         den = a.abs().sum(1).max() * xx.abs().max() + b.abs().max()
         worst = max(worst, float(r.abs().max() / den))
     plan.close()
+    cpu = None
+    if cpu_sample:
+        import oracle as O
+
+        # The first 8 systems of the same sweep on the host: LU, solve, rcond.
+        sample = list(zip(mats, rhs))[:8]
+        hosts = [(m.T.cpu().numpy(), b.T.cpu().numpy()) for m, b in sample]
+        t0 = time.perf_counter()
+        for a_h, b_h in hosts:
+            f = O.getrf(a_h)
+            O.getrs(f, b_h)
+            O.gecon(f)
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(len(hosts) / cs, 2), "unit": "solves+rcond/s", "cores": O.threads(), "kind": "port",
+               "sample": "the sweep's first %d systems (orders %s): LU, 4-RHS solve and rcond each, %.2f s"
+                         % (len(hosts), [int(m.shape[0]) for m, _ in sample], cs)}
     return {"workload": "%d complex128 systems, n ~ U{%d..%d}, %d RHS, one graph launch" % (count, nmin, nmax, nrhs),
             "solve_ms": round(best * 1e3, 2), "solves_per_s": round(count / best, 1),
             "gflops": round(W.sweep_flops(orders, nrhs) / best / 1e9, 1),
@@ -220,10 +310,10 @@ def sweep_leg(ws, count=512, nmin=64, nmax=1024, nrhs=4):
             "tuned": {"panel_width": block, "zgemm_tile": "64x64" if cfg == 1 else "64x32", "graph_ms": round(tuned_ms, 2),
                       "tuning_s": round(tune_s, 2)},
             "singular": int((sing >= 0).sum()), "max_backward_error": worst,
-            "rcond_range": [float(rc.min()), float(rc.max())]}
+            "rcond_range": [float(rc.min()), float(rc.max())], "cpu_baseline": cpu}
 
 
-def grad_leg(ws, sizes=(256, 1024)):
+def grad_leg(ws, sizes=(256, 1024), cpu_sample=False):
     """Gradients of L = ||C exp(tA) B - Y||_F^2 (t = 1, ||A||_1 = 20, B n x 4,
     C 8 x n, Y 8 x 4) for real and complex A: forward Expmv + the adjoint
     (dL/dA, dL/dB), with the error against the 2n x 2n block exponential
@@ -263,8 +353,20 @@ def grad_leg(ws, sizes=(256, 1024)):
             ra = e[:n, n:]
             got = ga.owner.cpu().numpy().T
             err = float(np.abs(got - ra).max() / np.abs(ra).max())
-            out.append({"n": n, "complex": cplx, "forward_ms": round(t_fwd * 1e3, 2),
-                        "backward_ms": round(t_bwd * 1e3, 2), "grad_a_rel_error_vs_block_expm": err})
+            row = {"n": n, "complex": cplx, "forward_ms": round(t_fwd * 1e3, 2),
+                   "backward_ms": round(t_bwd * 1e3, 2), "grad_a_rel_error_vs_block_expm": err}
+            if cpu_sample and n <= 256:
+                import oracle as O
+
+                # The host's way to this gradient: the oracle's Expm of the
+                # 2n x 2n block matrix (the reference has no adjoint).
+                t0 = time.perf_counter()
+                O.expm(blk)
+                cs = time.perf_counter() - t0
+                row["cpu_baseline"] = {"value": round(cs * 1e3, 1), "unit": "ms", "cores": O.threads(),
+                                       "kind": "port",
+                                       "sample": "dL/dA by the oracle's Expm of the %dx%d block matrix" % (2 * n, 2 * n)}
+            out.append(row)
     return out
 
 
@@ -339,6 +441,43 @@ def kernels_leg(ws, n=8192):
     return {"trsv_n%d_us" % n: round(trsv * 1e6, 1), "trsv_gbs": round(4.0 * n * n / trsv / 1e9, 1)}
 
 
+def lu_device(ws, la, lb):
+    """Config 1 with every operand in device memory: getrf, the 16-RHS getrs
+    and gecon through the C ABI (best of three)."""
+    import ctypes as C
+
+    import torch
+
+    from csharp_linearalgebra_amd import _lib as L
+
+    lib = ws._lib
+    n, nrhs = la.shape[0], lb.shape[1]
+    at = torch.from_numpy(np.ascontiguousarray(la.T)).cuda()
+    bt = torch.from_numpy(np.ascontiguousarray(lb.T)).cuda()
+    f, x = torch.empty_like(at), torch.empty_like(bt)
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = L.LuInfo()
+    anorm = float(np.abs(la).sum(0).max())
+    best = None
+    for _ in range(4):
+        f.copy_(at)
+        x.copy_(bt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ws.check(lib.tns_dgetrf(ws.handle, n, n, f.data_ptr(), n, 0, piv.data_ptr(), C.byref(info)))
+        t1 = time.perf_counter()
+        ws.check(lib.tns_dgetrs(ws.handle, 0, n, nrhs, f.data_ptr(), n, piv.data_ptr(), -1, x.data_ptr(), n))
+        ws.synchronize()
+        t2 = time.perf_counter()
+        rc = C.c_double()
+        ws.check(lib.tns_dgecon(ws.handle, n, f.data_ptr(), n, piv.data_ptr(), -1, anorm, 2, C.byref(rc)))
+        t3 = time.perf_counter()
+        row = (t1 - t0, t2 - t1, t3 - t2)
+        best = row if best is None or sum(row) < sum(best) else best
+    return {"getrf_ms": round(best[0] * 1e3, 3), "getrs16_ms": round(best[1] * 1e3, 3),
+            "gecon_ms": round(best[2] * 1e3, 3), "total_ms": round(sum(best) * 1e3, 3)}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
     if not os.path.exists(path):
@@ -411,9 +550,29 @@ def main():
     flops = W.expm_flops(n, diag.squarings, True)
     gflops = flops / step_s / 1e9
 
+    # ---- the step's split: three more steps with HIP events between phases ------
+    ph = T.ExpmPhaseTimings()
+    for _ in range(3):
+        ws.expm(dA, out=dE, phases=ph)
+    phases = ph.per_call()
+    families = {"gemm_products": round(phases["powers"] + phases["pade"] + phases["squaring"], 3),
+                "norm_estimates": phases["select"], "lu": phases["lu"], "solve": phases["solve"],
+                "total": phases["total"]}
+    # The LU's own split at the denominator's order (the sequential loop:
+    # its phase clock turns the look-ahead off).
+    lph = T.LuPhaseTimings()
+    dL = empty_device(n, n, True)
+    for _ in range(3):
+        dL.owner.copy_(dA.owner)
+        torch.cuda.synchronize()
+        ws.factor_lu_device(dL, timings=lph)
+    lu_split = {k: round(v / 3, 3) for k, v in (("panel", lph.panel), ("swaps", lph.swaps),
+                                                ("trsm", lph.triangular), ("gemm", lph.gemm),
+                                                ("total", lph.total))}
+
     # ---- dominant kernel: 4M complex GEMM, n = 1000 ---------------------------------
     dC = empty_device(n, n, True)
-    g_s = event_time(ws, lambda: ws.multiply(dA, dA, dC), 20)
+    g_s = launch_time(ws, lambda: ws.multiply(dA, dA, dC), 20)
     zg_tflops = 8.0 * n ** 3 / g_s / 1e12
 
     # ---- exponential action exp(tA)B, B n x 8, t = 0.5 ---------------------------
@@ -443,6 +602,7 @@ def main():
         rc = lu.reciprocal_condition()
     solve_s = (time.perf_counter() - t0) / reps
     lu_bwd = float(np.abs(la @ x - lb).max() / (np.abs(la).sum(1).max() * np.abs(x).max()))
+    dev = lu_device(ws, la, lb)
 
     # ---- FP64 GEMM, n = 8192 -----------------------------------------------------
     ng = 8192
@@ -471,13 +631,18 @@ def main():
                    "n": n, "degree": diag.degree, "squarings": diag.squarings},
         "roofline": {"bound": "mfma", "kernel": "zgemm_kernel (4M, n=1000)", "achieved": round(zg_tflops, 2),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(zg_tflops / FP64_PEAK_TFLOPS, 4),
-                     "traffic": load_traffic(), "avg_launch_us": round(g_s * 1e6, 1)},
+                     "traffic": load_traffic(), "avg_launch_us": round(g_s * 1e6, 1),
+                     "algorithmic_flops_per_launch": 8.0 * n ** 3},
         "expmv": {"workload": "complex128 exp(tA)B, n=1000, B 1000x8, t=0.5", "ms": round(expmv_s * 1e3, 3),
                   "degree": vd.degree, "scaling": vd.scaling, "applications": vd.applications,
                   "gflops": round(expmv_flops / expmv_s / 1e9, 2)},
-        "lu_n1024": {"getrf_ms": round(lu_s * 1e3, 3),
-                     "getrf_gflops": round(W.getrf_flops(1024) / lu_s / 1e9, 1),
-                     "solve16_plus_rcond_ms": round(solve_s * 1e3, 3), "rcond": rc, "backward_error": lu_bwd,
+        "step_phases_ms": phases,
+        "step_families_ms": families,
+        "lu_phase_split_n1000_complex_ms": lu_split,
+        "lu_n1024": {"host_operands": {"getrf_ms": round(lu_s * 1e3, 3),
+                                       "getrf_gflops": round(W.getrf_flops(1024) / lu_s / 1e9, 1),
+                                       "solve16_plus_rcond_ms": round(solve_s * 1e3, 3)},
+                     "device_resident": dev, "rcond": rc, "backward_error": lu_bwd,
                      "identity_pivots": bool(np.array_equal(lu.pivots, np.arange(1024)))},
         "dgemm_n8192": {"tflops": round(dg_tflops, 2), "frac": round(dg_tflops / FP64_PEAK_TFLOPS, 4)},
     }
@@ -489,16 +654,42 @@ def main():
     row = result["mixed_lu"]["8192"][0]
     result["mixed_speedup_n8192"] = {"kappa": row["kappa"], "fp64_ms": row["fp64_ms"],
                                      "fp32": row["fp32"]["speedup"], "bf16": row["bf16"]["speedup"],
-                                     "backward_error_fp32": row["fp32"]["backward_error"]}
+                                     "backward_error_fp32": row["fp32"]["backward_error"],
+                                     "fp32_partial_pivoting_ir": row["fp32_partial_pivoting_ir"]["speedup"]}
 
     # ---- batched frequency sweep ------------------------------------------------
-    result["batched_sweep"] = sweep_leg(ws)
+    result["batched_sweep"] = sweep_leg(ws, cpu_sample=not args.no_cpu)
 
     # ---- gradients through exp(tA)B ---------------------------------------------
-    result["expmv_gradients"] = grad_leg(ws)
+    result["expmv_gradients"] = grad_leg(ws, cpu_sample=not args.no_cpu)
 
     if not args.no_cpu:
         import oracle as O
+
+        # Config 1 on the host: LU, 16-RHS solve and rcond of the same system.
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            f = O.getrf(la)
+            O.getrs(f, lb)
+            O.gecon(f)
+            reps += 1
+            cs = time.perf_counter() - t0
+            if cs >= 3.0:
+                break
+        result["lu_n1024"]["cpu_baseline"] = {
+            "value": round(cs / reps * 1e3, 2), "unit": "ms per LU+solve16+rcond", "cores": O.threads(),
+            "kind": "port", "gflops": round(W.getrf_flops(1024) * reps / cs / 1e9, 1),
+            "sample": "%d runs of the same n=1024 system, %.1f s" % (reps, cs)}
+        # The FP64 LU the mixed solves are measured against, at n = 4096, on
+        # the host (one factorization of a matrix of the same kind).
+        g4 = np.random.default_rng(0)
+        m4 = g4.uniform(-1, 1, (4096, 4096))
+        t0 = time.perf_counter()
+        O.getrf(m4)
+        cs = time.perf_counter() - t0
+        result["mixed_lu"]["cpu_baseline"] = {
+            "value": round(W.getrf_flops(4096) / cs / 1e9, 1), "unit": "GFLOP/s (FP64 LU)", "cores": O.threads(),
+            "kind": "port", "ms": round(cs * 1e3, 1), "sample": "one FP64 LU of a 4096 x 4096 matrix"}
 
         # A bounded sample: whole exponentials of the same operator until
         # about 10 s of host time (at most 30 s) have passed.

@@ -24,6 +24,7 @@ from .tensile import (  # noqa: F401
     InvalidOperationError,
     LuDecomposition,
     LuPhaseTimings,
+    ExpmPhaseTimings,
     MixedInfo,
     NormEstimateResult,
     ObjectDisposedError,

@@ -48,6 +48,12 @@ class LuPhases(C.Structure):
                 ("gemm_ms", C.c_double), ("total_ms", C.c_double), ("block_steps", C.c_int)]
 
 
+class ExpmPhases(C.Structure):
+    _fields_ = [("powers_ms", C.c_double), ("select_ms", C.c_double), ("pade_ms", C.c_double),
+                ("lu_ms", C.c_double), ("solve_ms", C.c_double), ("squaring_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+
 class ExpmvDiag(C.Structure):
     _fields_ = [("degree", C.c_int), ("scaling", C.c_int), ("applications", C.c_int), ("early_exits", C.c_int)]
 
@@ -100,6 +106,8 @@ SIGNATURES = {
     "tns_zbatch_destroy": (_I, [_P]),
     "tns_dexpm": (_I, [_P, _I, _P, _I, _P, _I, C.POINTER(ExpmDiag)]),
     "tns_zexpm": (_I, [_P, _I, _P, _I, _P, _I, C.POINTER(ExpmDiag)]),
+    "tns_dexpm_timed": (_I, [_P, _I, _P, _I, _P, _I, C.POINTER(ExpmDiag), C.POINTER(ExpmPhases)]),
+    "tns_zexpm_timed": (_I, [_P, _I, _P, _I, _P, _I, C.POINTER(ExpmDiag), C.POINTER(ExpmPhases)]),
     "tns_dexpmv": (_I, [_P, _I, _I, _P, _I, _P, _I, _D, _P, _I, C.POINTER(ExpmvDiag)]),
     "tns_zexpmv": (_I, [_P, _I, _I, _P, _I, _P, _I, _D, _P, _I, C.POINTER(ExpmvDiag)]),
     "tns_dexpmv_op": (_I, [_P, _I, DAPPLY, _P, _I, _I, _P, _I, _D, _D, _P, _I, C.POINTER(ExpmvDiag)]),

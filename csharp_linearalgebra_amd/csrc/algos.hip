@@ -532,16 +532,18 @@ void product(Context* ctx, int n, const T* x, const T* y, T* out)
 
 // V - U \ V + U into out (MatrixExponential.cs:372-405).
 template <typename T>
-void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out)
+void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out, EventClock* clk = nullptr)
 {
     Mat<T> den(ctx, n);
     sub_add<T>(ctx, (size_t)n * n, u, v, den.p(), out);
     DevBuf<int> piv(ctx, n), perm(ctx, n);
     int* d_info = ctx->d_int;
+    if (clk) clk->mark(ExpmPhases::PADE);
     getrf_t<T>(ctx, n, den.p(), n, piv.get(), perm.get(), d_info);
     int info = 0;
     TNS_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TNS_HIP(hipStreamSynchronize(ctx->stream));
+    if (clk) clk->mark(ExpmPhases::LU);
     if (info == LU_ABORTED) fail(TNS_EDEVICE, "the Pade denominator's LU panel exchange timed out");
     if (info >= 0)
         fail(TNS_ESTATE, "The Pade denominator is exactly singular at column " + std::to_string(info) +
@@ -550,11 +552,12 @@ void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out)
     // selection allows, so its n-column solve takes the block-inverse form.
     if constexpr (sizeof(T) == 8) dgetrs_blockinv(ctx, n, n, (const double*)den.p(), n, perm.get(), (double*)out, n);
     else zgetrs_blockinv(ctx, n, n, (const zd*)den.p(), n, perm.get(), (zd*)out, n);
+    if (clk) clk->mark(ExpmPhases::SOLVE);
 }
 
 template <typename T>
 void low_degree(Context* ctx, const double* b, int m, int n, const T* a, const T* a2, const T* a4, const T* a6,
-                const T* a8, T* out)
+                const T* a8, T* out, EventClock* clk)
 {
     LinTerms<T> lw{}, lv{};
     const T* pw[4] = {a2, a4, a6, a8};
@@ -569,12 +572,12 @@ void low_degree(Context* ctx, const double* b, int m, int n, const T* a, const T
     lincomb<T>(ctx, n, b[1], terms, lw, w.p());
     product<T>(ctx, n, a, w.p(), u.p());
     lincomb<T>(ctx, n, b[0], terms, lv, v.p());
-    solve_pade<T>(ctx, n, u.p(), v.p(), out);
+    solve_pade<T>(ctx, n, u.p(), v.p(), out, clk);
 }
 
 // Degree 13 by Paterson-Stockmeyer (MatrixExponential.cs:323-360).
 template <typename T>
-void degree13(Context* ctx, int n, const T* a, const T* a2, const T* a4, const T* a6, T* out)
+void degree13(Context* ctx, int n, const T* a, const T* a2, const T* a4, const T* a6, T* out, EventClock* clk)
 {
     const double* b = kPade13;
     Mat<T> inner(ctx, n), w(ctx, n), u(ctx, n), v(ctx, n);
@@ -598,16 +601,19 @@ void degree13(Context* ctx, int n, const T* a, const T* a2, const T* a4, const T
     l.src[2] = a6; l.coef[2] = b[6];
     lincomb<T>(ctx, n, b[0], 3, l, v.p());
     gemm_t<T>(ctx, n, n, n, 1.0, a6, n, inner.p(), n, 1.0, v.p(), n);
-    solve_pade<T>(ctx, n, u.p(), v.p(), out);
+    solve_pade<T>(ctx, n, u.p(), v.p(), out, clk);
 }
 
 }  // namespace
 
 template <typename T>
-void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* diag)
+void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* diag, ExpmPhases* phases)
 {
     if (diag) *diag = ExpmDiag{};
+    if (phases) *phases = ExpmPhases{};
     if (n <= 0) return;
+    EventClock clock(ctx), *clk = phases ? &clock : nullptr;
+    if (clk) clk->mark(-1);
     Mat<T> a(ctx, n), a2(ctx, n), a4(ctx, n), a6(ctx, n), x(ctx, n);
     copy_t<T>(ctx, n, n, ain, lda, a.p(), n);
     if (n == 1) {
@@ -632,16 +638,19 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
     norm_t<T>(ctx, '1', n, n, a6.p(), n, ctx->d_scal + 1);
     double nn[2];
     read_doubles(ctx, ctx->d_scal, nn, 2);
+    if (clk) clk->mark(ExpmPhases::POWERS);
     const double d4 = std::pow(nn[0], 1.0 / 4.0), d6 = std::pow(nn[1], 1.0 / 6.0);
     const double eta1 = std::fmax(d4, d6);
     int degree = 13, s = 0;
 
     if (eta1 < 1.495585217958292e-2 && ell<T>(ctx, n, a.p(), 3) == 0) {
         degree = 3;
-        low_degree<T>(ctx, kPade3, 3, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
+        if (clk) clk->mark(ExpmPhases::SELECT);
+        low_degree<T>(ctx, kPade3, 3, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p(), clk);
     } else if (eta1 < 2.539398330063230e-1 && ell<T>(ctx, n, a.p(), 5) == 0) {
         degree = 5;
-        low_degree<T>(ctx, kPade5, 5, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
+        if (clk) clk->mark(ExpmPhases::SELECT);
+        low_degree<T>(ctx, kPade5, 5, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p(), clk);
     } else {
         // d8 and d10 in one lockstep run (d10 is read only past the degree 7
         // and 9 tests, as the reference computes it; estimating it with d8
@@ -651,12 +660,14 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
         const double eta3 = std::fmax(d6, d8);
         if (eta3 < 9.504178996162932e-1 && ell<T>(ctx, n, a.p(), 7) == 0) {
             degree = 7;
-            low_degree<T>(ctx, kPade7, 7, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p());
+            if (clk) clk->mark(ExpmPhases::SELECT);
+            low_degree<T>(ctx, kPade7, 7, n, a.p(), a2.p(), a4.p(), a6.p(), nullptr, x.p(), clk);
         } else if (eta3 < 2.097847961257068e0 && ell<T>(ctx, n, a.p(), 9) == 0) {
             degree = 9;
+            if (clk) clk->mark(ExpmPhases::SELECT);
             Mat<T> a8(ctx, n);
             product<T>(ctx, n, a4.p(), a4.p(), a8.p());
-            low_degree<T>(ctx, kPade9, 9, n, a.p(), a2.p(), a4.p(), a6.p(), a8.p(), x.p());
+            low_degree<T>(ctx, kPade9, 9, n, a.p(), a2.p(), a4.p(), a6.p(), a8.p(), x.p(), clk);
         } else {
             const double d10 = std::pow(d810[1].value, 1.0 / 10.0);
             const double eta4 = std::fmax(d8, d10);
@@ -668,6 +679,7 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
                 scale_to<T>(ctx, (size_t)n * n, a.p(), std::ldexp(1.0, -s), sc.p());
                 s += ell<T>(ctx, n, sc.p(), 13);
             }
+            if (clk) clk->mark(ExpmPhases::SELECT);
             const double f = std::ldexp(1.0, -s);
             // Rescaled from the originals, in place: a, a2, a4, a6 are not
             // needed unscaled again.
@@ -675,7 +687,7 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
             scale_to<T>(ctx, (size_t)n * n, a2.p(), f * f, a2.p());
             scale_to<T>(ctx, (size_t)n * n, a4.p(), std::pow(f, 4), a4.p());
             scale_to<T>(ctx, (size_t)n * n, a6.p(), std::pow(f, 6), a6.p());
-            degree13<T>(ctx, n, a.p(), a2.p(), a4.p(), a6.p(), x.p());
+            degree13<T>(ctx, n, a.p(), a2.p(), a4.p(), a6.p(), x.p(), clk);
             // exp(A) = exp(A/2^s)^(2^s): s squarings, ping-ponging buffers.
             T* cur = x.p();
             T* other = a2.p();
@@ -691,6 +703,10 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
         diag->squarings = s;
     }
     copy_t<T>(ctx, n, n, x.p(), n, e, lde);
+    if (clk) {
+        clk->mark(ExpmPhases::SQUARE);
+        clk->read(phases->ms, 6, &phases->total_ms);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1160,8 +1176,8 @@ template DevOp<double> inverse_operator<double>(Context*, int, const double*, in
 template DevOp<zd> inverse_operator<zd>(Context*, int, const zd*, int, const int*);
 template double gecon<double>(Context*, int, const double*, int, const int*, int, double, int);
 template double gecon<zd>(Context*, int, const zd*, int, const int*, int, double, int);
-template void expm<double>(Context*, int, const double*, int, double*, int, ExpmDiag*);
-template void expm<zd>(Context*, int, const zd*, int, zd*, int, ExpmDiag*);
+template void expm<double>(Context*, int, const double*, int, double*, int, ExpmDiag*, ExpmPhases*);
+template void expm<zd>(Context*, int, const zd*, int, zd*, int, ExpmDiag*, ExpmPhases*);
 template void expmv<double>(Context*, int, int, const double*, int, const double*, int, double, double*, int,
                             ExpmvDiag*);
 template void expmv<zd>(Context*, int, int, const zd*, int, const zd*, int, double, zd*, int,

@@ -78,11 +78,21 @@ struct ExpmDiag {
     int squarings = 0;
 };
 
+// Device milliseconds of one expm by phase: the powers A^2, A^4, A^6 and
+// their norms; the degree and scaling selection; the Pade products; the
+// denominator's LU; its solve; the squarings.
+struct ExpmPhases {
+    enum { POWERS = 0, SELECT = 1, PADE = 2, LU = 3, SOLVE = 4, SQUARE = 5 };
+    double ms[6] = {0, 0, 0, 0, 0, 0};
+    double total_ms = 0;
+};
+
 // MatrixExponential.ScalingAndSquaring (MatrixExponential.cs:181-270) on a
 // device matrix a (n x n); writes exp(A) to e (ld lde). Throws TNS_ESTATE if
-// the Pade denominator is exactly singular.
+// the Pade denominator is exactly singular. phases, if given, receives the
+// phase split (the call then waits for the result).
 template <typename T>
-void expm(Context* ctx, int n, const T* a, int lda, T* e, int lde, ExpmDiag* diag);
+void expm(Context* ctx, int n, const T* a, int lda, T* e, int lde, ExpmDiag* diag, ExpmPhases* phases = nullptr);
 
 struct ExpmvDiag {
     int degree = 0;

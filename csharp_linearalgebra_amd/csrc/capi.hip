@@ -160,6 +160,29 @@ bool is_device_pointer(const void* p)
 
 void sync(Context* ctx) { TNS_HIP(hipStreamSynchronize(ctx->stream)); }
 
+void EventClock::mark(int b)
+{
+    hipEvent_t e = nullptr;
+    TNS_HIP(hipEventCreate(&e));
+    ev.push_back(e);
+    bucket.push_back(b);
+    TNS_HIP(hipEventRecord(e, ctx->stream));
+}
+
+void EventClock::read(double* out, int nb, double* total)
+{
+    if (ev.empty()) return;
+    TNS_HIP(hipEventSynchronize(ev.back()));
+    for (size_t i = 1; i < ev.size(); i++) {
+        float ms = 0.f;
+        TNS_HIP(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+        if (bucket[i] >= 0 && bucket[i] < nb) out[bucket[i]] += ms;
+    }
+    float all = 0.f;
+    TNS_HIP(hipEventElapsedTime(&all, ev.front(), ev.back()));
+    *total += all;
+}
+
 }  // namespace tns
 
 namespace {
@@ -508,7 +531,8 @@ void expmv_grad_body(Context* ctx, int n, int nb, const T* a, int lda, const T* 
 }
 
 template <typename T>
-void expm_body(Context* ctx, int n, const T* a, int lda, T* e, int lde, tns_expm_diag* diag)
+void expm_body(Context* ctx, int n, const T* a, int lda, T* e, int lde, tns_expm_diag* diag,
+               tns_expm_phases* phases = nullptr)
 {
     if (n < 0) fail(TNS_EINVAL, "negative order");
     check_operand(a, n, n, lda, "A");
@@ -520,10 +544,13 @@ void expm_body(Context* ctx, int n, const T* a, int lda, T* e, int lde, tns_expm
     View<T> va = bring(ctx, a, n, n, lda, true);
     View<T> ve = bring(ctx, e, n, n, lde, false);
     tns::ExpmDiag d;
-    tns::expm<T>(ctx, n, va.d, va.ld, ve.d, ve.ld, &d);
+    tns::ExpmPhases ph;
+    tns::expm<T>(ctx, n, va.d, va.ld, ve.d, ve.ld, &d, phases ? &ph : nullptr);
     send_back(ctx, ve, e, n, n, lde);
     tns::sync(ctx);
     if (diag) *diag = tns_expm_diag{d.degree, d.squarings};
+    if (phases)
+        *phases = tns_expm_phases{ph.ms[0], ph.ms[1], ph.ms[2], ph.ms[3], ph.ms[4], ph.ms[5], ph.total_ms};
 }
 
 template <typename T>
@@ -1130,6 +1157,20 @@ TNS_API int tns_zexpm(tns_context* ctx, int n, const tns_complex* a, int lda, tn
 {
     return guarded(ctx, [&](Context* c) {
         expm_body<zd>(c, n, (const zd*)a, lda, (zd*)e, lde, diag);
+    });
+}
+
+TNS_API int tns_dexpm_timed(tns_context* ctx, int n, const double* a, int lda, double* e, int lde,
+                            tns_expm_diag* diag, tns_expm_phases* phases)
+{
+    return guarded(ctx, [&](Context* c) { expm_body<double>(c, n, a, lda, e, lde, diag, phases); });
+}
+
+TNS_API int tns_zexpm_timed(tns_context* ctx, int n, const tns_complex* a, int lda, tns_complex* e, int lde,
+                            tns_expm_diag* diag, tns_expm_phases* phases)
+{
+    return guarded(ctx, [&](Context* c) {
+        expm_body<zd>(c, n, (const zd*)a, lda, (zd*)e, lde, diag, phases);
     });
 }
 

@@ -655,10 +655,11 @@ __global__ void __launch_bounds__(SPREAD_NT)
 getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv, int off, int* __restrict__ perm,
                     int* __restrict__ list, int P, unsigned long long* __restrict__ xbuf, int* __restrict__ info)
 {
-    static_assert(NB * sizeof(T) == 4 * SPREAD_ROW_WORDS, "a panel row is 128 exchange words");
+    static_assert(NB * sizeof(T) <= 4 * SPREAD_ROW_WORDS, "a panel row fits 128 exchange words");
     static_assert(NB <= 64 && NB % 8 == 0, "one ballot covers the row; columns go in groups of 8");
     constexpr int NT = SPREAD_NT, NW = NT / 64, GW = 8, NG = NB / GW;
     constexpr int WPE = (int)sizeof(T) / 4;  // exchange words per element
+    constexpr int RW = NB * WPE;              // exchange words per row
     int g = 0;
     if (P > 1) {
         if (blockIdx.x & 7) return;
@@ -735,8 +736,10 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
                 unsigned long long* recs = xbuf + 1 + (size_t)buf * SPREAD_MAXP * SPREAD_REC;
                 unsigned long long* mine = recs + (size_t)g * SPREAD_REC;
                 const unsigned* words = reinterpret_cast<const unsigned*>(prow);
-                // Only the columns from j on are read from a fetched row.
-                if (tid >= j * WPE && tid < SPREAD_ROW_WORDS) {
+                // Only the columns from j on are read from a fetched row,
+                // in pairs of words from an even word on.
+                const int lo = (j * WPE) & ~1;
+                if (tid >= lo && tid < RW) {
                     xstore(mine + 3 + tid, seq, words[tid]);
                 } else if (tid >= SPREAD_ROW_WORDS && tid < SPREAD_ROW_WORDS + 3) {
                     const int q = tid - SPREAD_ROW_WORDS;
@@ -756,7 +759,7 @@ getf2_spread_kernel(int m, int w, T* __restrict__ a, int lda, int* __restrict__ 
                     int gim;
                     const int gw = wave_argmax_u<false>(gk, gi, gkm, gim);
                     unsigned rw[2] = {0u, 0u};
-                    const bool need = 2 * lane + 1 >= j * WPE;
+                    const bool need = 2 * lane >= lo && 2 * lane < RW;
                     if (__all(ok) && need)
                         ok = xwait<2>(recs + (size_t)gw * SPREAD_REC + 3 + 2 * lane, seq, abort_word, rw);
                     if (need)
@@ -1847,39 +1850,18 @@ __global__ void __launch_bounds__(256) pivot_range_kernel(int n, const T* __rest
 // the stream after each phase, and the interval it closes is charged to that
 // phase's bucket; `total` is measured from the first event to the last, so
 // time no phase accounts for stays visible as the difference.
-struct PhaseClock {
+struct PhaseClock : EventClock {
     enum { NONE = -1, PANEL = 0, SWAPS = 1, TRSM = 2, GEMM = 3 };
-    Context* ctx;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> bucket;
-    explicit PhaseClock(Context* c) : ctx(c) {}
-    PhaseClock(const PhaseClock&) = delete;
-    PhaseClock& operator=(const PhaseClock&) = delete;
-    ~PhaseClock()
-    {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    }
-    void mark(int b)
-    {
-        hipEvent_t e = nullptr;
-        TNS_HIP(hipEventCreate(&e));
-        ev.push_back(e);
-        bucket.push_back(b);
-        TNS_HIP(hipEventRecord(e, ctx->stream));
-    }
+    using EventClock::EventClock;
     // Adds this factorization's split to *out (pooling, as LuPhaseTimings.Add).
     void read(LuPhases* out)
     {
-        TNS_HIP(hipEventSynchronize(ev.back()));
-        double* dst[4] = {&out->panel_ms, &out->swaps_ms, &out->trsm_ms, &out->gemm_ms};
-        for (size_t i = 1; i < ev.size(); i++) {
-            float ms = 0.f;
-            TNS_HIP(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
-            if (bucket[i] >= 0) *dst[bucket[i]] += ms;
-        }
-        float all = 0.f;
-        TNS_HIP(hipEventElapsedTime(&all, ev.front(), ev.back()));
-        out->total_ms += all;
+        double ms[4] = {0, 0, 0, 0};
+        EventClock::read(ms, 4, &out->total_ms);
+        out->panel_ms += ms[0];
+        out->swaps_ms += ms[1];
+        out->trsm_ms += ms[2];
+        out->gemm_ms += ms[3];
     }
 };
 
@@ -2170,11 +2152,11 @@ int lu_outer_block(int m, int n, int nb)
     return nb;
 }
 
-// Widest panel the spread kernel takes for T (0: it does not take T).
+// Widest panel the spread kernel takes for T.
 template <typename T>
 constexpr int spread_nb()
 {
-    return sizeof(T) == 8 ? 64 : sizeof(T) == 16 ? 32 : 0;
+    return sizeof(T) == 16 ? 32 : 64;
 }
 
 // Factors the m x w panel at a with getf2_spread_kernel and moves its

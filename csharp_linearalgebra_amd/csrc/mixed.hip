@@ -596,10 +596,11 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
                 DevBuf<double> V(ctx, (size_t)(mall + 1) * n * t), Z(ctx, (size_t)mall * n * t);
                 DevBuf<double> h(ctx, (size_t)(2 * (mall + 1) + 2 + mall) * t);
                 std::vector<double> target(t);
-                // The FP64 LU and solve take about 3.5 to 4 times as long as
-                // the low-precision factorization (measured at n = 4096 and
-                // 8192); refinement steps that would cost more are not taken.
-                const double fallback_ms = 3.5 * out.factor_ms;
+                // The FP64 LU and solve take about 2.5 times as long as the
+                // low-precision factorization (measured at n = 8192: 48 ms
+                // against 17.5 ms); refinement steps that would cost more
+                // are not taken.
+                const double fallback_ms = 2.5 * out.factor_ms;
                 double prev = berr;
                 while (iters < max_iter) {
                     const int m = std::min(mall, max_iter - iters);

@@ -166,6 +166,26 @@ struct LuPhases {
     int steps = 0;
 };
 
+// Device-time split of a sequence of launches on ctx->stream: mark(b)
+// records an event, and the time from the previous mark to it is charged
+// to bucket b (b < 0: charged to none). read() waits for the last mark.
+struct EventClock {
+    Context* ctx;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> bucket;
+    explicit EventClock(Context* c) : ctx(c) {}
+    EventClock(const EventClock&) = delete;
+    EventClock& operator=(const EventClock&) = delete;
+    ~EventClock()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void mark(int b);
+    // Adds each bucket's milliseconds to out[b] (b < nb) and the span from
+    // the first mark to the last to *total.
+    void read(double* out, int nb, double* total);
+};
+
 // LU (lu.hip). d_piv: min(m,n) LAPACK-order interchanges (0-based);
 // d_perm: m entries, row i of P*A is row d_perm[i] of A; d_info: the first
 // exactly-zero pivot column, -1, or LU_ABORTED if a panel's cross-workgroup

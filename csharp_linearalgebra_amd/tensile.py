@@ -324,8 +324,10 @@ class Workspace:
         return X, MixedInfo(info.precision, info.iterations, bool(info.fell_back), info.backward_error,
                             info.factor_ms, info.refine_ms, info.fallback_ms, info.first_error)
 
-    def expm(self, a, out=None):
-        """exp(A) and its ExpmDiagnostics (MatrixExponential.cs:143-270)."""
+    def expm(self, a, out=None, phases: "ExpmPhaseTimings" = None):
+        """exp(A) and its ExpmDiagnostics (MatrixExponential.cs:143-270).
+        phases, if given, receives this call's device-time split (powers,
+        degree/scaling selection, Pade products, LU, solve, squarings)."""
         cplx = _is_complex(a)
         A = _Operand(a, cplx)
         if A.rows != A.cols:
@@ -335,8 +337,14 @@ class Workspace:
         Eo = _Operand(E, cplx) if isinstance(E, DeviceMatrix) else None
         eptr, eld = (Eo.ptr, Eo.ld) if Eo else (E.ctypes.data if E.size else None, max(1, n))
         d = L.ExpmDiag()
-        fn = self._lib.tns_zexpm if cplx else self._lib.tns_dexpm
-        self.check(fn(self._h, n, A.ptr, A.ld, eptr, eld, C.byref(d)))
+        if phases is None:
+            fn = self._lib.tns_zexpm if cplx else self._lib.tns_dexpm
+            self.check(fn(self._h, n, A.ptr, A.ld, eptr, eld, C.byref(d)))
+        else:
+            ph = L.ExpmPhases()
+            fn = self._lib.tns_zexpm_timed if cplx else self._lib.tns_dexpm_timed
+            self.check(fn(self._h, n, A.ptr, A.ld, eptr, eld, C.byref(d), C.byref(ph)))
+            phases.add(ph)
         return E, ExpmDiagnostics(d.degree, d.squarings)
 
     def expmv(self, a, b, t=1.0, out=None):
@@ -726,6 +734,30 @@ class LuPhaseTimings:
     def add(self, other: "LuPhaseTimings"):
         for f in ("panel_ms", "swaps_ms", "trsm_ms", "gemm_ms", "total_ms", "block_steps"):
             setattr(self._raw, f, getattr(self._raw, f) + getattr(other._raw, f))
+
+
+class ExpmPhaseTimings:
+    """Device milliseconds of matrix exponentials by phase: the powers A^2,
+    A^4, A^6 and their norms; the degree and scaling selection (norm
+    estimates, ell corrections); the Pade products and combinations; the
+    denominator's LU; its solve; the squarings; and the total. An Expm given
+    this object adds its split (pooling over calls)."""
+
+    FIELDS = ("powers_ms", "select_ms", "pade_ms", "lu_ms", "solve_ms", "squaring_ms", "total_ms")
+
+    def __init__(self):
+        self.calls = 0
+        for f in self.FIELDS:
+            setattr(self, f, 0.0)
+
+    def add(self, raw):
+        self.calls += 1
+        for f in self.FIELDS:
+            setattr(self, f, getattr(self, f) + getattr(raw, f))
+
+    def per_call(self) -> dict:
+        c = max(1, self.calls)
+        return {f[:-3]: round(getattr(self, f) / c, 3) for f in self.FIELDS}
 
 
 # ---- fluent functions over the shared workspace (MatrixOperations.cs) ---------------------

@@ -230,6 +230,26 @@ TNS_API int tns_dexpm(tns_context* ctx, int n, const double* a, int lda, double*
                       tns_expm_diag* diag);
 TNS_API int tns_zexpm(tns_context* ctx, int n, const tns_complex* a, int lda, tns_complex* e, int lde,
                       tns_expm_diag* diag);
+/* The same with its device-time split, from HIP events between the phases
+ * (MatrixExponential.cs:181-270): the powers A^2, A^4, A^6 and their norms;
+ * the selection of degree and scaling (normest1 of the higher powers and the
+ * ell corrections); the Pade polynomial products and combinations; the LU
+ * of the denominator; its solve; the squarings. The diagnostics tool's
+ * phase report (tools/Tensile.Diagnostics/Program.cs:196-284) for Expm has no
+ * other counterpart in the reference. */
+typedef struct {
+    double powers_ms;
+    double select_ms;
+    double pade_ms;
+    double lu_ms;
+    double solve_ms;
+    double squaring_ms;
+    double total_ms;
+} tns_expm_phases;
+TNS_API int tns_dexpm_timed(tns_context* ctx, int n, const double* a, int lda, double* e, int lde,
+                            tns_expm_diag* diag, tns_expm_phases* phases);
+TNS_API int tns_zexpm_timed(tns_context* ctx, int n, const tns_complex* a, int lda, tns_complex* e, int lde,
+                            tns_expm_diag* diag, tns_expm_phases* phases);
 
 /* ---- exponential action: MatrixExponentialAction.Expmv
  * (MatrixExponentialAction.cs:102-178, MatrixExponentialAction.Complex.cs:34-63).

@@ -47,6 +47,7 @@ def test_a_destroyed_context_refuses_every_entry_point(ws):
     fbefore = fl.copy()
     info, phases, ne = L.LuInfo(), L.LuPhases(), L.NormEst()
     ed, evd, mi, out = L.ExpmDiag(), L.ExpmvDiag(), L.MixedInfo(), C.c_double(-7.0)
+    ep = L.ExpmPhases()
     one, zero = L.Complex(1, 0), L.Complex(0, 0)
     a, b, c, d, e = (addr(mats[k]) for k in "abcde")
     za, zb, zc, zd, ze = (addr(zmats[k]) for k in "abcde")
@@ -75,6 +76,8 @@ def test_a_destroyed_context_refuses_every_entry_point(ws):
         "tns_zgecon": (h, n, za, n, addr(piv), -1, 1.0, 2, C.byref(out)),
         "tns_dexpm": (h, n, a, n, c, n, C.byref(ed)),
         "tns_zexpm": (h, n, za, n, zc, n, C.byref(ed)),
+        "tns_dexpm_timed": (h, n, a, n, c, n, C.byref(ed), C.byref(ep)),
+        "tns_zexpm_timed": (h, n, za, n, zc, n, C.byref(ed), C.byref(ep)),
         "tns_dexpmv": (h, n, 2, a, n, b, n, 1.0, c, n, C.byref(evd)),
         "tns_zexpmv": (h, n, 2, za, n, zb, n, 1.0, zc, n, C.byref(evd)),
         "tns_dexpmv_op": (h, n, cb, None, 0, 2, b, n, 1.0, 1.0, c, n, C.byref(evd)),

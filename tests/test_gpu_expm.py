@@ -145,6 +145,28 @@ def test_config2_expm(ws):
     assert np.abs(e @ em - np.eye(1000)).max() <= 1e-9
 
 
+@pytest.mark.parametrize("norm", [0.05, 2.0, 50.0])
+def test_expm_phase_split(ws, norm):
+    """The timed Expm returns the same exponential bit for bit, and its
+    phases add up to at most its total (each phase is a span between two
+    events on the stream, the total the span from the first to the last)."""
+    import csharp_linearalgebra_amd as T
+
+    r = np.random.default_rng(3)
+    n = 200
+    a = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    a *= norm / np.linalg.norm(a, 1)
+    e0, d0 = ws.expm(a)
+    ph = T.ExpmPhaseTimings()
+    e1, d1 = ws.expm(a, phases=ph)
+    assert np.array_equal(e0, e1) and d0 == d1
+    parts = [ph.powers_ms, ph.select_ms, ph.pade_ms, ph.lu_ms, ph.solve_ms, ph.squaring_ms]
+    assert all(p >= 0.0 for p in parts) and ph.total_ms > 0.0
+    assert sum(parts) <= ph.total_ms * (1 + 1e-6) + 1e-3
+    assert ph.lu_ms > 0.0 and ph.solve_ms > 0.0 and ph.pade_ms > 0.0
+    assert ph.calls == 1 and ph.per_call()["total"] == pytest.approx(ph.total_ms, abs=1e-3)
+
+
 @pytest.mark.parametrize("name", ["e0", "e1", "e2", "e3", "e4", "e5", "e6"])
 def test_expmv_against_golden(ws, name):
     a, b, ref = GOLD[f"{name}_a"], GOLD[f"{name}_b"], GOLD[f"{name}_expmv"]

@@ -158,7 +158,7 @@ def test_three_part_bf16_products(ws, rng, m, n, k):
     assert err3.max() * 20 < err1.max()
 
 
-@pytest.mark.parametrize("n", [48, 200, 700, 1100])
+@pytest.mark.parametrize("n", [48, 200, 700, 1100, 2600])
 def test_single_precision_lu_matches_lapack_sgetrf(ws, rng, n):
     import scipy.linalg as sl
 
