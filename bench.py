@@ -478,6 +478,34 @@ def lu_device(ws, la, lb):
             "gecon_ms": round(best[2] * 1e3, 3), "total_ms": round(sum(best) * 1e3, 3)}
 
 
+def lu_leg(ws, n=8192):
+    """For rocprofv3: one FP64 LU of order n (device-resident, look-ahead)
+    and three FP64 GEMMs n x n x n, on uniform(-1, 1) matrices."""
+    import ctypes as C
+
+    import torch
+
+    from csharp_linearalgebra_amd import _lib as L
+
+    lib = ws._lib
+    g = torch.Generator(device="cuda").manual_seed(0)
+    a = torch.rand((n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    f = a.clone()
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = L.LuInfo()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ws.check(lib.tns_dgetrf(ws.handle, n, n, f.data_ptr(), n, 0, piv.data_ptr(), C.byref(info)))
+    ws.synchronize()
+    t_lu = time.perf_counter() - t0
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ws.check(lib.tns_dgemm(ws.handle, n, n, n, 1.0, a.data_ptr(), n, f.data_ptr(), n, 0.0, c.data_ptr(), n))
+    ws.synchronize()
+    return {"getrf_n%d_ms" % n: round(t_lu * 1e3, 2)}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "zgemm_n1000_traffic.json")
     if not os.path.exists(path):
@@ -493,7 +521,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--roofline-only", action="store_true", help="run only the dominant kernel (for rocprofv3)")
-    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels", "expm", "expmv"],
+    ap.add_argument("--leg", choices=["sweep", "mixed", "grad", "kernels", "expm", "expmv", "lu"],
                     help="run one extra leg alone and print its JSON (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -516,6 +544,8 @@ def main():
             out = expm_leg(ws)
         elif args.leg == "expmv":
             out = expmv_leg(ws)
+        elif args.leg == "lu":
+            out = lu_leg(ws)
         else:
             out = grad_leg(ws)
         print(json.dumps(out), flush=True)
