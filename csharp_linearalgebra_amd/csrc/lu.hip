@@ -27,8 +27,9 @@
 //    column: 64-row diagonal blocks solved by one wavefront with shuffles,
 //    the rest updated by the whole workgroup. Many right-hand sides (the
 //    Pade denominator's n) use blocked substitution: each diagonal block
-//    solved by workgroups of 32 columns with the block in LDS, the
-//    off-diagonal update as a GEMM.
+//    solved by workgroups of 32 columns with the block in LDS (the unit
+//    lower one, the LU's U12 solve, by one wavefront per column with x_k
+//    broadcast by readlane), the off-diagonal update as a GEMM.
 #include <algorithm>
 #include <climits>
 #include <type_traits>
@@ -1158,6 +1159,56 @@ trsm_block_kernel(int w, int ncols, const T* __restrict__ A, int lda, T* __restr
     trsm_block_body<T, BS, CB, LOWER, TR, UNIT>(w, ncols, A, lda, B, ldb, blockIdx.x);
 }
 
+// x_k of the wavefront, a uniform value: lane k's register by readlane.
+__device__ __forceinline__ double lane_value(double x, int k)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ zd lane_value(zd x, int k) { return zd{lane_value(x.x, k), lane_value(x.y, k)}; }
+__device__ __forceinline__ float lane_value(float x, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+
+// L X = B in place for the unit lower triangle L of order w <= BS (64 real,
+// 32 complex): one wavefront per column of B, lane i holding x_i, the
+// columns of L in LDS (D[k][i] = L(i, k), staged once per workgroup).
+// Step k takes x_k from lane k by readlane, so no barrier and no LDS round
+// trip separates the steps, and the lanes below k update with column k of
+// L. Each x_i takes its updates in k order and none for a zero x_k: the
+// operations and order of trsm_block_body. Workgroups walk the columns in
+// strides of the grid.
+template <typename T, int BS>
+__global__ void __launch_bounds__(256) trsm_llnu_wave_kernel(int w, int ncols, const T* __restrict__ L, int ldl,
+                                                             T* __restrict__ B, int ldb)
+{
+    __shared__ T D[BS][BS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    for (int e = tid; e < BS * BS; e += 256) {
+        const int i = e % BS, k = e / BS;
+        D[k][i] = load_if(L + i + (size_t)k * ldl, i < w && k < w && i > k);
+    }
+    __syncthreads();
+    const bool live = lane < w;
+    for (int c = blockIdx.x * 4 + (tid >> 6); c < ncols; c += gridDim.x * 4) {
+        T* b = B + (size_t)c * ldb;
+        T x = load_if(b + lane, live);
+#pragma unroll
+        for (int k = 0; k < BS - 1; k++) {
+            if (k + 1 >= w) break;
+            const T xk = lane_value(x, k);
+            if (!s_iszero(xk)) {
+                const T d = D[k][lane < BS ? lane : 0];
+                if (lane > k && live) x = s_add(x, s_mul(s_neg(xk), d));
+            }
+        }
+        if (live) b[lane] = x;
+    }
+}
+
 // U12 := L11^-1 A12 of many matrices in one launch.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -1991,6 +2042,14 @@ void sync_columns(Context* ctx, int m, int base, T* a, int lda, int c0, int c1, 
 template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ldb)
 {
+    if constexpr (LOWER && TR == 0 && UNIT) {
+        // The LU's U12 solve (and getrs' forward pass with many columns).
+        const int waves = (ncols + 3) / 4;
+        const int blocks = waves < 1024 ? waves : 1024;
+        hipLaunchKernelGGL((trsm_llnu_wave_kernel<T, tri_block<T>()>), dim3(blocks), dim3(256), 0, ctx->stream, w,
+                           ncols, A, lda, B, ldb);
+        return;
+    }
     constexpr int CB = 32;
     const int blocks = (ncols + CB - 1) / CB;
     hipLaunchKernelGGL((trsm_block_kernel<T, tri_block<T>(), CB, LOWER, TR, UNIT>), dim3(blocks), dim3(256), 0,
@@ -2385,7 +2444,7 @@ void getrf_impl(Context* ctx, int m, int n, T* a, int lda, int nb, int* d_piv, i
 
 // LU with the pivots of each block step chosen inside its NB x NB
 // diagonal block (block-local partial pivoting), right-looking in blocks of
-// NB = 64 columns: the diagonal block by the register panel kernel, its
+// NB = 64 columns: the diagonal block by local_block_lu_kernel, its
 // interchanges moved across the whole rows, the inverses of its two
 // triangles, then the rows below it L21 = A21 U11^-1 and the block row
 // U12 = L11^-1 A12 as products with those inverses (in place: each product
@@ -2404,40 +2463,64 @@ constexpr int LOCAL_NB = 64;
 constexpr int LOCAL_OB = 256;  // outer step of getrf_local_impl, a multiple of LOCAL_NB
 
 // The LOCAL_NB x LOCAL_NB diagonal block of getrf_local_impl factored by
-// one wavefront: lane i holds row i of the block in registers throughout,
-// and the column loop is unrolled, so every register index is a constant.
-// As in getf2_reg_body, interchanges move no data (each lane carries its
-// row's current position; the pivot takes position j, the row at j takes
-// the pivot's), the pivot is the largest |x| with ties to the lower
-// position, a zero pivot leaves the rows in place, and the multipliers and
-// updates are formed the same way; the pivot row reaches the other lanes
-// by readlane instead of LDS, and no barrier is needed.
-// The rows are stored to their final positions at the end; piv and perm
-// (from row jb) and the move list (block-relative) are as getf2_reg writes
-// them, and info is min-reduced with the first zero-pivot column.
-__global__ void __launch_bounds__(64) local_block_lu_kernel(float* __restrict__ a, int lda, int jb,
-                                                            int* __restrict__ piv, int* __restrict__ perm,
-                                                            int* __restrict__ list, int* __restrict__ info)
+// four wavefronts: lane i of every wavefront holds row i of the block, and
+// wavefront q its columns [16q, 16q + 16), in registers throughout. As in
+// getf2_reg_body, interchanges move no data (each lane carries its row's
+// current position; the pivot takes position j, the row at j takes the
+// pivot's), the pivot is the largest |x| with ties to the lower position,
+// a zero pivot leaves the rows in place, and the multipliers and updates
+// are formed the same way. Column j's wavefront finds the pivot (DPP
+// argmax), forms the multipliers and leaves them in LDS with the pivot's
+// lane and position; after one barrier every wavefront updates its columns
+// right of j, reading the pivot row's entries from its own registers by
+// readlane. Rows already pivoted (this step's pivot row included) take a
+// zero multiplier, which leaves them unchanged, so the update needs no
+// per-element select; unlike the register panel kernel, zero entries of
+// the pivot row are not skipped (x + (-0) l = x for the finite l of a
+// nonzero pivot). The rows are stored to their final positions at the
+// end; piv and perm (from row jb) and the move list (block-relative) are as
+// getf2_reg writes them, and info is min-reduced with the first zero-pivot
+// column.
+constexpr int LOCAL_WAVES = 4;
+constexpr int LOCAL_WC = LOCAL_NB / LOCAL_WAVES;  // columns per wavefront
+
+template <int WQ>
+__device__ __forceinline__ void local_block_body(float (&v)[LOCAL_WC], int lane, int& pos, bool& done, int& my_piv,
+                                                 int& first_zero, int jb, float (*lm)[LOCAL_NB], int (*pk)[4])
 {
-    constexpr int W = LOCAL_NB;
-    const int lane = threadIdx.x;
-    float* blk = a + jb + (size_t)jb * lda;
-    float v[W];
-#pragma unroll
-    for (int c = 0; c < W; c++) v[c] = blk[lane + (size_t)c * lda];
-    const int p0 = perm[jb + lane];
-    int pos = lane;
-    bool done = false;
-    int first_zero = -1;
-    int my_piv = 0;  // lane j keeps piv[jb + j]
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-        double bv = done ? -2.0 : pivot_key(v[j], pos, j);
-        int bi = done ? INT_MAX : pos;
-        wave_best_dpp(bv, bi);
-        const int pl = __ffsll((unsigned long long)__ballot(!done && pos == bi)) - 1;  // the pivot's lane
-        const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[j]), pl));
-        const bool zero_pivot = pv == 0.0f;
+    constexpr int W = LOCAL_NB, WC = LOCAL_WC;
+    auto column = [&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        constexpr int buf = j & 1;
+        if constexpr (j / WC == WQ) {
+            constexpr int cl = j % WC;
+            double bv = done ? -2.0 : pivot_key(v[cl], pos, j);
+            int bi = done ? INT_MAX : pos;
+            wave_best_dpp(bv, bi);
+            const int pl = __ffsll((unsigned long long)__ballot(!done && pos == bi)) - 1;  // the pivot's lane
+            const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[cl]), pl));
+            const bool zero_pivot = pv == 0.0f;
+            // Multipliers: x * (1/pivot) when |pivot| >= the safe minimum,
+            // x / pivot below it, x itself for a zero pivot.
+            float l = v[cl];
+            if (!zero_pivot) {
+                if (fabsf(pv) >= (float)safe_min<float>()) l = l * (1.0f / pv);
+                else l = l / pv;
+            }
+            const int pe = zero_pivot ? j : bi;
+            const bool now_done = done || (pos == pe);
+            lm[buf][lane] = now_done ? 0.0f : l;
+            if (!now_done) v[cl] = l;
+            if (lane == 0) {
+                pk[buf][0] = pl;
+                pk[buf][1] = bi;
+                pk[buf][2] = zero_pivot;
+            }
+        }
+        __syncthreads();
+        const int pl = pk[buf][0], bi = pk[buf][1];
+        const bool zero_pivot = pk[buf][2] != 0;
+        const float lmv = lm[buf][lane];
         if (lane == j) my_piv = jb + bi;
         if (zero_pivot && first_zero < 0) first_zero = jb + j;
         const int pe = zero_pivot ? j : bi;
@@ -2445,40 +2528,59 @@ __global__ void __launch_bounds__(64) local_block_lu_kernel(float* __restrict__ 
         const bool is_j = !done && pos == j && !is_p;
         done = done || is_p;
         pos = is_p ? j : (is_j ? pe : pos);
-        // Multipliers: x * (1/pivot) when |pivot| >= the safe minimum,
-        // x / pivot below it, x itself for a zero pivot.
-        float l = v[j];
-        if (!zero_pivot) {
-            if (fabsf(pv) >= (float)safe_min<float>()) l = l * (1.0f / pv);
-            else l = l / pv;
-        }
-        if (!done) v[j] = l;
-        // Rows already pivoted (this step's pivot row included) take a zero
-        // multiplier, which leaves them unchanged, so the update needs no
-        // per-element select; unlike the register panel kernel, zero
-        // entries of the pivot row are not skipped (x + (-0) l = x for the
-        // finite l of a nonzero pivot).
-        const float lm = done ? 0.0f : l;
 #pragma unroll
-        for (int c = j + 1; c < W; c++) {
-            const float u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[c]), pl));
-            v[c] = v[c] + (-u) * lm;
+        for (int cl = 0; cl < WC; cl++) {
+            if constexpr (true) {
+                const int c = WQ * WC + cl;
+                if (c > j) {
+                    const float u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[cl]), pl));
+                    v[cl] = v[cl] + (-u) * lmv;
+                }
+            }
         }
+    };
+    static_for<W>(column);
+}
+
+__global__ void __launch_bounds__(64 * LOCAL_WAVES) local_block_lu_kernel(float* __restrict__ a, int lda, int jb,
+                                                                         int* __restrict__ piv, int* __restrict__ perm,
+                                                                         int* __restrict__ list, int* __restrict__ info)
+{
+    constexpr int WC = LOCAL_WC;
+    __shared__ float lm[2][LOCAL_NB];
+    __shared__ int pk[2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float* blk = a + jb + (size_t)jb * lda;
+    float v[WC];
+#pragma unroll
+    for (int cl = 0; cl < WC; cl++) v[cl] = blk[lane + (size_t)(wave * WC + cl) * lda];
+    const int p0 = perm[jb + lane];
+    int pos = lane;
+    bool done = false;
+    int first_zero = -1;
+    int my_piv = 0;  // lane j keeps piv[jb + j]
+    switch (wave) {
+    case 0: local_block_body<0>(v, lane, pos, done, my_piv, first_zero, jb, lm, pk); break;
+    case 1: local_block_body<1>(v, lane, pos, done, my_piv, first_zero, jb, lm, pk); break;
+    case 2: local_block_body<2>(v, lane, pos, done, my_piv, first_zero, jb, lm, pk); break;
+    default: local_block_body<3>(v, lane, pos, done, my_piv, first_zero, jb, lm, pk); break;
     }
     __syncthreads();  // every lane's perm entry read before any is written
 #pragma unroll
-    for (int c = 0; c < W; c++) blk[pos + (size_t)c * lda] = v[c];
-    perm[jb + pos] = p0;
-    piv[jb + lane] = my_piv;
-    const unsigned long long moved = __ballot(pos != lane);
-    if (pos != lane) {
-        const int at = __popcll(moved & ((1ull << lane) - 1ull));
-        list[1 + 2 * at] = pos;
-        list[2 + 2 * at] = lane;
-    }
-    if (lane == 0) {
-        list[0] = __popcll(moved);
-        if (first_zero >= 0) atomicMin(info, first_zero);
+    for (int cl = 0; cl < WC; cl++) blk[pos + (size_t)(wave * WC + cl) * lda] = v[cl];
+    if (wave == 0) {
+        perm[jb + pos] = p0;
+        piv[jb + lane] = my_piv;
+        const unsigned long long moved = __ballot(pos != lane);
+        if (pos != lane) {
+            const int at = __popcll(moved & ((1ull << lane) - 1ull));
+            list[1 + 2 * at] = pos;
+            list[2 + 2 * at] = lane;
+        }
+        if (lane == 0) {
+            list[0] = __popcll(moved);
+            if (first_zero >= 0) atomicMin(info, first_zero);
+        }
     }
 }
 
@@ -2511,10 +2613,10 @@ void getrf_local_impl(Context* ctx, int n, T* a, int lda, int* d_piv, int* d_per
             const int w = NB < n - jb ? NB : n - jb;
             T* pa = a + jb + (size_t)jb * lda;
             if (w == NB) {
-                // One wavefront for the whole diagonal block, then its
+                // Four wavefronts for the whole diagonal block, then its
                 // interchanges across the rest of the rows.
-                hipLaunchKernelGGL(local_block_lu_kernel, dim3(1), dim3(64), 0, ctx->stream, a, lda, jb, d_piv,
-                                   d_perm, list.get(), d_info);
+                hipLaunchKernelGGL(local_block_lu_kernel, dim3(1), dim3(64 * LOCAL_WAVES), 0, ctx->stream, a, lda,
+                                   jb, d_piv, d_perm, list.get(), d_info);
                 apply_moves<T>(ctx, a + jb, lda, jb, jb + w, n, list, 2 * w);
             } else {
                 panel<T>(ctx, w, w, pa, lda, d_piv + jb, jb, d_perm + jb, rc.get() + jb, d_info, list, jb,
