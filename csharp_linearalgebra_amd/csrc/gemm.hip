@@ -78,11 +78,36 @@ __device__ __forceinline__ void a_coords(int e, int& i, int& kk)
     }
 }
 
+// Products with a triangular operand (op(A) untransposed): the k range of
+// the tile at (m0, n0) shrinks to where the triangle can be nonzero.
+// kmode 1: A lower (k < m0 + BM); 2: A upper (k >= m0); 3: B upper
+// (k < n0 + BN); 4: B lower (k >= n0). The range keeps at least one slab
+// and starts at a multiple of BK, so aligned operands stay aligned; the
+// parts of the k range it still covers beyond the triangle are zeros of
+// the operand, which add nothing.
+template <int BK, int TA, typename PA, typename PB>
+__device__ __forceinline__ void tri_k_range(int kmode, int m0, int n0, int BM, int BN, int& k, PA& A, int lda, PB& B)
+{
+    if (kmode == 0 || TA != 0) return;
+    if (kmode == 1 || kmode == 3) {
+        const int end = kmode == 1 ? m0 + BM : n0 + BN;
+        k = k < end ? k : end;
+    } else {
+        int kb = kmode == 2 ? m0 : n0;
+        const int last = (k - 1) / BK * BK;
+        kb = kb < last ? kb / BK * BK : last;
+        A += (size_t)kb * lda;
+        B += kb;
+        k -= kb;
+    }
+}
+
 template <int BM, int BN, int BK, int WM, int WN, int TA>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, (BM / WM) * (BN / WN) * 64 > 256 ? 1 : 2)
 dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, int lda,
              const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
-             int tiles_m, int tiles_n, int aligned, int kc, long long cs)
+             int tiles_m, int tiles_n, int aligned, int kc, long long cs, int kmode, long long sa, long long sb,
+             long long sc)
 {
     // Split over k (gridDim.y > 1): part blockIdx.y takes k in [y kc,
     // (y + 1) kc) and writes its product to its own C at C + y cs.
@@ -93,6 +118,11 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
         B += kbeg;
         C += blockIdx.y * cs;
     }
+    // A batch of products (gridDim.z > 1): product z at A + z sa, B + z sb,
+    // C + z sc.
+    A += blockIdx.z * sa;
+    B += blockIdx.z * sb;
+    C += blockIdx.z * sc;
     using S = Shape<BM, BN, BK, WM, WN>;
     using P = Planes<BM, BN, BK, TA>;
     __shared__ double As[P::A_SIZE];
@@ -101,6 +131,7 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
     int tmi, tni;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
     const int m0 = tmi * BM, n0 = tni * BN;
+    tri_k_range<BK, TA>(kmode, m0, n0, BM, BN, k, A, lda, B);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % S::WAVES_M, wn = wave / S::WAVES_M;
 
@@ -468,10 +499,16 @@ template <int BM, int BN, int BK, int WM, int WN, int TA>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, 2)
 zgemm_kernel(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A, int lda,
              const double2* __restrict__ B, int ldb, double br_, double bi_, double2* __restrict__ C,
-             int ldc, int flags, int tiles_m, int tiles_n)
+             int ldc, int flags, int tiles_m, int tiles_n, int kmode, long long sa, long long sb, long long sc)
 {
+    // A batch of products (gridDim.y > 1): product y at A + y sa, B + y sb,
+    // C + y sc.
+    A += blockIdx.y * sa;
+    B += blockIdx.y * sb;
+    C += blockIdx.y * sc;
     int tmi, tni;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tmi, tni);
+    tri_k_range<BK, TA>(kmode, tmi * BM, tni * BN, BM, BN, k, A, lda, B);
     zgemm_tile<BM, BN, BK, WM, WN, TA>(m, n, k, ar_, ai_, A, lda, B, ldb, br_, bi_, C, ldc, flags, tmi * BM,
                                        tni * BN);
 }
@@ -561,7 +598,7 @@ void launch_d(Context* ctx, int m, int n, int k, double alpha, const double* a, 
         DevBuf<double> w(ctx, (size_t)parts * m * n);
         hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tiles, parts),
                            dim3(Shape<BM, BN, BK, WM, WN>::NT), 0, ctx->stream, m, n, k, 1.0, a, lda, b, ldb, 0.0,
-                           w.get(), m, tm, tn, aligned, kc, (long long)m * n);
+                           w.get(), m, tm, tn, aligned, kc, (long long)m * n, 0, 0LL, 0LL, 0LL);
         const size_t total = (size_t)m * n;
         const int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
         hipLaunchKernelGGL(dsplit_reduce_kernel, dim3(blocks), dim3(256), 0, ctx->stream, m, n, parts, w.get(), alpha,
@@ -569,7 +606,23 @@ void launch_d(Context* ctx, int m, int n, int k, double alpha, const double* a, 
         return;
     }
     hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tiles), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
-                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn, aligned, k, 0LL);
+                       ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tm, tn, aligned, k, 0LL, 0, 0LL,
+                       0LL, 0LL);
+}
+
+// A batch of products of one shape at fixed strides, with an optional
+// triangular operand (tri_k_range), untransposed, in one launch.
+template <int BM, int BN, int BK, int WM, int WN>
+void launch_d_batch(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda, long long sa,
+                    const double* b, int ldb, long long sb, double beta, double* c, int ldc, long long sc, int batch,
+                    int kmode)
+{
+    const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    const int aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0 &&
+                        sa % 2 == 0 && sb % 2 == 0;
+    hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, 0>), dim3(tm * tn, 1, batch),
+                       dim3(Shape<BM, BN, BK, WM, WN>::NT), 0, ctx->stream, m, n, k, alpha, a, lda, b, ldb, beta, c,
+                       ldc, tm, tn, aligned, k, 0LL, kmode, sa, sb, sc);
 }
 
 template <int BM, int BN, int BK, int WM, int WN, int TA>
@@ -580,7 +633,19 @@ void launch_z(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int 
     hipLaunchKernelGGL((zgemm_kernel<BM, BN, BK, WM, WN, TA>), dim3(tm * tn), dim3(Shape<BM, BN, BK, WM, WN>::NT), 0,
                        ctx->stream, m, n, k, alpha.x, alpha.y, reinterpret_cast<const double2*>(a), lda,
                        reinterpret_cast<const double2*>(b), ldb, beta.x, beta.y, reinterpret_cast<double2*>(c), ldc,
-                       flags, tm, tn);
+                       flags, tm, tn, 0, 0LL, 0LL, 0LL);
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+void launch_z_batch(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda, long long sa,
+                    const cplx* b, int ldb, long long sb, cplx beta, cplx* c, int ldc, long long sc, int batch,
+                    int flags, int kmode)
+{
+    const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    hipLaunchKernelGGL((zgemm_kernel<BM, BN, BK, WM, WN, 0>), dim3(tm * tn, batch),
+                       dim3(Shape<BM, BN, BK, WM, WN>::NT), 0, ctx->stream, m, n, k, alpha.x, alpha.y,
+                       reinterpret_cast<const double2*>(a), lda, reinterpret_cast<const double2*>(b), ldb, beta.x,
+                       beta.y, reinterpret_cast<double2*>(c), ldc, flags, tm, tn, kmode, sa, sb, sc);
 }
 
 long tiles(int m, int n, int bm, int bn) { return (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn); }
@@ -675,6 +740,33 @@ void zgemm_ex(Context* ctx, int ta, int m, int n, int k, cplx alpha, const cplx*
     }
     if (ta == 0) { TNS_Z(0) } else if (ta == 1) { TNS_Z(1) } else { TNS_Z(2) }
 #undef TNS_Z
+    TNS_HIP(hipGetLastError());
+}
+
+void dgemm_batch(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda, long long sa,
+                 const double* b, int ldb, long long sb, double beta, double* c, int ldc, long long sc, int batch,
+                 int kmode)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return;
+    launch_d_batch<64, 64, 16, 32, 32>(ctx, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, kmode);
+    TNS_HIP(hipGetLastError());
+}
+
+void zgemm_batch(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda, long long sa, const cplx* b,
+                 int ldb, long long sb, cplx beta, cplx* c, int ldc, long long sc, int batch, int kmode)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return;
+    int flags = 0;
+    if (alpha.x == 1.0 && alpha.y == 0.0) flags |= 1;
+    if (alpha.y == 0.0) flags |= 2;
+    if (beta.x == 0.0 && beta.y == 0.0) flags |= 4;
+    if (beta.x == 1.0 && beta.y == 0.0) flags |= 8;
+    if (tiles(m, n, 64, 64) * batch >= 512)
+        launch_z_batch<64, 64, 32, 32, 32>(ctx, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, flags,
+                                           kmode);
+    else
+        launch_z_batch<64, 32, 32, 32, 16>(ctx, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, flags,
+                                           kmode);
     TNS_HIP(hipGetLastError());
 }
 

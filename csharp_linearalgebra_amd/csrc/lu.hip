@@ -1925,6 +1925,12 @@ struct Ops<double> {
     {
         dgemm_ex(c, ta, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
     }
+    static void batch(Context* c, int m, int n, int k, double alpha, const double* a, int lda, long long sa,
+                      const double* b, int ldb, long long sb, double beta, double* cc, int ldc, long long sc, int count,
+                      int kmode)
+    {
+        dgemm_batch(c, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, cc, ldc, sc, count, kmode);
+    }
     static double one() { return 1.0; }
     static double neg_one() { return -1.0; }
 };
@@ -1945,6 +1951,11 @@ struct Ops<zd> {
                      const zd* b, int ldb, zd beta, zd* cc, int ldc)
     {
         zgemm_ex(c, ta, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
+    }
+    static void batch(Context* c, int m, int n, int k, zd alpha, const zd* a, int lda, long long sa, const zd* b,
+                      int ldb, long long sb, zd beta, zd* cc, int ldc, long long sc, int count, int kmode)
+    {
+        zgemm_batch(c, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, cc, ldc, sc, count, kmode);
     }
     static zd one() { return make_zd(1.0, 0.0); }
     static zd neg_one() { return make_zd(-1.0, 0.0); }
@@ -2826,8 +2837,49 @@ void getrs_impl(Context* ctx, int trans, int n, int nrhs, const T* lu, int ldlu,
 // of the block. The products go between two buffers, so none is in place.
 // The rounding differs from the substitution's; the backward error of a
 // well-conditioned block's explicit inverse is of the same order.
+// Writes every diagonal block's two inverses (tri_inverse_lds_kernel's
+// layout: per block, inv(L_kk) then inv(U_kk), each BS x BS column-major)
+// into the diagonal blocks of the n x n matrices il and iu.
+template <typename T, int BS>
+__global__ void scatter_block_inverses_kernel(int n, const T* __restrict__ inv, T* __restrict__ il,
+                                              T* __restrict__ iu)
+{
+    const int blk = blockIdx.x, k0 = blk * BS;
+    const int kb = BS < n - k0 ? BS : n - k0;
+    const T* src = inv + (size_t)blk * 2 * BS * BS;
+    for (int e = threadIdx.x; e < BS * BS; e += blockDim.x) {
+        const int i = e % BS, j = e / BS;
+        if (i < kb && j < kb) {
+            il[(k0 + i) + (size_t)(k0 + j) * n] = src[e];
+            iu[(k0 + i) + (size_t)(k0 + j) * n] = src[BS * BS + e];
+        }
+    }
+}
+
 template <typename T>
-void getrs_blockinv_impl(Context* ctx, int n, int nrhs, const T* lu, int ldlu, const int* d_perm, T* b, int ldb)
+__global__ void zero_fill_kernel(size_t count, T* __restrict__ p)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        p[e] = s_zero(T{});
+}
+
+// X = (LU)^-1 P B for many right-hand sides (the Pade denominator's
+// n-column solve) by blocked substitution over diagonal blocks of order
+// HB = 128, with those blocks' inverses. The inverses of the BS-order
+// diagonal blocks come from one launch; pairs of them are merged into
+// inverses of twice the order until HB,
+//     inv(L) = [inv(L11) 0; -inv(L22) L21 inv(L11)  inv(L22)],
+//     inv(U) = [inv(U11) -inv(U11) U12 inv(U22); 0  inv(U22)],
+// each merge level two batched products over its pairs (the last, shorter
+// pair on its own), every product skipping the zero half of its
+// triangular factor. Then per HB block, forward: X_k = inv(L_kk) B_k,
+// B_rest -= L_rest,k X_k; backward likewise with U. Updates of depth HB
+// instead of BS: a quarter of the launches and of the passes over the
+// right-hand sides. The denominator is well conditioned at the degrees and
+// scalings the selection allows, so the explicit inverses cost no accuracy
+// the tests can see.
+template <typename T>
+void getrs_inverse_impl(Context* ctx, int n, int nrhs, const T* lu, int ldlu, const int* d_perm, T* b, int ldb)
 {
     if (n <= 0 || nrhs <= 0) return;
     constexpr int BS = tri_block<T>();
@@ -2835,30 +2887,65 @@ void getrs_blockinv_impl(Context* ctx, int n, int nrhs, const T* lu, int ldlu, c
     DevBuf<T> inv(ctx, (size_t)nblk * 2 * BS * BS);
     hipLaunchKernelGGL((tri_inverse_lds_kernel<T, BS>), dim3(nblk), dim3(2 * BS), 0, ctx->stream, n, lu, ldlu,
                        inv.get());
-    TNS_HIP(hipGetLastError());
-    DevBuf<T> t1(ctx, (size_t)n * nrhs), t2(ctx, (size_t)n * nrhs);
-    permute_rows<T>(ctx, n, nrhs, d_perm, b, ldb, t1, n, 0);
+    const size_t nn = (size_t)n * n;
+    DevBuf<T> il(ctx, nn), iu(ctx, nn), tmp(ctx, (size_t)n * (nrhs > n ? nrhs : n));
+    {
+        const int blocks = (int)((2 * nn + 255) / 256 < 8192 ? (2 * nn + 255) / 256 : 8192);
+        hipLaunchKernelGGL((zero_fill_kernel<T>), dim3(blocks), dim3(256), 0, ctx->stream, nn, il.get());
+        hipLaunchKernelGGL((zero_fill_kernel<T>), dim3(blocks), dim3(256), 0, ctx->stream, nn, iu.get());
+    }
+    hipLaunchKernelGGL((scatter_block_inverses_kernel<T, BS>), dim3(nblk), dim3(256), 0, ctx->stream, n, inv.get(),
+                       il.get(), iu.get());
     const T one = Ops<T>::one(), neg = Ops<T>::neg_one(), zero{};
-    // Forward with the unit lower triangle: t2_k = inv(L_kk) t1_k, then
-    // t1_{>k} -= L_{>k,k} t2_k.
-    for (int k = 0; k < nblk; k++) {
-        const int k0 = k * BS, kb = n - k0 < BS ? n - k0 : BS;
-        const T* il = inv.get() + (size_t)k * 2 * BS * BS;
-        Ops<T>::gemm(ctx, 0, kb, nrhs, kb, one, il, BS, t1.get() + k0, n, zero, t2.get() + k0, n);
+    T* IL = il.get();
+    T* IU = iu.get();
+    T* W = tmp.get();
+    constexpr int HB = 128;
+    for (int sz = BS; sz < HB && sz < n; sz *= 2) {
+        // Merge pairs of inverted blocks of order sz that lie inside one
+        // HB block (HB is a multiple of 2 sz, so every pair does).
+        const int full = n / (2 * sz);  // pairs of two whole blocks of order sz
+        auto level = [&](int lo, int r, int count) {
+            // Pair(s) from row/column lo: first block [lo, lo + sz), second
+            // [lo + sz, lo + sz + r); consecutive pairs 2 sz apart.
+            const int mid = lo + sz;
+            const long long sl = 2LL * sz * (1 + (long long)ldlu), sn = 2LL * sz * (1 + (long long)n);
+            // W21 = L21 inv(L11) (inv(L11) lower: B lower), then
+            // inv(L)21 = -inv(L22) W21 (A lower).
+            Ops<T>::batch(ctx, r, sz, sz, one, lu + mid + (size_t)lo * ldlu, ldlu, sl, IL + lo + (size_t)lo * n, n, sn,
+                          zero, W + mid + (size_t)lo * n, n, sn, count, 4);
+            Ops<T>::batch(ctx, r, sz, r, neg, IL + mid + (size_t)mid * n, n, sn, W + mid + (size_t)lo * n, n, sn, zero,
+                          IL + mid + (size_t)lo * n, n, sn, count, 1);
+            // W12 = inv(U11) U12 (A upper), then inv(U)12 = -W12 inv(U22)
+            // (B upper).
+            Ops<T>::batch(ctx, sz, r, sz, one, IU + lo + (size_t)lo * n, n, sn, lu + lo + (size_t)mid * ldlu, ldlu, sl,
+                          zero, W + lo + (size_t)mid * n, n, sn, count, 2);
+            Ops<T>::batch(ctx, sz, r, r, neg, W + lo + (size_t)mid * n, n, sn, IU + mid + (size_t)mid * n, n, sn, zero,
+                          IU + lo + (size_t)mid * n, n, sn, count, 3);
+        };
+        if (full > 0) level(0, sz, full);
+        const int lo = full * 2 * sz;
+        if (lo + sz < n) level(lo, n - lo - sz, 1);
+    }
+    // Blocked substitution over the HB blocks, ping-ponging t1 and W.
+    DevBuf<T> t1(ctx, (size_t)n * nrhs);
+    T* x1 = t1.get();
+    permute_rows<T>(ctx, n, nrhs, d_perm, b, ldb, x1, n, 0);
+    const int hblk = (n + HB - 1) / HB;
+    for (int k = 0; k < hblk; k++) {
+        const int k0 = k * HB, kb = n - k0 < HB ? n - k0 : HB;
+        Ops<T>::batch(ctx, kb, nrhs, kb, one, IL + k0 + (size_t)k0 * n, n, 0, x1 + k0, n, 0, zero, W + k0, n, 0, 1, 1);
         if (k0 + kb < n)
-            Ops<T>::gemm(ctx, 0, n - k0 - kb, nrhs, kb, neg, lu + (k0 + kb) + (size_t)k0 * ldlu, ldlu, t2.get() + k0, n,
-                         one, t1.get() + k0 + kb, n);
+            Ops<T>::gemm(ctx, 0, n - k0 - kb, nrhs, kb, neg, lu + (k0 + kb) + (size_t)k0 * ldlu, ldlu, W + k0, n, one,
+                         x1 + k0 + kb, n);
     }
-    // Backward with the upper triangle: t1_k = inv(U_kk) t2_k, then
-    // t2_{<k} -= U_{<k,k} t1_k.
-    for (int k = nblk - 1; k >= 0; k--) {
-        const int k0 = k * BS, kb = n - k0 < BS ? n - k0 : BS;
-        const T* iu = inv.get() + (size_t)k * 2 * BS * BS + BS * BS;
-        Ops<T>::gemm(ctx, 0, kb, nrhs, kb, one, iu, BS, t2.get() + k0, n, zero, t1.get() + k0, n);
+    for (int k = hblk - 1; k >= 0; k--) {
+        const int k0 = k * HB, kb = n - k0 < HB ? n - k0 : HB;
+        Ops<T>::batch(ctx, kb, nrhs, kb, one, IU + k0 + (size_t)k0 * n, n, 0, W + k0, n, 0, zero, x1 + k0, n, 0, 1, 2);
         if (k0 > 0)
-            Ops<T>::gemm(ctx, 0, k0, nrhs, kb, neg, lu + (size_t)k0 * ldlu, ldlu, t1.get() + k0, n, one, t2.get(), n);
+            Ops<T>::gemm(ctx, 0, k0, nrhs, kb, neg, lu + (size_t)k0 * ldlu, ldlu, x1 + k0, n, one, W, n);
     }
-    copy2d<T>(ctx, n, nrhs, t1, n, b, ldb);
+    copy2d<T>(ctx, n, nrhs, x1, n, b, ldb);
     TNS_HIP(hipGetLastError());
 }
 
@@ -2921,12 +3008,12 @@ void zgetrs(Context* ctx, int trans, int n, int nrhs, const cplx* lu, int ldlu, 
 
 void dgetrs_blockinv(Context* ctx, int n, int nrhs, const double* lu, int ldlu, const int* d_perm, double* b, int ldb)
 {
-    getrs_blockinv_impl<double>(ctx, n, nrhs, lu, ldlu, d_perm, b, ldb);
+    getrs_inverse_impl<double>(ctx, n, nrhs, lu, ldlu, d_perm, b, ldb);
 }
 
 void zgetrs_blockinv(Context* ctx, int n, int nrhs, const cplx* lu, int ldlu, const int* d_perm, cplx* b, int ldb)
 {
-    getrs_blockinv_impl<zd>(ctx, n, nrhs, lu, ldlu, d_perm, b, ldb);
+    getrs_inverse_impl<zd>(ctx, n, nrhs, lu, ldlu, d_perm, b, ldb);
 }
 
 void dtrsm(Context* ctx, int kind, int m, int n, const double* a, int lda, double* b, int ldb)

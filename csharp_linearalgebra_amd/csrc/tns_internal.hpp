@@ -155,6 +155,16 @@ void dgemm_ex(Context* ctx, int ta, int m, int n, int k, double alpha, const dou
               const double* b, int ldb, double beta, double* c, int ldc);
 void zgemm_ex(Context* ctx, int ta, int m, int n, int k, cplx alpha, const cplx* a, int lda,
               const cplx* b, int ldb, cplx beta, cplx* c, int ldc);
+// `batch` products of one shape, untransposed, product z at a + z sa,
+// b + z sb, c + z sc, in one launch. kmode marks a triangular operand, whose
+// zero part the tiles skip: 1 A lower, 2 A upper, 3 B upper, 4 B lower
+// (0 none); the zero half must be stored as zeros (the tiles skip most of
+// it, not all).
+void dgemm_batch(Context* ctx, int m, int n, int k, double alpha, const double* a, int lda, long long sa,
+                 const double* b, int ldb, long long sb, double beta, double* c, int ldc, long long sc, int batch,
+                 int kmode);
+void zgemm_batch(Context* ctx, int m, int n, int k, cplx alpha, const cplx* a, int lda, long long sa, const cplx* b,
+                 int ldb, long long sb, cplx beta, cplx* c, int ldc, long long sc, int batch, int kmode);
 // Tile configuration override for the autotuner: -1 = heuristic.
 void gemm_force_config(int real_cfg, int cplx_cfg);
 int gemm_config_count(bool complex_);
