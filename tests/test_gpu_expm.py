@@ -102,7 +102,8 @@ def test_expm_against_golden(ws, name):
 
 
 @pytest.mark.parametrize("n,norm,cplx", [(8, 0.4, False), (17, 12.0, False), (32, 30.0, False), (64, 100.0, False),
-                                         (50, 2.5, True), (120, 40.0, True), (300, 8.0, False), (300, 8.0, True)])
+                                         (50, 2.5, True), (120, 40.0, True), (300, 8.0, False), (300, 8.0, True),
+                                         (513, 20.0, False), (257, 20.0, True)])
 def test_expm_against_the_oracle(ws, n, norm, cplx):
     r = np.random.default_rng(n)
     a = r.standard_normal((n, n)) + (1j * r.standard_normal((n, n)) if cplx else 0)

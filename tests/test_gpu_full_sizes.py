@@ -129,6 +129,22 @@ def test_triangular_solves_at_large_order(ws, kind):
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("kind", range(6))
+def test_triangular_solves_many_right_hand_sides(ws, kind):
+    """More than 32 right-hand sides take the blocked path (diagonal blocks by
+    substitution, the coupling by GEMM) for every structure kind: the device
+    side of Structures.cs's UpperTriangular / LowerTriangular /
+    UnitLowerTriangular solves. Within 1e-12 of the oracle's column-oriented
+    substitution (the blocked coupling sums in another order)."""
+    n, nrhs = 700, 100
+    r = np.random.default_rng(100 + kind)
+    a = r.uniform(-1, 1, (n, n)) / n + np.eye(n) * (1.0 + r.uniform(0, 1, n))
+    b = r.uniform(-1, 1, (n, nrhs))
+    x = ws.solve_triangular(kind, a, b)
+    ref = O.trsv(kind, a, b)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("m,n", [(4500, 4500), (6000, 400), (16384, 64), (8192, 8192)])
 def test_dgetrf_tall_panels_pivots_equal_the_oracle(ws, m, n):
     r = np.random.default_rng(m + n)
