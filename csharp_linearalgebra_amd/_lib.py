@@ -69,6 +69,8 @@ SIGNATURES = {
     "tns_context_destroy": (_I, [_P]),
     "tns_last_error": (C.c_char_p, [_P]),
     "tns_context_set_limit": (_I, [_P, _SZ]),
+    "tns_context_set_retention": (_I, [_P, _SZ]),
+    "tns_context_memory": (_I, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "tns_context_stream": (_I, [_P, C.POINTER(C.c_void_p)]),
     "tns_synchronize": (_I, [_P]),
     "tns_version": (C.c_char_p, []),

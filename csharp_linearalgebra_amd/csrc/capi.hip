@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 #include <memory>
 #include <new>
 #include <numeric>
@@ -131,6 +132,23 @@ void Pool::trim()
         sizes_.erase(kv.second);
     }
     free_.clear();
+}
+
+// The retention cap of ComplexScratch (ComplexScratch.cs:12-60, 77-122):
+// what is kept between calls is bounded, whatever a call needed. Free blocks
+// go back largest first until the pool holds at most the cap (blocks still
+// lent are not touched; they come back with the next call's end).
+void Pool::shrink_to_retention()
+{
+    if (held_ <= retain_ || free_.empty()) return;
+    (void)hipDeviceSynchronize();
+    while (held_ > retain_ && !free_.empty()) {
+        auto it = std::prev(free_.end());
+        (void)hipFree(it->second);
+        held_ -= it->first;
+        sizes_.erase(it->second);
+        free_.erase(it);
+    }
 }
 
 // Frees every block, lent or not; a handle that later gives one back finds
@@ -334,11 +352,13 @@ int guarded(tns_context* ctx, F&& body)
     try {
         TNS_HIP(hipSetDevice(ctx->impl.device));
         body(&ctx->impl);
+        ctx->impl.pool.shrink_to_retention();
         ctx->impl.last_error.clear();
         return TNS_OK;
     } catch (const tns::Error& e) {
         ctx->impl.last_error = e.message;
         (void)hipStreamSynchronize(ctx->impl.stream);
+        ctx->impl.pool.shrink_to_retention();
         return e.status;
     } catch (const std::bad_alloc&) {
         ctx->impl.last_error = "host allocation failed";
@@ -784,6 +804,20 @@ TNS_API int tns_context_set_limit(tns_context* ctx, size_t bytes)
     return guarded(ctx, [&](Context* c) {
         c->pool.set_limit(bytes);
         c->pool.trim();
+    });
+}
+
+TNS_API int tns_context_set_retention(tns_context* ctx, size_t bytes)
+{
+    return guarded(ctx, [&](Context* c) { c->pool.set_retention(bytes); });
+}
+
+TNS_API int tns_context_memory(tns_context* ctx, size_t* held, size_t* limit, size_t* retention)
+{
+    return guarded(ctx, [&](Context* c) {
+        if (held) *held = c->pool.held();
+        if (limit) *limit = c->pool.limit();
+        if (retention) *retention = c->pool.retention();
     });
 }
 

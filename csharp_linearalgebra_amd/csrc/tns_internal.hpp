@@ -47,6 +47,11 @@ public:
     size_t held() const { return held_; }
     size_t limit() const { return limit_; }
     void set_limit(size_t l) { limit_ = l; }
+    // Bytes the pool keeps between calls: after a call, free blocks beyond
+    // it are returned to the device, the largest first.
+    size_t retention() const { return retain_; }
+    void set_retention(size_t r) { retain_ = r; }
+    void shrink_to_retention();
 
 private:
     static size_t round(size_t bytes);
@@ -54,6 +59,7 @@ private:
     std::map<void*, size_t> sizes_;
     size_t held_ = 0;
     size_t limit_;
+    size_t retain_ = ~size_t(0);
 };
 
 struct Context;

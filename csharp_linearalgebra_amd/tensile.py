@@ -171,6 +171,17 @@ class Workspace:
     def set_limit(self, nbytes: int):
         self.check(self._lib.tns_context_set_limit(self._h, int(nbytes)))
 
+    def set_retention(self, nbytes: int):
+        """Cap on the free device memory kept between calls (the retention
+        cap of ComplexScratch, ComplexScratch.cs:12-60)."""
+        self.check(self._lib.tns_context_set_retention(self._h, int(nbytes)))
+
+    def memory(self):
+        """(held, limit, retention) of the context's device pool, in bytes."""
+        held, limit, ret = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        self.check(self._lib.tns_context_memory(self._h, C.byref(held), C.byref(limit), C.byref(ret)))
+        return held.value, limit.value, ret.value
+
     def synchronize(self):
         self.check(self._lib.tns_synchronize(self._h))
 

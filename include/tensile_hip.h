@@ -104,6 +104,13 @@ TNS_API const char* tns_last_error(const tns_context* ctx);
 /* Ceiling on the device memory the context's pool may hold, the analogue of
  * TensileLimits.MaxElements (TensileLimits.cs:27-147). */
 TNS_API int tns_context_set_limit(tns_context* ctx, size_t bytes);
+/* Bytes of free device memory the pool keeps between calls (default: no
+ * cap below the limit); after each call, free blocks beyond it go back to
+ * the device, largest first. The retention cap of ComplexScratch's retained
+ * storage (ComplexScratch.cs:12-60, 77-122). */
+TNS_API int tns_context_set_retention(tns_context* ctx, size_t bytes);
+/* The pool's bytes held (lent and free), its limit and its retention cap. */
+TNS_API int tns_context_memory(tns_context* ctx, size_t* held, size_t* limit, size_t* retention);
 /* The context's HIP stream (hipStream_t), for callers chaining device work. */
 TNS_API int tns_context_stream(tns_context* ctx, void** stream);
 TNS_API int tns_synchronize(tns_context* ctx);

@@ -8,6 +8,8 @@
     TensileLimits.MaxElements, TensileLimits.cs:59-65) refuses with
     TNS_ENOMEM before the destination is touched, and the same call succeeds
     once the ceiling is raised;
+  * the retention cap (tns_context_set_retention, ComplexScratch's
+    RetainedLimit) bounds the free memory kept between calls;
   * host operands whose single column is longer than the 64 MiB page-locked
     staging buffer travel in pieces;
   * a device-resident ILinearOperator (LinearOperators.cs:37-75) drives the
@@ -54,6 +56,8 @@ def test_a_destroyed_context_refuses_every_entry_point(ws):
     cb = L.DAPPLY(lambda *args: 0)
     calls = {
         "tns_context_set_limit": (h, 1 << 20),
+        "tns_context_set_retention": (h, 1 << 20),
+        "tns_context_memory": (h, None, None, None),
         "tns_context_stream": (h, C.byref(C.c_void_p())),
         "tns_synchronize": (h,),
         "tns_dgemm": (h, n, n, n, 1.0, a, n, b, n, 0.0, c, n),
@@ -152,6 +156,29 @@ def test_the_memory_ceiling_refuses_before_touching_the_destination(ws):
     e, _ = w.expm(a, out=sentinel)
     ref, _ = O.expm(a)
     assert np.abs(e - ref).max() <= 1e-12 * np.abs(ref).max()
+    w.dispose()
+
+
+def test_the_retention_cap_bounds_what_the_pool_keeps(ws):
+    """ComplexScratch's retention cap (ComplexScratch.cs:12-60, 77-122): the
+    free device memory a context keeps between calls is bounded, whatever a
+    call needed; results are unchanged."""
+    w = T.Workspace(0)
+    n = 300
+    r = np.random.default_rng(5)
+    a = r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n))
+    e1, _ = w.expm(a)
+    held, limit, retention = w.memory()
+    assert held >= 5 * 16 * n * n  # the exponential's working matrices stay cached
+    assert retention >= limit
+    w.set_retention(1 << 20)
+    e2, _ = w.expm(a)
+    held2, _, retention2 = w.memory()
+    assert retention2 == 1 << 20 and held2 <= 1 << 20
+    w.set_retention(0)
+    e3, _ = w.expm(a)
+    assert w.memory()[0] == 0
+    assert np.array_equal(e1, e2) and np.array_equal(e1, e3)
     w.dispose()
 
 
