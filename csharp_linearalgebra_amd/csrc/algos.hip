@@ -24,13 +24,16 @@ namespace tns {
 double read_double(Context* ctx, const double* d)
 {
     double h = 0.0;
-    TNS_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, &h, d, sizeof(double));
     return h;
 }
 
 void read_doubles(Context* ctx, const double* d, double* h, int count)
 {
+    if ((size_t)count * sizeof(double) <= kMailBytes) {
+        read_back(ctx, h, d, sizeof(double) * count);
+        return;
+    }
     TNS_HIP(hipMemcpyAsync(h, d, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
     TNS_HIP(hipStreamSynchronize(ctx->stream));
 }
@@ -94,17 +97,22 @@ double column_one_norm(const T* x, int n)
     return s;
 }
 
+// An estimator round's transfers, through the page-locked panel area: the
+// upload is followed by the round's products and a download that waits,
+// before the next upload reuses the area.
 template <typename T>
 void upload(Context* ctx, T* d, const std::vector<T>& h)
 {
-    TNS_HIP(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    write_pinned(ctx, d, h.data(), sizeof(T) * h.size());
 }
 
 template <typename T>
 void download(Context* ctx, std::vector<T>& h, const T* d)
 {
-    TNS_HIP(hipMemcpyAsync(h.data(), d, sizeof(T) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    T* p = static_cast<T*>(pinned_panel(ctx, sizeof(T) * h.size()));
+    TNS_HIP(hipMemcpyAsync(p, d, sizeof(T) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     TNS_HIP(hipStreamSynchronize(ctx->stream));
+    std::copy(p, p + h.size(), h.begin());
 }
 
 template <typename T>
@@ -364,7 +372,11 @@ std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, co
     const int t = runs[0]->columns();
     const size_t panel = (size_t)n * t;
     DevBuf<T> dx(ctx, panel * R), dy0(ctx, panel * R), dy1(ctx, panel * R);
-    std::vector<T> hx(panel * R), hy0(panel * R), hy1(panel * R);
+    // Page-locked: each round uploads, multiplies and reads back with a wait
+    // before the next round's upload.
+    T* hx = static_cast<T*>(pinned_panel(ctx, sizeof(T) * panel * R * 3));
+    T* hy0 = hx + panel * R;
+    T* hy1 = hy0 + panel * R;
     std::vector<int> act;
     while (true) {
         act.clear();
@@ -376,8 +388,8 @@ std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, co
             if (runs[r]->want() != kind) fail(TNS_ESTATE, "norm estimates out of step");
         const int A = (int)act.size();
         for (int i = 0; i < A; i++)
-            std::copy(runs[act[i]]->input().begin(), runs[act[i]]->input().end(), hx.begin() + panel * i);
-        TNS_HIP(hipMemcpyAsync(dx.get(), hx.data(), sizeof(T) * panel * A, hipMemcpyHostToDevice, ctx->stream));
+            std::copy(runs[act[i]]->input().begin(), runs[act[i]]->input().end(), hx + panel * i);
+        TNS_HIP(hipMemcpyAsync(dx.get(), hx, sizeof(T) * panel * A, hipMemcpyHostToDevice, ctx->stream));
         const T* src = dx.get();
         T* bufs[2] = {dy0.get(), dy1.get()};
         for (int k = 1;; k++) {
@@ -395,13 +407,13 @@ std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, co
             }
             src = dst;
         }
-        TNS_HIP(hipMemcpyAsync(hy0.data(), dy0.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
-        TNS_HIP(hipMemcpyAsync(hy1.data(), dy1.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
+        TNS_HIP(hipMemcpyAsync(hy0, dy0.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
+        TNS_HIP(hipMemcpyAsync(hy1, dy1.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i < A; i++) {
             NormEstRun<T>& run = *runs[act[i]];
-            const std::vector<T>& hy = (powers[act[i]] & 1) ? hy1 : hy0;
-            std::copy(hy.begin() + panel * i, hy.begin() + panel * (i + 1), run.output().begin());
+            const T* hy = (powers[act[i]] & 1) ? hy1 : hy0;
+            std::copy(hy + panel * i, hy + panel * (i + 1), run.output().begin());
             run.resume();
         }
     }
@@ -541,8 +553,7 @@ void solve_pade(Context* ctx, int n, const T* u, const T* v, T* out, EventClock*
     if (clk) clk->mark(ExpmPhases::PADE);
     getrf_t<T>(ctx, n, den.p(), n, piv.get(), perm.get(), d_info);
     int info = 0;
-    TNS_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, &info, d_info, sizeof(int));
     if (clk) clk->mark(ExpmPhases::LU);
     if (info == LU_ABORTED) fail(TNS_EDEVICE, "the Pade denominator's LU panel exchange timed out");
     if (info >= 0)
@@ -631,13 +642,37 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
         TNS_HIP(hipStreamSynchronize(ctx->stream));
         return;
     }
-    product<T>(ctx, n, a.p(), a.p(), a2.p());
-    product<T>(ctx, n, a2.p(), a2.p(), a4.p());
-    product<T>(ctx, n, a2.p(), a4.p(), a6.p());
+    // d8 and d10 (the estimates of ||A^8|| and ||A^10|| the degree 7, 9 and
+    // 13 tests read) depend on A alone. When ||A||_1 >= theta_5 (so eta1 =
+    // max(d4, d6) may pass it and the estimates will be read) they run on
+    // the second stream while A^2, A^4 and A^6 are formed: the estimator's
+    // narrow products and read-backs are latency-bound and hide under the
+    // matrix-core products. The same runs, probes and values as when they
+    // follow the powers. Blocks the products release (split products'
+    // scratch) are held back from the pool until the main stream has
+    // finished with them, so the estimator never receives one.
+    std::vector<NormEst> d810;
+    bool have810 = false;
+    const bool speculate = ctx->side && !ctx->hold && one_norm<T>(ctx, n, a.p()) >= 2.539398330063230e-1;
+    std::vector<std::pair<void*, size_t>> held;
+    if (speculate) TNS_HIP(hipEventRecord(ctx->ev_panel, ctx->stream));
+    {
+        HoldReleases guard(ctx, speculate ? &held : ctx->hold);
+        product<T>(ctx, n, a.p(), a.p(), a2.p());
+        product<T>(ctx, n, a2.p(), a2.p(), a4.p());
+        product<T>(ctx, n, a2.p(), a4.p(), a6.p());
+    }
+    if (speculate) {
+        OnStream on(ctx, ctx->side);
+        TNS_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_panel, 0));
+        d810 = normest_powers<T>(ctx, n, a.p(), n, {8, 10}, 2, 5, 20260915);
+        have810 = true;
+    }
     norm_t<T>(ctx, '1', n, n, a4.p(), n, ctx->d_scal);
     norm_t<T>(ctx, '1', n, n, a6.p(), n, ctx->d_scal + 1);
     double nn[2];
     read_doubles(ctx, ctx->d_scal, nn, 2);
+    for (auto& h : held) ctx->pool.give(h.first, h.second);
     if (clk) clk->mark(ExpmPhases::POWERS);
     const double d4 = std::pow(nn[0], 1.0 / 4.0), d6 = std::pow(nn[1], 1.0 / 6.0);
     const double eta1 = std::fmax(d4, d6);
@@ -655,7 +690,7 @@ void expm(Context* ctx, int n, const T* ain, int lda, T* e, int lde, ExpmDiag* d
         // d8 and d10 in one lockstep run (d10 is read only past the degree 7
         // and 9 tests, as the reference computes it; estimating it with d8
         // changes no value, each estimate having its own probes).
-        const std::vector<NormEst> d810 = normest_powers<T>(ctx, n, a.p(), n, {8, 10}, 2, 5, 20260915);
+        if (!have810) d810 = normest_powers<T>(ctx, n, a.p(), n, {8, 10}, 2, 5, 20260915);
         const double d8 = std::pow(d810[0].value, 1.0 / 8.0);
         const double eta3 = std::fmax(d6, d8);
         if (eta3 < 9.504178996162932e-1 && ell<T>(ctx, n, a.p(), 7) == 0) {
@@ -843,8 +878,7 @@ void run(Context* ctx, const DevOp<T>& op, const T* dense, int lda, int nb, cons
                 taylor_exit(ctx, d_norms, d_state, kUnitRoundoff);
             }
             int hs[2] = {0, 0};
-            TNS_HIP(hipMemcpyAsync(hs, d_state, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-            TNS_HIP(hipStreamSynchronize(ctx->stream));
+            read_back(ctx, hs, d_state, sizeof(hs));
             if (diag) {
                 diag->applications += hs[1];
                 diag->early_exits += hs[0];
@@ -888,8 +922,7 @@ void expmv(Context* ctx, int n, int nb, const T* a, int lda, const T* b, int ldb
     // The trace shift (MatrixExponentialAction.cs:124-128, Complex.cs:53-60).
     trace<T>(ctx, n, a, lda, (T*)ctx->d_scal);
     T tr{};
-    TNS_HIP(hipMemcpyAsync(&tr, ctx->d_scal, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, &tr, ctx->d_scal, sizeof(T));
     T mu;
     if constexpr (sizeof(T) == 8) mu = tr / n;
     else mu = make_zd(tr.x / n, tr.y / n);
@@ -1006,8 +1039,7 @@ template <typename T>
 T read_scalar(Context* ctx, const T* d)
 {
     T h{};
-    TNS_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, &h, d, sizeof(T));
     return h;
 }
 

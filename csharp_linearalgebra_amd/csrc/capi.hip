@@ -164,6 +164,45 @@ void Pool::release_all()
 
 Pool::~Pool() { release_all(); }
 
+void read_back(Context* ctx, void* h, const void* d, size_t bytes)
+{
+    if (bytes > kMailBytes) fail(TNS_EINVAL, "read_back: more than the mailbox holds");
+    TNS_HIP(hipMemcpyAsync(ctx->h_mail, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(h, ctx->h_mail, bytes);
+}
+
+void* pinned_panel(Context* ctx, size_t bytes)
+{
+    if (ctx->pin_pending) {
+        TNS_HIP(hipEventSynchronize(ctx->ev_pin));
+        ctx->pin_pending = false;
+    }
+    if (ctx->h_pin_bytes < bytes) {
+        // Growing frees the old area: its last copies must be done.
+        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_pin) TNS_HIP(hipHostFree(ctx->h_pin));
+        ctx->h_pin = nullptr;
+        ctx->h_pin_bytes = 0;
+        void* h = nullptr;
+        const size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
+        TNS_HIP(hipHostMalloc(&h, want, hipHostMallocDefault));
+        ctx->h_pin = static_cast<char*>(h);
+        ctx->h_pin_bytes = want;
+    }
+    return ctx->h_pin;
+}
+
+void write_pinned(Context* ctx, void* d, const void* h, size_t bytes)
+{
+    void* p = pinned_panel(ctx, bytes);
+    std::memcpy(p, h, bytes);
+    TNS_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    TNS_HIP(hipEventRecord(ctx->ev_pin, ctx->stream));
+    ctx->pin_pending = true;
+}
+
+
 bool is_device_pointer(const void* p)
 {
     if (!p) return false;
@@ -423,9 +462,13 @@ void getrf_body(Context* ctx, int m, int n, T* a, int lda, int nb, int* pivots, 
     }
     int sing = -1;
     double range[2] = {0.0, 0.0};
-    TNS_HIP(hipMemcpyAsync(&sing, ctx->d_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    if (limit > 0) TNS_HIP(hipMemcpyAsync(range, ctx->d_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // Through the page-locked mailbox: the pivot range at 0, the info at 16.
+    if (limit > 0)
+        TNS_HIP(hipMemcpyAsync(ctx->h_mail, ctx->d_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TNS_HIP(hipMemcpyAsync(ctx->h_mail + 16, ctx->d_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     tns::sync(ctx);
+    std::memcpy(&sing, ctx->h_mail + 16, sizeof(int));
+    if (limit > 0) std::memcpy(range, ctx->h_mail, 2 * sizeof(double));
     // Checked before any output leaves the device.
     if (sing == tns::LU_ABORTED) fail(TNS_EDEVICE, "an LU panel's cross-workgroup exchange timed out");
     if (limit > 0) {
@@ -729,6 +772,8 @@ TNS_API int tns_context_create(int device, tns_context** out)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[1], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c.d_scal, 64 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c.d_int, 64 * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c.h_mail, tns::kMailBytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_pin, hipEventDisableTiming);
     if (e == hipSuccess) {
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
@@ -746,9 +791,10 @@ TNS_API int tns_context_create(int device, tns_context** out)
         (void)hipGetLastError();
         if (c.d_scal) (void)hipFree(c.d_scal);
         if (c.d_int) (void)hipFree(c.d_int);
+        if (c.h_mail) (void)hipHostFree(c.h_mail);
         if (c.stream) (void)hipStreamDestroy(c.stream);
         if (c.side) (void)hipStreamDestroy(c.side);
-        for (hipEvent_t ev : {c.ev_panel, c.ev_rest[0], c.ev_rest[1]})
+        for (hipEvent_t ev : {c.ev_panel, c.ev_rest[0], c.ev_rest[1], c.ev_pin})
             if (ev) (void)hipEventDestroy(ev);
         delete ctx;
         return e == hipErrorOutOfMemory ? TNS_ENOMEM : TNS_EDEVICE;
@@ -778,11 +824,17 @@ TNS_API int tns_context_destroy(tns_context* ctx)
     if (c.h_stage) (void)hipHostFree(c.h_stage);
     c.h_stage = nullptr;
     c.h_stage_bytes = 0;
+    if (c.h_pin) (void)hipHostFree(c.h_pin);
+    c.h_pin = nullptr;
+    c.h_pin_bytes = 0;
+    if (c.h_mail) (void)hipHostFree(c.h_mail);
+    c.h_mail = nullptr;
     (void)hipFree(c.d_scal);
     (void)hipFree(c.d_int);
     (void)hipStreamDestroy(c.stream);
     (void)hipStreamDestroy(c.side);
-    for (hipEvent_t ev : {c.ev_panel, c.ev_rest[0], c.ev_rest[1]}) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {c.ev_panel, c.ev_rest[0], c.ev_rest[1], c.ev_pin}) (void)hipEventDestroy(ev);
+    c.ev_pin = nullptr;
     c.side = nullptr;
     c.ev_panel = c.ev_rest[0] = c.ev_rest[1] = nullptr;
     c.d_scal = nullptr;

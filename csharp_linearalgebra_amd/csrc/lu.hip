@@ -1209,6 +1209,104 @@ __global__ void __launch_bounds__(256) trsm_llnu_wave_kernel(int w, int ncols, c
     }
 }
 
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d_t mfma_f64x4(double a, double b, v4d_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// One look-ahead step's update of the next panel's nc columns in one launch:
+// A22 -= L21 U12 for the m2 rows below the block row, with U12 = L11^-1 A12
+// (L11 the unit lower w x w triangle at pa, A12 at b12, L21 at pa + w, A22
+// at b12 + w). Every workgroup forms U12 in LDS from A12, one wavefront per
+// column with x_k by readlane (the operations and order of
+// trsm_llnu_wave_kernel), and leaves A12 as it is: a workgroup storing U12
+// there could overtake another's read of A12. The caller solves A12 in
+// place later (trsm_llnu, the same bits); nothing reads those rows before.
+// Sixteen wavefronts: four columns each for the solve, then one 16 x 16
+// block each of the workgroup's 64 rows of A22 on the matrix cores, with
+// the operand mapping, k order (groups of
+// four from k = 0, zeros past w), MFMA sequence (complex: the 4M order of
+// zgemm_tile) and write-back (C + (-acc)) of the GEMM kernels, so every
+// element takes the bits the solve and the product give it: the launch
+// replaces those two on the critical path, and the gap between them.
+template <typename T, int W>
+__global__ void __launch_bounds__(1024) next_panel_update_kernel(int m2, int w, int nc, const T* __restrict__ pa,
+                                                                 int lda, T* __restrict__ b12)
+{
+    constexpr bool CPLX = sizeof(T) == 16;
+    constexpr int NWAVE = 16;
+    constexpr int TN = W / 16;  // 16-column blocks of the widest update
+    __shared__ T Ls[W][W];      // Ls[k][i] = L(i, k) below the diagonal
+    __shared__ T Us[W][W + 1];  // Us[c][k] = U12(k, c)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < W * W; e += 64 * NWAVE) {
+        const int i = e % W, k = e / W;
+        Ls[k][i] = load_if(pa + i + (size_t)k * lda, i < w && k < w && i > k);
+    }
+    __syncthreads();
+    const bool live = lane < w;
+    for (int c = wave; c < nc; c += NWAVE) {
+        T x = load_if(b12 + lane + (size_t)c * lda, live);
+#pragma unroll
+        for (int k = 0; k < W - 1; k++) {
+            if (k + 1 >= w) break;
+            const T xk = lane_value(x, k);
+            if (!s_iszero(xk)) {
+                const T d = Ls[k][lane < W ? lane : 0];
+                if (lane > k && live) x = s_add(x, s_mul(s_neg(xk), d));
+            }
+        }
+        if (live) Us[c][lane] = x;
+    }
+    for (int e = tid; e < W * W; e += 64 * NWAVE) {
+        // Columns past nc and rows past w are the products' zero padding.
+        const int k = e % W, c = e / W;
+        if (c >= nc || k >= w) Us[c][k] = s_zero(T{});
+    }
+    __syncthreads();
+    // Wavefront (rb, j): rows r0 + 16 rb of the workgroup's 64, columns 16 j.
+    const int rb = wave & 3, j = wave >> 2;
+    const int r0 = blockIdx.x * 64 + rb * 16;
+    if (j >= TN || 16 * j >= nc || r0 >= m2) return;
+    const T* l21 = pa + w;
+    T* a22 = b12 + w;
+    const int kq = lane >> 4, col_in = lane & 15;
+    const int arow = r0 + col_in;
+    v4d_t re = v4d_t{0.0, 0.0, 0.0, 0.0}, im = v4d_t{0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < w; kk += 4) {
+        const int k = kk + kq;
+        const T av = load_if(l21 + (arow < m2 ? arow : 0) + (size_t)(k < w ? k : 0) * lda, arow < m2 && k < w);
+        const T bv = Us[16 * j + col_in][k < W ? k : 0];
+        if constexpr (CPLX) {
+            const double a_r = av.x, a_i = av.y, a_n = -a_i;
+            re = mfma_f64x4(a_r, bv.x, re);
+            im = mfma_f64x4(a_r, bv.y, im);
+            re = mfma_f64x4(a_n, bv.y, re);
+            im = mfma_f64x4(a_i, bv.x, im);
+        } else {
+            re = mfma_f64x4(av, bv, re);
+        }
+    }
+    // D layout of v_mfma_f64_16x16x4_f64: lane l, register r holds row
+    // 4r + l/16, column l%16 of the block.
+    const int col = 16 * j + col_in;
+    if (col >= nc) return;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int row = r0 + 4 * r + kq;
+        if (row >= m2) continue;
+        T* p = a22 + row + (size_t)col * lda;
+        if constexpr (CPLX) {
+            const T c0 = *p;
+            *p = make_zd(c0.x + (-re[r]), c0.y + (-im[r]));
+        } else {
+            *p = *p + (-re[r]);
+        }
+    }
+}
+
 // U12 := L11^-1 A12 of many matrices in one launch.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -2247,17 +2345,6 @@ void panel_spread(Context* ctx, int m, int w, T* a, int lda, int* piv, int off, 
     }
 }
 
-// Points ctx->stream at s for the guard's life: the helpers launch on
-// ctx->stream.
-struct OnStream {
-    Context* ctx;
-    hipStream_t saved;
-    OnStream(Context* c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
-    ~OnStream() { ctx->stream = saved; }
-    OnStream(const OnStream&) = delete;
-    OnStream& operator=(const OnStream&) = delete;
-};
-
 // getrf_impl's blocked loop with look-ahead (spread panels, panel width ib
 // inside outer steps of ob columns). Step k:
 //   on ctx->stream: factor panel k; once the side stream's step k - 1 is
@@ -2285,6 +2372,18 @@ void getrf_lookahead(Context* ctx, int m, int n, T* a, int lda, int ib, int ob, 
         const int limit = m < n ? m : n;
         hipStream_t side = ctx->side;
         int step = 0;
+        // A block row left unsolved by next_panel_update_kernel: solved in
+        // place on the next step's side stream (after the event that follows
+        // the kernel), or at the end.
+        struct Pending {
+            int w = 0, nc = 0;
+            const T* l = nullptr;
+            T* b = nullptr;
+        } pending;
+        auto flush_pending = [&]() {
+            if (pending.nc > 0) trsm_llnu<T>(ctx, pending.w, pending.nc, pending.l, lda, pending.b, lda);
+            pending = Pending{};
+        };
         for (int jo = 0; jo < limit; jo += ob) {
             const int wo = ob < limit - jo ? ob : limit - jo;
             const int end = jo + wo < n ? jo + wo : n;  // columns the inner steps update
@@ -2314,6 +2413,26 @@ void getrf_lookahead(Context* ctx, int m, int n, T* a, int lda, int ib, int ob, 
                 // product inside the outer step, the outer step's past it.
                 auto update = [&](int lo, int hi) {
                     if (hi <= lo) return;
+                    // The step's own block width (the panel's, or at an outer
+                    // step's last panel the outer step's) and top row.
+                    const int bw = last ? wo : w, top = last ? jo : jb;
+                    if constexpr (sizeof(T) == 16) {
+                        // Complex: the next panel's columns in one launch; their
+                        // block row's in-place solve follows on the side stream.
+                        // (Measured: the Expm step's LU 3.69 -> 3.58 ms. The real
+                        // LU keeps the two launches: its 64-column form needs
+                        // 1024-thread, 66 KB workgroups, which wait for CUs while
+                        // the side stream's product fills them: n = 8192 41.5 ->
+                        // 43.1 ms, n = 1024 2.76 -> 2.99 ms.)
+                        if (bw <= NB && hi - lo <= NB && ctx->stream != side && m > top + bw) {
+                            const int m2 = m - top - bw;
+                            hipLaunchKernelGGL((next_panel_update_kernel<T, NB>), dim3((m2 + 63) / 64), dim3(1024),
+                                               0, ctx->stream, m2, bw, hi - lo, last ? po : pa, lda,
+                                               a + top + (size_t)lo * lda);
+                            pending = Pending{bw, hi - lo, last ? po : pa, a + top + (size_t)lo * lda};
+                            return;
+                        }
+                    }
                     if (!last) {
                         trsm_llnu<T>(ctx, w, hi - lo, pa, lda, a + jb + (size_t)lo * lda, lda);
                         if (jb + w < m)
@@ -2332,6 +2451,7 @@ void getrf_lookahead(Context* ctx, int m, int n, T* a, int lda, int ib, int ob, 
                     // The rest, on the side stream.
                     OnStream on(ctx, side);
                     TNS_HIP(hipStreamWaitEvent(side, ctx->ev_panel, 0));
+                    flush_pending();
                     apply_moves<T>(ctx, a + jb, lda, jo, c2, n, list, moves);
                     update(c2, last ? n : end);
                     TNS_HIP(hipEventRecord(ctx->ev_rest[step & 1], side));
@@ -2340,6 +2460,7 @@ void getrf_lookahead(Context* ctx, int m, int n, T* a, int lda, int ib, int ob, 
             }
         }
         if (step > 0) TNS_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_rest[(step - 1) & 1], 0));
+        flush_pending();
     }
 }
 

@@ -376,7 +376,7 @@ int fgmres_cycle(const LowSolve& M, const double* a, int lda, const double* r, d
         double* vk = V + (size_t)k * vs;
         double* w = V + (size_t)(k + 1) * vs;
         double* zk = Z + (size_t)k * vs;
-        TNS_HIP(hipMemcpyAsync(dsc, scale.data(), sizeof(double) * t, hipMemcpyHostToDevice, ctx->stream));
+        write_pinned(ctx, dsc, scale.data(), sizeof(double) * t);
         hipLaunchKernelGGL(col_mul_kernel, rows, dim3(256), 0, ctx->stream, n, vk, n, dsc);
         M.apply(vk, n, zk, n, false);
         dgemm_ex(ctx, 0, n, t, n, 1.0, a, lda, zk, n, 0.0, w, n);
@@ -448,8 +448,7 @@ int fgmres_cycle(const LowSolve& M, const double* a, int lda, const double* r, d
         }
     }
     if (kmax > 0) {
-        TNS_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
-        TNS_HIP(hipStreamSynchronize(ctx->stream));  // y is a host temporary
+        write_pinned(ctx, dy, y.data(), sizeof(double) * y.size());
         hipLaunchKernelGGL(block_combine_kernel, rows, dim3(256), 0, ctx->stream, n, Z, vs, kmax, dy, m, 1, x, ldx);
         TNS_HIP(hipGetLastError());
     }
@@ -567,8 +566,7 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
             sgetrf_local(ctx, n4, s.get(), n4, piv.get(), perm.get(), inv.get(), dinfo.get());
         }
         int hinfo = -1;
-        TNS_HIP(hipMemcpyAsync(&hinfo, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        read_back(ctx, &hinfo, dinfo.get(), sizeof(int));
         out.factor_ms = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
         singular = hinfo >= 0;
 
@@ -631,8 +629,7 @@ void dgesv_mixed(Context* ctx, int n, int t, const double* a, int lda, const dou
         DevBuf<int> piv(ctx, n), perm(ctx, n), dinfo(ctx, 1);
         dgetrf(ctx, n, n, f.get(), n, 0, piv.get(), perm.get(), dinfo.get());
         int hinfo = -1;
-        TNS_HIP(hipMemcpyAsync(&hinfo, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        TNS_HIP(hipStreamSynchronize(ctx->stream));
+        read_back(ctx, &hinfo, dinfo.get(), sizeof(int));
         if (hinfo == LU_ABORTED) fail(TNS_EDEVICE, "the FP64 fallback's panel exchange timed out");
         if (hinfo >= 0) fail(TNS_ESTATE, "the matrix is singular: the FP64 fallback met a zero pivot");
         dcopy2d(ctx, n, t, b, ldb, x, ldx);

@@ -116,10 +116,42 @@ struct Context {
     // kStageMax bytes; larger matrices pass through it in column chunks).
     char* h_stage = nullptr;
     size_t h_stage_bytes = 0;
+    // Page-locked host memory for read-backs and the estimators' rounds: a
+    // copy between pageable memory and the device costs 70-170 us of wall
+    // time for a few kilobytes, against a few for page-locked memory.
+    // h_mail (kMailBytes, fixed) takes scalar read-backs; h_pin (grow-only)
+    // the panels of one estimator or refinement loop at a time.
+    char* h_mail = nullptr;
+    char* h_pin = nullptr;
+    size_t h_pin_bytes = 0;
+    hipEvent_t ev_pin = nullptr;  // the last upload from h_pin
+    bool pin_pending = false;
     std::string last_error;
     std::mutex gate;            // the Workspace lock (Workspace.cs:136-139)
     // This device's copy of the no-allocation LU's file-scope scratch.
     int* noalloc_scratch = nullptr;
+};
+
+// Points ctx->stream at s for the guard's life: the helpers launch on
+// ctx->stream.
+struct OnStream {
+    Context* ctx;
+    hipStream_t saved;
+    OnStream(Context* c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
+    ~OnStream() { ctx->stream = saved; }
+    OnStream(const OnStream&) = delete;
+    OnStream& operator=(const OnStream&) = delete;
+};
+
+// Points ctx->hold at v for the guard's life: buffers released meanwhile
+// are kept there instead of returning to the pool.
+struct HoldReleases {
+    Context* ctx;
+    std::vector<std::pair<void*, size_t>>* saved;
+    HoldReleases(Context* c, std::vector<std::pair<void*, size_t>>* v) : ctx(c), saved(c->hold) { c->hold = v; }
+    ~HoldReleases() { ctx->hold = saved; }
+    HoldReleases(const HoldReleases&) = delete;
+    HoldReleases& operator=(const HoldReleases&) = delete;
 };
 
 template <typename T>
@@ -139,6 +171,17 @@ void DevBuf<T>::release()
     p_ = nullptr;
     bytes_ = 0;
 }
+
+constexpr size_t kMailBytes = 4096;
+// Device -> host for at most kMailBytes through the page-locked mailbox;
+// waits for the copy (and everything before it on ctx->stream).
+void read_back(Context* ctx, void* h, const void* d, size_t bytes);
+// The page-locked panel area, grown to at least `bytes`, once the last
+// upload from it (write_pinned) has been read; the pointer stays valid
+// until the next call.
+void* pinned_panel(Context* ctx, size_t bytes);
+// Host -> device through the panel area, asynchronous on ctx->stream.
+void write_pinned(Context* ctx, void* d, const void* h, size_t bytes);
 
 // Whether p is device (or managed) memory this process can hand to a kernel.
 bool is_device_pointer(const void* p);
