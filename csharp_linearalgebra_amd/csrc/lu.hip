@@ -1556,8 +1556,9 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
     for (int i = tid; i < n; i += NT) xg[i] = xs[i];
 }
 
-// One step of a few-right-hand-side solve op(A) X = B, A lower unit
-// (forward) or upper (backward), not transposed, spread over the chip: the
+// One step of a few-right-hand-side solve op(A) X = B, op(A) lower
+// (forward) or upper (backward), op(A) = A, A^T (TR 1) or A^H (TR 2),
+// spread over the chip: the
 // rows still to come are split into slabs of 256, one workgroup each, and
 // every row takes block [k0, k0 + kb)'s final x in the order trsv_lds_kernel
 // applies it (column by column, skipping zero entries of x); then the
@@ -1567,10 +1568,17 @@ trsv_lds_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int 
 // computed by the same operations in the same order as trsv_lds_kernel, but
 // the updates read the triangle with the whole chip instead of one
 // workgroup per column. Columns blockIdx.y * NC ...
-template <typename T, bool LOWER, bool UNIT, int NC>
+template <typename T, bool LOWER, int TR, bool UNIT, int NC>
 __global__ void __launch_bounds__(256) trsv_step_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B,
                                                         int ldb, int nrhs, int k0, int kb, int next0, int nextb)
 {
+    // op(A)(i, k), read where the stored triangle keeps it (a transposed
+    // form reads along the stored row, contiguous in the row update).
+    auto opa = [&](int i, int k, bool ok) __attribute__((always_inline)) {
+        T v = load_if(TR == 0 ? A + i + (size_t)k * lda : A + k + (size_t)i * lda, ok);
+        if (TR == 2) v = s_conj(v);
+        return v;
+    };
     constexpr int BS = tri_block<T>();
     constexpr int U = sizeof(T) == 16 ? 4 : 8;
     static_assert(NC <= 4, "one wavefront per column for the diagonal block");
@@ -1598,7 +1606,7 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(int n, const T* __restri
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const int p = LOWER ? q + u : kb - 1 - (q + u);
-                    av[u] = load_if(A + i + (size_t)(k0 + p) * lda, q + u < kb);
+                    av[u] = opa(i, k0 + p, q + u < kb);
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
@@ -1626,7 +1634,7 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(int n, const T* __restri
         const bool mine = lane < nextb;
         T d[BS];
 #pragma unroll
-        for (int k = 0; k < BS; k++) d[k] = load_if(A + (next0 + lane) + (size_t)(next0 + k) * lda, mine && k < nextb);
+        for (int k = 0; k < BS; k++) d[k] = opa(next0 + lane, next0 + k, mine && k < nextb);
         T xi = load_if(x + next0 + lane, mine);
 #pragma unroll
         for (int st = 0; st < BS; st++) {
@@ -2166,11 +2174,19 @@ void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ld
 }
 
 // op(A) X = B in place, A n x n triangular.
+// Order from which few-right-hand-side solves take trsv_step_kernel (one
+// launch per diagonal block, the triangle read by the whole chip) instead of
+// one workgroup per column. Measured at 1024 (2 and 16 right-hand sides):
+// the 16 dependent launches cost more than one workgroup streams (gecon
+// 2.1 -> 2.8 ms); at 8192 the step form is 5x faster.
+constexpr int TRSV_STEP_FROM = 2048;
+
 template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb)
 {
     if (n <= 0 || nrhs <= 0) return;
-    if ((nrhs <= 32 || n <= 64) && TR == 0 && n >= 2048) {
+    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    if ((nrhs <= 32 || n <= 64) && n >= TRSV_STEP_FROM) {
         // Spread over the chip, one launch per diagonal block (the same
         // operations per element as trsv_lds_kernel).
         constexpr int BS = tri_block<T>(), NC = 4;
@@ -2180,14 +2196,14 @@ void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb
         for (int bq = 0; bq <= nblk; bq++) {
             int next0 = 0, nextb = 0;
             if (bq < nblk) {
-                const int blk = LOWER ? bq : nblk - 1 - bq;
+                const int blk = FORWARD ? bq : nblk - 1 - bq;
                 next0 = blk * BS;
                 nextb = n - next0 < BS ? n - next0 : BS;
             }
-            const int rows = kb == 0 ? 0 : (LOWER ? n - k0 - kb : k0);
+            const int rows = kb == 0 ? 0 : (FORWARD ? n - k0 - kb : k0);
             if (nextb == 0 && rows == 0) break;
             const int slabs = rows > 0 ? (rows + 255) / 256 : 1;
-            hipLaunchKernelGGL((trsv_step_kernel<T, LOWER, UNIT, NC>), dim3(slabs, gy), dim3(256), 0, ctx->stream, n, A,
+            hipLaunchKernelGGL((trsv_step_kernel<T, FORWARD, TR, UNIT, NC>), dim3(slabs, gy), dim3(256), 0, ctx->stream, n, A,
                                lda, B, ldb, nrhs, k0, kb, next0, nextb);
             k0 = next0;
             kb = nextb;
@@ -2211,7 +2227,6 @@ void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb
         }
         return;
     }
-    constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
     constexpr int BS = tri_block<T>();
     const int nblk = (n + BS - 1) / BS;
     const int ta = TR;
