@@ -62,12 +62,15 @@ hipGraphExec_t capture(Context* ctx, std::vector<std::pair<void*, size_t>>* hold
     hipGraph_t graph = nullptr;
     TNS_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
     ctx->hold = hold;
+    ctx->capturing = true;
     try {
         body();
+        ctx->capturing = false;
         TNS_HIP(hipStreamEndCapture(origin, &graph));
     } catch (...) {
         ctx->stream = origin;
         ctx->hold = nullptr;
+        ctx->capturing = false;
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(origin, &g);
         if (g) (void)hipGraphDestroy(g);

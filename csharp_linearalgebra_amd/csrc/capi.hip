@@ -384,6 +384,16 @@ void require(bool cond, const char* what)
 template <typename F>
 int guarded(tns_context* ctx, F&& body)
 {
+    // After a call: the pool keeps at most its retention in free blocks; the
+    // solve's exchange area counts as kept memory and goes back first.
+    auto settle = [](Context* c) {
+        if (c->flow && c->pool.held() > c->pool.retention()) {
+            c->pool.give(c->flow, c->flow_bytes);
+            c->flow = nullptr;
+            c->flow_bytes = 0;
+        }
+        c->pool.shrink_to_retention();
+    };
     if (!ctx) return TNS_EINVAL;
     if (ctx->disposed.load()) return TNS_EDISPOSED;
     std::lock_guard<std::mutex> lock(ctx->impl.gate);
@@ -391,13 +401,13 @@ int guarded(tns_context* ctx, F&& body)
     try {
         TNS_HIP(hipSetDevice(ctx->impl.device));
         body(&ctx->impl);
-        ctx->impl.pool.shrink_to_retention();
+        settle(&ctx->impl);
         ctx->impl.last_error.clear();
         return TNS_OK;
     } catch (const tns::Error& e) {
         ctx->impl.last_error = e.message;
         (void)hipStreamSynchronize(ctx->impl.stream);
-        ctx->impl.pool.shrink_to_retention();
+        settle(&ctx->impl);
         return e.status;
     } catch (const std::bad_alloc&) {
         ctx->impl.last_error = "host allocation failed";
@@ -767,9 +777,13 @@ TNS_API int tns_context_create(int device, tns_context** out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_panel, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[0], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[1], hipEventDisableTiming);
+    // The two streams' ordering events: only this device's streams wait on
+    // them, so their record needs no system-scope fence (the cache write-back
+    // it costs sat on the LU's critical path, once per panel).
+    const unsigned order_flags = hipEventDisableTiming | hipEventDisableSystemFence;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_panel, order_flags);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[0], order_flags);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_rest[1], order_flags);
     if (e == hipSuccess) e = hipMalloc(&c.d_scal, 64 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c.d_int, 64 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.h_mail, tns::kMailBytes, hipHostMallocDefault);
@@ -820,7 +834,9 @@ TNS_API int tns_context_destroy(tns_context* ctx)
     // handles stay valid and refuse further calls with TNS_EDISPOSED.
     for (tns_batch* b : ctx->batches) b->plan.reset();
     ctx->batches.clear();
-    c.pool.release_all();
+    c.pool.release_all();  // the solve's exchange area among the blocks
+    c.flow = nullptr;
+    c.flow_bytes = 0;
     if (c.h_stage) (void)hipHostFree(c.h_stage);
     c.h_stage = nullptr;
     c.h_stage_bytes = 0;
@@ -962,6 +978,11 @@ TNS_API int tns_dgetrf_noalloc(tns_context* ctx, int m, int n, double* a, int ld
         // the row payloads.
         int* work = c->noalloc_scratch;
         int* perm = work + (tns::lu_work_ints(m, n, nb) + 1) / 2 * 2;
+        struct NoAlloc {
+            Context* c;
+            explicit NoAlloc(Context* x) : c(x) { c->no_alloc = true; }
+            ~NoAlloc() { c->no_alloc = false; }
+        } no_alloc(c);
         tns::dgetrf(c, m, n, a, lda, nb, pivots, perm, info, nullptr, work);
     });
 }

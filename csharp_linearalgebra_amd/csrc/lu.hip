@@ -1650,6 +1650,321 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(int n, const T* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Few-right-hand-side solve op(A) X = B, op(A) lower (FORWARD) or upper, in
+// one launch. The right-hand sides go in chains of up to FLOW_COLS columns
+// (one per wavefront: more columns per chain lengthen every block's step,
+// 16 took twice as long a block as 4);
+// chain h runs on the workgroups blockIdx.x = 8 g + h, g < G, which
+// round-robin dispatch places on one XCD, so its exchange words meet in one
+// L2 (as the spread panel's do). Workgroup g takes the diagonal blocks g,
+// g + G, ... in solve order. For a block it streams the tiles of op(A) in
+// its rows, one per block solved before it, through LDS, each tile loaded
+// before the x it multiplies is known, and applies a tile as soon as that x
+// block is published: every row takes the blocks in solve order and each
+// block column by column (ascending forward, descending backward), skipping
+// zero entries of x, the operations trsv_lds_kernel (op(A) = A) and
+// trsv_step_kernel (every form) give each element. Then each wavefront
+// solves the diagonal block for its column by readlane substitution
+// (trsv_lds_kernel's steps), publishes x as sequence-tagged words (the
+// launch's own number, so a word left by an earlier launch is never taken
+// for current) and stores x to B. A reader polls one word of the block with
+// one lane (every lane polling its own words kept the L2 lines of the
+// newest block busy), then reads its words once, each checked for the
+// sequence number and waited for if not yet current. Only x crosses
+// workgroups, and
+// the triangle is read once, by G workgroups at a time instead of one per
+// column: the chain's time is the diagonal solves and one L2 hop a block.
+// The G <= FLOW_MAXG workgroups of a chain fit on one XCD together (two per
+// CU: 66.5 KB of LDS and 256 VGPRs a lane at most for the real forms) and wait
+// only on each other's earlier blocks, so every wait ends; each also gives
+// up after SPREAD_SPIN polls or once the abort word (xbuf[0]) is raised, and
+// a block solved after a wait gave up is published and stored as NaN.
+constexpr int FLOW_COLS = 4;   // right-hand sides per chain, one per wavefront
+constexpr int FLOW_MAXG = 64;  // workgroups per chain: two per CU of one XCD
+
+// a / b, correctly rounded, from y = RN(1/b), by Markstein's correction
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990, as in Muller et al.,
+// Handbook of Floating-Point Arithmetic, the theorem on FMA division): q =
+// RN(a y) lies within one ulp of a/b, r = a - b q is exact (one FMA), and
+// RN(q + r y) is RN(a/b), for operands whose quotient and residual stay
+// clear of underflow and overflow (both in [2^-400, 2^400] for double,
+// [2^-50, 2^50] for float); other operands, zeros and non-finite values
+// take the division itself. Three dependent operations instead of the
+// division's sequence, with y computed once per divisor off the chain.
+__device__ __forceinline__ double div_known(double a, double b, double y)
+{
+    const double aa = fabs(a), ab = fabs(b);
+    if (aa >= 0x1p-400 && aa <= 0x1p400 && ab >= 0x1p-400 && ab <= 0x1p400) {
+        const double q = a * y;
+        return fma(fma(-q, b, a), y, q);
+    }
+    return a / b;
+}
+__device__ __forceinline__ float div_known(float a, float b, float y)
+{
+    const float aa = fabsf(a), ab = fabsf(b);
+    if (aa >= 0x1p-50f && aa <= 0x1p50f && ab >= 0x1p-50f && ab <= 0x1p50f) {
+        const float q = a * y;
+        return fmaf(fmaf(-q, b, a), y, q);
+    }
+    return a / b;
+}
+__device__ __forceinline__ zd div_known(zd a, zd b, zd) { return s_div(a, b); }
+__device__ __forceinline__ double recip_of(double b) { return 1.0 / b; }
+__device__ __forceinline__ float recip_of(float b) { return 1.0f / b; }
+__device__ __forceinline__ zd recip_of(zd) { return make_zd(0.0, 0.0); }
+
+template <typename T>
+__device__ __forceinline__ T from_words(const unsigned (&w)[sizeof(T) / 4])
+{
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
+template <typename T, bool FORWARD, int TR, bool UNIT>
+__global__ void __launch_bounds__(256)
+trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb, int nrhs, int G,
+                 unsigned long long* __restrict__ xbuf, int slot_words, unsigned seq)
+{
+    constexpr int BS = tri_block<T>();
+    constexpr int WPE = (int)sizeof(T) / 4;  // exchange words per element
+    constexpr int NCW = FLOW_COLS / 4;       // columns per wavefront
+    constexpr int R = BS * BS / 256;         // tile elements per thread
+    const int chain = blockIdx.x & 7, g = blockIdx.x >> 3;
+    const int c0 = chain * FLOW_COLS;
+    if (c0 >= nrhs || g >= G) return;
+    const int ncc = nrhs - c0 < FLOW_COLS ? nrhs - c0 : FLOW_COLS;
+    const int nblk = (n + BS - 1) / BS;
+    unsigned long long* abort_word = xbuf;
+    unsigned long long* slots = xbuf + 1 + (size_t)chain * nblk * slot_words;
+    __shared__ T tile[2][BS][BS + 1];  // tile[p][i] = op(A)(row i of the block, column p of the source)
+    // The wavefront index as a uniform value, so the per-column tests below
+    // are scalar branches rather than lane masks.
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int cw0 = wave * NCW;  // this wavefront's first column in the chain
+    const int ncw = ncc - cw0 <= 0 ? 0 : (ncc - cw0 < NCW ? ncc - cw0 : NCW);
+    T* X = B + (size_t)(c0 + cw0) * ldb;
+    // op(A)(i, k), read where the stored triangle keeps it.
+    auto opa = [&](int i, int k, bool ok) __attribute__((always_inline)) {
+        T v = load_if(TR == 0 ? A + i + (size_t)k * lda : A + k + (size_t)i * lda, ok);
+        if (TR == 2) v = s_conj(v);
+        return v;
+    };
+    // Tile elements by thread, consecutive threads along the stored column.
+    auto tile_ip = [&](int r, int& i, int& p) __attribute__((always_inline)) {
+        const int e = tid + 256 * r;
+        i = TR == 0 ? e % BS : e / BS;
+        p = TR == 0 ? e / BS : e % BS;
+    };
+    auto source = [&](int t, int& ks, int& kbs) __attribute__((always_inline)) {
+        const int sb = FORWARD ? t : nblk - 1 - t;
+        ks = sb * BS;
+        kbs = n - ks < BS ? n - ks : BS;
+    };
+    // Loads are unconditional, from indexes clamped into the block, and the
+    // entries past it are zeroed by selection: a guarded load compiles to a
+    // branch that waits for its own data, which serialized a tile's loads
+    // (about 10 us a tile).
+    T reg[R];
+    auto load_tile = [&](int t, int k0, int kb) __attribute__((always_inline)) {
+        int ks, kbs;
+        source(t, ks, kbs);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int i, p;
+            tile_ip(r, i, p);
+            const int ic = i < kb ? i : kb - 1, pc = p < kbs ? p : kbs - 1;
+            const T v = opa(k0 + ic, ks + pc, true);
+            reg[r] = s_sel(i < kb && p < kbs, v, s_zero(T{}));
+        }
+    };
+    // x of block sb for this wavefront's NC columns, lane l holding entry l
+    // (zero past the block): one lane polls the block's last word, then every
+    // lane reads its words once, and a word not yet current is waited for.
+    // False if a wait gave up.
+    auto fetch_x = [&](auto nc_, int sb, int kbs, T (&xv)[NCW]) __attribute__((always_inline)) -> bool {
+        constexpr int NC = decltype(nc_)::value;
+        const unsigned long long* xs = slots + (size_t)sb * slot_words + (size_t)cw0 * BS * WPE;
+        bool ok = true;
+        if (lane == 0) {
+            unsigned f[1];
+            ok = xwait<1>(xs + ((size_t)(NC - 1) * BS + kbs - 1) * WPE + WPE - 1, seq, abort_word, f);
+        }
+        if (!__all(ok)) return false;
+        const int lc = lane < kbs ? lane : kbs - 1;
+        unsigned long long xw[NC][WPE];
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < WPE; q++) xw[c][q] = xload(xs + ((size_t)c * BS + lc) * WPE + q);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            unsigned w[WPE];
+            bool cur = true;
+#pragma unroll
+            for (int q = 0; q < WPE; q++) {
+                w[q] = (unsigned)xw[c][q];
+                cur = cur && (unsigned)(xw[c][q] >> 32) == seq;
+            }
+            if (!cur) ok = xwait<WPE>(xs + ((size_t)c * BS + lc) * WPE, seq, abort_word, w);
+            xv[c] = s_sel(lane < kbs, from_words<T>(w), s_zero(T{}));
+        }
+        return __all(ok);
+    };
+    for (int bq = g; bq < nblk; bq += G) {
+        const int blk = FORWARD ? bq : nblk - 1 - bq;
+        const int k0 = blk * BS, kb = n - k0 < BS ? n - k0 : BS;
+        const bool mine = lane < kb;
+        // Every wavefront runs the same steps and barriers; NC (its column
+        // count, 0 for a wavefront without columns) is a constant inside.
+        auto block = [&](auto nc_) __attribute__((always_inline)) {
+            constexpr int NC = decltype(nc_)::value;
+            T acc[NCW];
+            if constexpr (NC > 0) {
+                const int lc = mine ? lane : kb - 1;
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[c] = s_sel(mine, X[k0 + lc + (size_t)c * ldb], s_zero(T{}));
+            }
+            bool bad = false;
+            // The diagonal block (tile bq) to the buffer bq & 1, visible
+            // after the barrier, and (non-unit) each lane's reciprocal of its
+            // own diagonal entry, its divisor in the substitution: staged
+            // before the last x is awaited.
+            T y = s_zero(T{});
+            auto stage_diagonal = [&]() __attribute__((always_inline)) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    int i, p;
+                    tile_ip(r, i, p);
+                    tile[bq & 1][p][i] = reg[r];
+                }
+                __syncthreads();
+                if (!UNIT && NC > 0) y = recip_of(tile[bq & 1][mine ? lane : 0][mine ? lane : 0]);
+            };
+            // Tiles 0 .. bq - 1 (the blocks solved before), then the
+            // diagonal block, each loaded one step ahead.
+            load_tile(0, k0, kb);
+            if (bq == 0) stage_diagonal();
+            for (int t = 0; t < bq; ++t) {
+                int ks, kbs;
+                source(t, ks, kbs);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    int i, p;
+                    tile_ip(r, i, p);
+                    tile[t & 1][p][i] = reg[r];
+                }
+                __syncthreads();
+                load_tile(t + 1, k0, kb);
+                // Buffer bq & 1 last held tile bq - 2, done before the barrier.
+                if (t + 1 == bq) stage_diagonal();
+                if constexpr (NC > 0) {
+                    T xv[NCW];
+                    if (!fetch_x(nc_, ks / BS, kbs, xv)) {
+                        bad = true;
+                        if (lane == 0) __hip_atomic_store(abort_word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                        for (int c = 0; c < NC; c++) xv[c] = s_zero(T{});
+                    }
+                    // acc -= x_p a_p, p in order, a zero x_p leaving acc as it
+                    // was. Without a zero x in the tile the skip cannot fire,
+                    // so the whole-tile path then runs without the select.
+                    bool zero = false;
+#pragma unroll
+                    for (int c = 0; c < NC; c++) zero = zero || (lane < kbs && s_iszero(xv[c]));
+                    if (kbs == BS && !__any(zero)) {
+                        constexpr int U = 8;
+#pragma unroll
+                        for (int u0 = 0; u0 < BS; u0 += U) {
+                            T av[U];
+#pragma unroll
+                            for (int v = 0; v < U; v++) av[v] = tile[t & 1][FORWARD ? u0 + v : BS - 1 - (u0 + v)][lane];
+#pragma unroll
+                            for (int v = 0; v < U; v++) {
+                                const int p = FORWARD ? u0 + v : BS - 1 - (u0 + v);
+#pragma unroll
+                                for (int c = 0; c < NC; c++)
+                                    acc[c] = s_add(acc[c], s_mul(s_neg(readlane_t(xv[c], p)), av[v]));
+                            }
+                        }
+                    } else {
+                        for (int u = 0; u < kbs; ++u) {
+                            const int p = FORWARD ? u : kbs - 1 - u;
+                            const T a = tile[t & 1][p][lane];
+#pragma unroll
+                            for (int c = 0; c < NC; c++) {
+                                const T xp = readlane_t(xv[c], p);
+                                acc[c] = s_sel(s_iszero(xp), acc[c], s_add(acc[c], s_mul(s_neg(xp), a)));
+                            }
+                        }
+                    }
+                }
+            }
+            if constexpr (NC > 0) {
+                // The diagonal block by substitution, a zero x_p skipping its
+                // update (selected).
+                // Entries of op(A) read from LDS eight steps at a time.
+                constexpr int U = 8;
+#pragma unroll
+                for (int s0 = 0; s0 < BS; s0 += U) {
+                    T dv[U];
+#pragma unroll
+                    for (int v = 0; v < U; v++) dv[v] = tile[bq & 1][FORWARD ? s0 + v : BS - 1 - (s0 + v)][lane];
+#pragma unroll
+                    for (int v = 0; v < U; v++) {
+                        const int p = FORWARD ? s0 + v : BS - 1 - (s0 + v);
+                        if (p >= kb) continue;
+                        // x_p: lane p's entry, read out and (non-unit) divided
+                        // as a value uniform over the wavefront, so the
+                        // division's range test is a scalar branch; the
+                        // divisor and its reciprocal come from lane p too,
+                        // independent of the chain.
+                        T dp = s_zero(T{}), yp = s_zero(T{});
+                        if (!UNIT) {
+                            dp = readlane_t(dv[v], p);
+                            yp = readlane_t(y, p);
+                        }
+#pragma unroll
+                        for (int c = 0; c < NC; c++) {
+                            T xp = readlane_t(acc[c], p);
+                            if (!UNIT) {
+                                xp = div_known(xp, dp, yp);
+                                acc[c] = s_sel(lane == p, xp, acc[c]);
+                            }
+                            const bool ahead = FORWARD ? (lane > p && lane < kb) : (lane < p);
+                            acc[c] = s_sel(ahead && !s_iszero(xp), s_add(acc[c], s_mul(s_neg(xp), dv[v])), acc[c]);
+                        }
+                    }
+                }
+                if (bad) {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) acc[c] = s_nan(T{});
+                }
+                if (bq + 1 < nblk) {
+                    // Blocks solved later read it: publish before the store to B.
+                    unsigned long long* xo = slots + (size_t)blk * slot_words + (size_t)cw0 * BS * WPE;
+#pragma unroll
+                    for (int c = 0; c < NC; c++) {
+                        if (mine) {
+                            unsigned w[WPE];
+                            __builtin_memcpy(w, &acc[c], sizeof(T));
+#pragma unroll
+                            for (int q = 0; q < WPE; q++) xstore(xo + ((size_t)c * BS + lane) * WPE + q, seq, w[q]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+                    if (mine) X[k0 + lane + (size_t)c * ldb] = acc[c];
+            }
+            __syncthreads();  // both tile buffers free for the next block
+        };
+        dispatch<0, NCW + 1>(ncw, block);
+    }
+}
+
 // trsv_lds_kernel for G right-hand sides at once, over columns already in
 // LDS (xs[c * n + i], c < nr <= G): every load of op(A) serves all of them.
 // Each column receives exactly trsv_lds_kernel's operations, in its order:
@@ -2181,11 +2496,69 @@ void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ld
 // 2.1 -> 2.8 ms); at 8192 the step form is 5x faster.
 constexpr int TRSV_STEP_FROM = 2048;
 
+// The exchange area of trsv_flow_kernel, at least `words` long: the context
+// keeps one pool block between calls; a larger need returns it (once the
+// stream that used it is idle) and takes a zeroed one.
+inline unsigned long long* flow_area(Context* ctx, size_t words)
+{
+    const size_t bytes = words * sizeof(unsigned long long);
+    if (bytes > ctx->flow_bytes) {
+        if (ctx->flow) {
+            TNS_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->pool.give(ctx->flow, ctx->flow_bytes);
+            ctx->flow = nullptr;
+            ctx->flow_bytes = 0;
+        }
+        const size_t want = bytes > ((size_t)1 << 20) ? bytes : ((size_t)1 << 20);
+        ctx->flow = static_cast<unsigned long long*>(ctx->pool.take(want));
+        ctx->flow_bytes = want;
+        TNS_HIP(hipMemsetAsync(ctx->flow, 0, want, ctx->stream));
+    }
+    return ctx->flow;
+}
+
+// op(A) X = B by trsv_flow_kernel, when it applies: more than one diagonal
+// block, at most 8 chains of right-hand sides, and a launch its sequence
+// number cannot be replayed under (no graph capture), on the context's own
+// stream (the exchange area is one per context), outside the no-allocation
+// entry point. Returns false otherwise.
+template <typename T, bool FORWARD, int TR, bool UNIT>
+bool trsv_flow(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb)
+{
+    constexpr int BS = tri_block<T>();
+    if (ctx->capturing || ctx->no_alloc || ctx->stream == ctx->side || n <= BS || nrhs > 8 * FLOW_COLS) return false;
+    const int nblk = (n + BS - 1) / BS;
+    const int ncc = nrhs < FLOW_COLS ? nrhs : FLOW_COLS;
+    const int slot_words = ncc * BS * (int)(sizeof(T) / 4);
+    const int chains = (nrhs + FLOW_COLS - 1) / FLOW_COLS;
+    unsigned long long* xbuf = flow_area(ctx, 1 + (size_t)chains * nblk * slot_words);
+    if (++ctx->flow_seq == 0) ++ctx->flow_seq;  // zero is the area's initial tag
+    // Workgroups of one chain: at most what one XCD holds at once (its CUs
+    // times the kernel's workgroups per CU), so all of them can be resident.
+    static const int per_xcd = [ctx] {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trsv_flow_kernel<T, FORWARD, TR, UNIT>, 256, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
+            (void)hipGetLastError();
+            return 1;
+        }
+        const int v = per_cu * (cus / 8);
+        return v > 0 ? v : 1;
+    }();
+    int G = nblk < FLOW_MAXG ? nblk : FLOW_MAXG;
+    if (G > per_xcd) G = per_xcd;
+    hipLaunchKernelGGL((trsv_flow_kernel<T, FORWARD, TR, UNIT>), dim3(8 * G), dim3(256), 0, ctx->stream, n, A, lda, B,
+                       ldb, nrhs, G, xbuf, slot_words, ctx->flow_seq);
+    return true;
+}
+
 template <typename T, bool LOWER, int TR, bool UNIT>
 void trsm_impl(Context* ctx, int n, int nrhs, const T* A, int lda, T* B, int ldb)
 {
     if (n <= 0 || nrhs <= 0) return;
     constexpr bool FORWARD = (LOWER && TR == 0) || (!LOWER && TR != 0);
+    if ((nrhs <= 32 || n <= 64) && trsv_flow<T, FORWARD, TR, UNIT>(ctx, n, nrhs, A, lda, B, ldb)) return;
     if ((nrhs <= 32 || n <= 64) && n >= TRSV_STEP_FROM) {
         // Spread over the chip, one launch per diagonal block (the same
         // operations per element as trsv_lds_kernel).

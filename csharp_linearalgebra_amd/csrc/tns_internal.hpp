@@ -130,6 +130,14 @@ struct Context {
     std::mutex gate;            // the Workspace lock (Workspace.cs:136-139)
     // This device's copy of the no-allocation LU's file-scope scratch.
     int* noalloc_scratch = nullptr;
+    // The one-launch triangular solve's exchange words: a pool block kept
+    // between calls (grown on demand, zeroed when taken), and the sequence
+    // number of its last launch (each launch tags its words with its own).
+    unsigned long long* flow = nullptr;
+    size_t flow_bytes = 0;
+    unsigned flow_seq = 0;
+    bool capturing = false;  // a graph capture is recording `stream`
+    bool no_alloc = false;   // inside the no-allocation entry point
 };
 
 // Points ctx->stream at s for the guard's life: the helpers launch on

@@ -53,6 +53,9 @@ __host__ __device__ inline double s_conj(double a) { return a; }
 __host__ __device__ inline zd s_conj(zd a) { return make_zd(a.x, -a.y); }
 __host__ __device__ inline bool s_iszero(double a) { return a == 0.0; }
 __host__ __device__ inline bool s_iszero(zd a) { return a.x == 0.0 && a.y == 0.0; }
+__host__ __device__ inline float s_nan(float) { return __builtin_nanf(""); }
+__host__ __device__ inline double s_nan(double) { return __builtin_nan(""); }
+__host__ __device__ inline zd s_nan(zd) { return make_zd(__builtin_nan(""), __builtin_nan("")); }
 __host__ __device__ inline double s_scale(double a, double f) { return a * f; }
 __host__ __device__ inline zd s_scale(zd a, double f) { return make_zd(a.x * f, a.y * f); }
 __host__ __device__ inline double s_re(double a) { return a; }

@@ -130,6 +130,42 @@ def test_triangular_solves_at_large_order(ws, kind):
 
 
 @pytest.mark.parametrize("kind", range(6))
+@pytest.mark.parametrize("n,nrhs", [(130, 1), (1000, 17), (1999, 32), (4097, 3)])
+def test_triangular_solves_in_one_launch(ws, kind, n, nrhs):
+    """Up to 32 right-hand sides past one diagonal block take the one-launch
+    solve: chains of 4 columns (17 makes five, the last partial; 32 eight,
+    one per XCD; 3 one partial chain),
+    orders that end in a partial block (130, 1999, 4097), every structure
+    kind. Within 1e-12 of the oracle's substitution; then a second call with
+    other right-hand sides, whose launch tags its words with the next
+    sequence number: a word of the first call taken for current would show
+    as a wrong solution."""
+    r = np.random.default_rng(1000 * kind + n + nrhs)
+    a = r.uniform(-1, 1, (n, n)) / n + np.eye(n) * (1.0 + r.uniform(0, 1, n))
+    for _ in range(2):
+        b = r.uniform(-1, 1, (n, nrhs))
+        x = ws.solve_triangular(kind, a, b)
+        ref = O.trsv(kind, a, b)
+        assert np.max(np.abs(x - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n,nrhs", [(1000, 1), (1000, 16), (333, 32)])
+def test_zgetrs_few_rhs_equals_the_oracle(ws, n, nrhs):
+    """The complex solve with few right-hand sides (32-row diagonal blocks in
+    the one-launch solve), and its adjoint, against the oracle's getrs."""
+    r = np.random.default_rng(n + nrhs)
+    a = r.uniform(-1, 1, (n, n)) + 1j * r.uniform(-1, 1, (n, n))
+    b = r.uniform(-1, 1, (n, nrhs)) + 1j * r.uniform(-1, 1, (n, nrhs))
+    lu = ws.factor_lu(a)
+    ref = O.getrf(a)
+    x = lu.solve(b)
+    xr = O.getrs(ref, b)
+    assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()
+    assert berr(a, x, b) <= n * EPS
+    assert berr(a.conj().T, lu.solve_adjoint(b), b) <= n * EPS
+
+
+@pytest.mark.parametrize("kind", range(6))
 def test_triangular_solves_many_right_hand_sides(ws, kind):
     """More than 32 right-hand sides take the blocked path (diagonal blocks by
     substitution, the coupling by GEMM) for every structure kind: the device
