@@ -325,10 +325,10 @@ dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, in
 // Complex, with ComplexKernels.Combine's scalar special cases folded into
 // the write-back (ComplexKernels.cs:213-240). flags: 1 unit alpha, 2 real
 // alpha, 4 zero beta, 8 unit beta. TA: 0 A, 1 A^T, 2 A^H.
-template <int BM, int BN, int BK, int WM, int WN, int TA>
-__device__ __forceinline__ void zgemm_tile(int m, int n, int k, double ar_, double ai_, const double2* __restrict__ A,
-                                           int lda, const double2* __restrict__ B, int ldb, double br_, double bi_,
-                                           double2* __restrict__ C, int ldc, int flags, int m0, int n0)
+template <int BM, int BN, int BK, int WM, int WN, int TA, typename PA, typename PB, typename PC>
+__device__ __forceinline__ void zgemm_tile(int m, int n, int k, double ar_, double ai_, PA __restrict__ A, int lda,
+                                           PB __restrict__ B, int ldb, double br_, double bi_, PC __restrict__ C,
+                                           int ldc, int flags, int m0, int n0)
 {
     using S = Shape<BM, BN, BK, WM, WN>;
     using P = Planes<BM, BN, BK, TA>;
@@ -535,10 +535,10 @@ zgemm_grouped_kernel(const GemmRec<zd>* __restrict__ recs, const int* __restrict
         const GemmRec<zd> r = recs[g];
         const int local = t - starts[g];
         const int tiles_m = (r.m + BM - 1) / BM;
-        zgemm_tile<BM, BN, BK, WM, WN, 0>(r.m, r.n, r.k, -1.0, 0.0, reinterpret_cast<const double2*>(r.a), r.lda,
-                                          reinterpret_cast<const double2*>(r.b), r.ldb, 1.0, 0.0,
-                                          reinterpret_cast<double2*>(r.c), r.ldc, 2 | 8, (local % tiles_m) * BM,
-                                          (local / tiles_m) * BN);
+        zgemm_tile<BM, BN, BK, WM, WN, 0>(r.m, r.n, r.k, -1.0, 0.0, as_global(reinterpret_cast<const double2*>(r.a)),
+                                          r.lda, as_global(reinterpret_cast<const double2*>(r.b)), r.ldb, 1.0, 0.0,
+                                          as_global(reinterpret_cast<double2*>(r.c)), r.ldc, 2 | 8,
+                                          (local % tiles_m) * BM, (local / tiles_m) * BN);
     }
 }
 

@@ -217,10 +217,9 @@ __device__ __forceinline__ zd blend(zd x, zd y, bool take) { return zd{blend(x.x
 // pivot row are skipped, as the reference's loop does (one ballot gives the
 // nonzero pattern); done slots are masked off.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int R, int NT>
-__device__ __forceinline__ void getf2_reg_body(int m, int w, T* __restrict__ a, int lda, int* __restrict__ piv,
-                                               int off, int* __restrict__ perm, int* __restrict__ list, int cap,
-                                               int* __restrict__ info)
+template <typename T, int NB, int R, int NT, typename PA, typename PI>
+__device__ __forceinline__ void getf2_reg_body(int m, int w, PA __restrict__ a, int lda, PI __restrict__ piv, int off,
+                                               PI __restrict__ perm, PI __restrict__ list, int cap, PI __restrict__ info)
 {
     static_assert(NB <= 64 && NB % 4 == 0, "the nonzero mask is one ballot; columns go in groups of 4");
     constexpr int GW = NB < 8 ? NB : 8;  // columns per group of pivot-row reads
@@ -431,7 +430,8 @@ template <typename T, int NB, int R, int NT>
 __global__ void __launch_bounds__(NT) getf2_grouped_kernel(const PanelRec<T>* __restrict__ recs)
 {
     const PanelRec<T> r = recs[blockIdx.x];
-    getf2_reg_body<T, NB, R, NT>(r.m, r.w, r.a, r.lda, r.piv, r.off, r.perm, r.list, 2 * r.w, r.info);
+    getf2_reg_body<T, NB, R, NT>(r.m, r.w, as_global(r.a), r.lda, as_global(r.piv), r.off, as_global(r.perm),
+                                 as_global(r.list), 2 * r.w, as_global(r.info));
 }
 
 // ---------------------------------------------------------------------------
@@ -1010,9 +1010,9 @@ compact_moves_kernel(int count, int base, int* __restrict__ rowid, int* __restri
 // A workgroup owns 256/QM columns and QM >= count move slots per column:
 // every thread reads its source element, one barrier, every thread writes
 // its destination, so all reads of a column precede its writes.
-template <typename T, int QM>
-__device__ __forceinline__ void apply_moves_body(T* __restrict__ a, int lda, int c0, int c1, int ncols,
-                                                 const int* __restrict__ list, int block)
+template <typename T, int QM, typename PA, typename PL>
+__device__ __forceinline__ void apply_moves_body(PA __restrict__ a, int lda, int c0, int c1, int ncols,
+                                                 PL __restrict__ list, int block)
 {
     constexpr int CPB = 256 / QM;
     const int q = threadIdx.x % QM;
@@ -1021,7 +1021,7 @@ __device__ __forceinline__ void apply_moves_body(T* __restrict__ a, int lda, int
     const int count = list[0] < QM ? list[0] : QM;
     const bool active = idx < left + right && q < count;
     T val{};
-    T* col = nullptr;
+    PA col = a;
     if (active) {
         const int j = idx < left ? idx : c1 + (idx - left);
         col = a + (size_t)j * lda;
@@ -1056,7 +1056,7 @@ __global__ void __launch_bounds__(256)
 apply_moves_grouped_kernel(const MovesRec<T>* __restrict__ recs, const int* __restrict__ starts, int count)
 {
     const MovesRec<T> r = recs[find_record(starts, count, blockIdx.x)];
-    apply_moves_body<T, QM>(r.a, r.lda, r.c0, r.c1, r.ncols, r.list, blockIdx.x - r.start);
+    apply_moves_body<T, QM>(as_global(r.a), r.lda, r.c0, r.c1, r.ncols, as_global(r.list), blockIdx.x - r.start);
 }
 
 // Larger move lists (never produced by nb <= 64, kept for generality).
@@ -1085,9 +1085,9 @@ __global__ void apply_moves_slow_kernel(T* __restrict__ a, int lda, int c0, int 
 // column-oriented substitution (Triangular.cs:45-57, 126-140), in the same
 // order for every element. The step loop is unrolled, so the owner's
 // register index p / G is a constant.
-template <typename T, int BS, int CB, bool LOWER, int TR, bool UNIT>
-__device__ __forceinline__ void trsm_block_body(int w, int ncols, const T* __restrict__ A, int lda,
-                                                T* __restrict__ B, int ldb, int block)
+template <typename T, int BS, int CB, bool LOWER, int TR, bool UNIT, typename PA, typename PB>
+__device__ __forceinline__ void trsm_block_body(int w, int ncols, PA __restrict__ A, int lda, PB __restrict__ B,
+                                                int ldb, int block)
 {
     constexpr int G = 256 / CB;
     constexpr int Q = BS / G;
@@ -1313,8 +1313,8 @@ __global__ void __launch_bounds__(256)
 trsm_grouped_kernel(const TrsmRec<T>* __restrict__ recs, const int* __restrict__ starts, int count)
 {
     const TrsmRec<T> r = recs[find_record(starts, count, blockIdx.x)];
-    trsm_block_body<T, tri_block<T>(), GROUPED_TRSM_COLUMNS, true, 0, true>(r.w, r.ncols, r.l, r.ldl, r.b, r.ldb,
-                                                                          blockIdx.x - r.start);
+    trsm_block_body<T, tri_block<T>(), GROUPED_TRSM_COLUMNS, true, 0, true>(
+        r.w, r.ncols, as_global(r.l), r.ldl, as_global(r.b), r.ldb, blockIdx.x - r.start);
 }
 
 // Whole triangular solve for few right-hand sides: one workgroup of 256
@@ -1710,10 +1710,57 @@ __device__ __forceinline__ float div_known(float a, float b, float y)
     }
     return a / b;
 }
-__device__ __forceinline__ zd div_known(zd a, zd b, zd) { return s_div(a, b); }
-__device__ __forceinline__ double recip_of(double b) { return 1.0 / b; }
-__device__ __forceinline__ float recip_of(float b) { return 1.0f / b; }
-__device__ __forceinline__ zd recip_of(zd) { return make_zd(0.0, 0.0); }
+
+// A divisor prepared once for many divisions giving s_div's results bit for
+// bit: real, the divisor and its correctly rounded reciprocal; complex,
+// Smith's scaling (which part is the larger, the ratio r and the scale den,
+// as s_div computes them) and den's reciprocal, so each quotient part is
+// one division by den, done as div_known does it.
+template <typename T>
+struct Divisor {
+    T b, y;
+};
+template <>
+struct Divisor<zd> {
+    double r, den, y;
+    int small_d;  // |Im| < |Re|
+};
+__device__ __forceinline__ Divisor<double> prep_div(double b) { return {b, 1.0 / b}; }
+__device__ __forceinline__ Divisor<float> prep_div(float b) { return {b, 1.0f / b}; }
+__device__ __forceinline__ Divisor<zd> prep_div(zd v)
+{
+    const double c = v.x, d = v.y;
+    Divisor<zd> q;
+    q.small_d = fabs(d) < fabs(c);
+    if (q.small_d) {
+        q.r = d / c;
+        q.den = c + d * q.r;
+    } else {
+        q.r = c / d;
+        q.den = d + c * q.r;
+    }
+    q.y = 1.0 / q.den;
+    return q;
+}
+__device__ __forceinline__ double div_prep(double a, const Divisor<double>& v) { return div_known(a, v.b, v.y); }
+__device__ __forceinline__ float div_prep(float a, const Divisor<float>& v) { return div_known(a, v.b, v.y); }
+__device__ __forceinline__ zd div_prep(zd x, const Divisor<zd>& v)
+{
+    const double a = x.x, b = x.y;
+    if (v.small_d) return make_zd(div_known(a + b * v.r, v.den, v.y), div_known(b - a * v.r, v.den, v.y));
+    return make_zd(div_known(b + a * v.r, v.den, v.y), div_known(-a + b * v.r, v.den, v.y));
+}
+// Lane p's prepared divisor, as a uniform value.
+template <typename T>
+__device__ __forceinline__ Divisor<T> readlane_div(const Divisor<T>& v, int p)
+{
+    return {readlane_t(v.b, p), readlane_t(v.y, p)};
+}
+template <>
+__device__ __forceinline__ Divisor<zd> readlane_div(const Divisor<zd>& v, int p)
+{
+    return {readlane_t(v.r, p), readlane_t(v.den, p), readlane_t(v.y, p), __builtin_amdgcn_readlane(v.small_d, p)};
+}
 
 template <typename T>
 __device__ __forceinline__ T from_words(const unsigned (&w)[sizeof(T) / 4])
@@ -1832,7 +1879,7 @@ trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int
             // after the barrier, and (non-unit) each lane's reciprocal of its
             // own diagonal entry, its divisor in the substitution: staged
             // before the last x is awaited.
-            T y = s_zero(T{});
+            Divisor<T> dvs{};
             auto stage_diagonal = [&]() __attribute__((always_inline)) {
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -1841,7 +1888,7 @@ trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int
                     tile[bq & 1][p][i] = reg[r];
                 }
                 __syncthreads();
-                if (!UNIT && NC > 0) y = recip_of(tile[bq & 1][mine ? lane : 0][mine ? lane : 0]);
+                if (!UNIT && NC > 0) dvs = prep_div(tile[bq & 1][mine ? lane : 0][mine ? lane : 0]);
             };
             // Tiles 0 .. bq - 1 (the blocks solved before), then the
             // diagonal block, each loaded one step ahead.
@@ -1921,16 +1968,13 @@ trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int
                         // division's range test is a scalar branch; the
                         // divisor and its reciprocal come from lane p too,
                         // independent of the chain.
-                        T dp = s_zero(T{}), yp = s_zero(T{});
-                        if (!UNIT) {
-                            dp = readlane_t(dv[v], p);
-                            yp = readlane_t(y, p);
-                        }
+                        Divisor<T> dp{};
+                        if (!UNIT) dp = readlane_div(dvs, p);
 #pragma unroll
                         for (int c = 0; c < NC; c++) {
                             T xp = readlane_t(acc[c], p);
                             if (!UNIT) {
-                                xp = div_known(xp, dp, yp);
+                                xp = div_prep(xp, dp);
                                 acc[c] = s_sel(lane == p, xp, acc[c]);
                             }
                             const bool ahead = FORWARD ? (lane > p && lane < kb) : (lane < p);
@@ -1971,8 +2015,8 @@ trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int
 // the left-looking dot products keep its lane split and reduction, wave c
 // solves column c's diagonal block, and the right-looking update keeps its
 // per-column skip of zero entries.
-template <typename T, bool LOWER, int TR, bool UNIT, int G>
-__device__ void trsv_lds_group(int n, const T* __restrict__ A, int lda, T* xs, int nr)
+template <typename T, bool LOWER, int TR, bool UNIT, int G, typename PA>
+__device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
 {
     constexpr int BS = tri_block<T>();
     static_assert(G >= 1 && G <= 4, "one wavefront per column for the diagonal blocks");
@@ -2104,28 +2148,32 @@ __global__ void __launch_bounds__(256) getrs_grouped_kernel(const SolveRec<T>* _
     extern __shared__ unsigned char getrs_smem[];
     T* xs = reinterpret_cast<T*>(getrs_smem);
     const SolveRec<T> r = recs[blockIdx.x];
+    const auto lu = as_global(r.lu);
+    const auto perm = as_global(r.perm);
+    const auto x = as_global(r.x);
+    const auto y = as_global(r.y);
     const int tid = threadIdx.x, n = r.n;
-    if (tid == 0 && r.info != nullptr && *r.info == INT_MAX) *r.info = -1;
+    if (tid == 0 && r.info != nullptr && *as_global(r.info) == INT_MAX) *as_global(r.info) = -1;
     constexpr int TR = sizeof(T) == 16 ? 2 : 1;
     for (int c0 = 0; c0 < r.nrhs; c0 += G) {
         const int nr = r.nrhs - c0 < G ? r.nrhs - c0 : G;
         for (int e = tid; e < nr * n; e += 256) {
             const int c = e / n, i = e % n;
-            const int src = TRANS ? i : r.perm[i];
-            xs[c * n + i] = r.x[src + (size_t)(c0 + c) * r.ldx];
+            const int src = TRANS ? i : perm[i];
+            xs[c * n + i] = x[src + (size_t)(c0 + c) * r.ldx];
         }
         __syncthreads();
         if (!TRANS) {
-            trsv_lds_group<T, true, 0, true, G>(n, r.lu, r.ldlu, xs, nr);
-            trsv_lds_group<T, false, 0, false, G>(n, r.lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, true, 0, true, G>(n, lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, false, 0, false, G>(n, lu, r.ldlu, xs, nr);
         } else {
-            trsv_lds_group<T, false, TR, false, G>(n, r.lu, r.ldlu, xs, nr);
-            trsv_lds_group<T, true, TR, true, G>(n, r.lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, false, TR, false, G>(n, lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, true, TR, true, G>(n, lu, r.ldlu, xs, nr);
         }
         for (int e = tid; e < nr * n; e += 256) {
             const int c = e / n, i = e % n;
-            const int dst = TRANS ? r.perm[i] : i;
-            r.y[dst + (size_t)(c0 + c) * r.ldy] = xs[c * n + i];
+            const int dst = TRANS ? perm[i] : i;
+            y[dst + (size_t)(c0 + c) * r.ldy] = xs[c * n + i];
         }
         __syncthreads();
     }
@@ -2145,12 +2193,19 @@ __global__ void __launch_bounds__(1024) lu_init_grouped_kernel(const InitRec<T>*
 {
     constexpr int W = 16;  // wavefronts
     __shared__ double red[W];
-    const InitRec<T> r = recs[blockIdx.x];
+    const InitRec<T> rec = recs[blockIdx.x];
+    struct {
+        decltype(as_global(rec.a)) a;
+        int lda, n;
+        decltype(as_global(rec.perm)) perm;
+        decltype(as_global(rec.anorm)) anorm;
+        decltype(as_global(rec.info)) info;
+    } r{as_global(rec.a), rec.lda, rec.n, as_global(rec.perm), as_global(rec.anorm), as_global(rec.info)};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < r.n; i += 64 * W) r.perm[i] = i;
     double best = 0.0;
     for (int j = wave; j < r.n; j += W) {
-        const T* col = r.a + (size_t)j * r.lda;
+        const auto col = r.a + (size_t)j * r.lda;
         double s = 0.0;
         // Four entries loaded together, summed in row order.
         int i = lane;

@@ -109,6 +109,21 @@ __host__ __device__ inline bool s_isfinite(float a) { return __builtin_isfinite(
 __host__ __device__ inline double s_sel(bool c, double a, double b) { return c ? a : b; }
 __host__ __device__ inline zd s_sel(bool c, zd a, zd b) { return make_zd(c ? a.x : b.x, c ? a.y : b.y); }
 
+// p as a pointer into global memory, by type: a pointer read from a record
+// in memory (the grouped kernels' arguments) is otherwise generic, and its
+// loads become flat loads, which the compiler must also count as LDS
+// operations (every wait on one waits on both).
+// (The host pass compiles these bodies too, where the pointer stays as it is.)
+template <typename T>
+__device__ __forceinline__ auto as_global(T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (T __attribute__((address_space(1)))*)p;
+#else
+    return p;
+#endif
+}
+
 // *p if ok, else zero. A conditional expression between two lvalues of a
 // struct type selects an address, and an address of a local puts the local
 // in scratch memory; returning by value keeps everything in registers.
