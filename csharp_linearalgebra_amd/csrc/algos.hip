@@ -288,7 +288,24 @@ void NormEstRun<T>::resume()
             keys_[i] = -h_[i];
             order_[i] = i;
         }
-        sort_keys_items(n, keys_.data(), order_.data());
+        // Only the first t unused indices of the sorted order are taken,
+        // and they lie among the smallest m = t + (indices used) + 1 keys.
+        // When those m keys are distinct, none equals the next key and no
+        // key is NaN, every sort puts the same indices there in the same
+        // order, Array.Sort included: a partial selection decides them
+        // (n log m, not the full sort). Otherwise the full restatement of
+        // Array.Sort gives its own order of the ties.
+        int m = t + used_count_ + 1;
+        bool partial = m < n;
+        for (int i = 0; partial && i < n; i++) partial = !std::isnan(keys_[i]);
+        if (partial) {
+            std::partial_sort(order_.begin(), order_.begin() + m + 1, order_.end(),
+                              [&](int x, int y) { return keys_[x] < keys_[y]; });
+            for (int i = 1; partial && i <= m; i++) partial = keys_[order_[i - 1]] < keys_[order_[i]];
+            if (!partial)
+                for (int i = 0; i < n; i++) order_[i] = i;
+        }
+        if (!partial) sort_keys_items(n, keys_.data(), order_.data());
         if (t > 1) {
             bool all_used = true;
             for (int j = 0; j < t; j++)
@@ -312,6 +329,7 @@ void NormEstRun<T>::resume()
         for (int j = 0; j < t; j++) {
             int idx = order_[j];
             current_[j] = idx;
+            used_count_ += used_[idx] ? 0 : 1;
             used_[idx] = 1;
             x_[(size_t)j * n + idx] = one;
         }

@@ -45,7 +45,7 @@ public:
 
 private:
     int n_ = 0, t_ = 1, itmax_ = 2, want_ = 0;
-    int k_ = 0, iterations_ = 0, products_ = 0, best_index_ = -1;
+    int k_ = 0, iterations_ = 0, products_ = 0, best_index_ = -1, used_count_ = 0;
     double est_ = 0.0, est_old_ = 0.0;
     std::vector<T> x_, s_, s_old_, out_;
     std::vector<double> h_, keys_;
