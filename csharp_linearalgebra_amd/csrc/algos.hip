@@ -382,8 +382,10 @@ std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, co
     const int R = (int)powers.size();
     std::vector<NormEst> out(R);
     if (R == 0 || n <= 0) return out;
+    // The runs' host steps (their set-up, and each round's step) in
+    // parallel on the context's host threads: they are independent.
     std::vector<std::unique_ptr<NormEstRun<T>>> runs(R);
-    for (int r = 0; r < R; r++) runs[r] = std::make_unique<NormEstRun<T>>(n, columns, max_iterations, seed);
+    ctx->host.run(R, [&](int r) { runs[r] = std::make_unique<NormEstRun<T>>(n, columns, max_iterations, seed); });
     std::vector<int> ord(R);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return powers[x] > powers[y]; });
@@ -405,8 +407,9 @@ std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, co
         for (int r : act)
             if (runs[r]->want() != kind) fail(TNS_ESTATE, "norm estimates out of step");
         const int A = (int)act.size();
-        for (int i = 0; i < A; i++)
+        ctx->host.run(A, [&](int i) {
             std::copy(runs[act[i]]->input().begin(), runs[act[i]]->input().end(), hx + panel * i);
+        });
         TNS_HIP(hipMemcpyAsync(dx.get(), hx, sizeof(T) * panel * A, hipMemcpyHostToDevice, ctx->stream));
         const T* src = dx.get();
         T* bufs[2] = {dy0.get(), dy1.get()};
@@ -428,12 +431,12 @@ std::vector<NormEst> normest_powers(Context* ctx, int n, const T* a, int lda, co
         TNS_HIP(hipMemcpyAsync(hy0, dy0.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
         TNS_HIP(hipMemcpyAsync(hy1, dy1.get(), sizeof(T) * panel * A, hipMemcpyDeviceToHost, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
-        for (int i = 0; i < A; i++) {
+        ctx->host.run(A, [&](int i) {
             NormEstRun<T>& run = *runs[act[i]];
             const T* hy = (powers[act[i]] & 1) ? hy1 : hy0;
             std::copy(hy + panel * i, hy + panel * (i + 1), run.output().begin());
             run.resume();
-        }
+        });
     }
     for (int r = 0; r < R; r++) out[r] = runs[r]->result();
     return out;

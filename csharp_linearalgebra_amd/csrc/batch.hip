@@ -85,36 +85,6 @@ hipGraphExec_t capture(Context* ctx, std::vector<std::pair<void*, size_t>>* hold
     return exec;
 }
 
-// fn(i) for i in [0, count) on up to 16 host threads (the estimates of a
-// batch advance independently). The first exception is rethrown here.
-template <typename F>
-void parallel_for(int count, F&& fn)
-{
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    const int workers = std::min(std::min(hw, 16), (count + 31) / 32);
-    if (workers <= 1) {
-        for (int i = 0; i < count; i++) fn(i);
-        return;
-    }
-    std::atomic<int> next{0};
-    std::exception_ptr error;
-    std::mutex error_gate;
-    auto work = [&]() {
-        try {
-            for (int i; (i = next.fetch_add(1)) < count;) fn(i);
-        } catch (...) {
-            std::lock_guard<std::mutex> lock(error_gate);
-            if (!error) error = std::current_exception();
-            next.store(count);
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < workers; w++) pool.emplace_back(work);
-    work();
-    for (auto& t : pool) t.join();
-    if (error) std::rethrow_exception(error);
-}
-
 // Flat launches: give each record its first workgroup; returns the total.
 template <typename R, typename Blocks>
 int assign_starts(std::vector<R>& recs, Blocks&& blocks_of)
@@ -494,7 +464,7 @@ void BatchPlan::rcond(double* out, int t)
     const size_t total = rc_off[count];
     std::vector<std::unique_ptr<NormEstRun<zd>>> runs(count);
     std::vector<char> active(count);
-    parallel_for(count, [&](int i) {
+    ctx->host.run(count, [&](int i) {
         runs[i] = std::make_unique<NormEstRun<zd>>(n[i], t, 5, 20260915);
         active[i] = sing[i] < 0 && an[i] != 0.0 && runs[i]->want() != 0;
     });
@@ -507,13 +477,13 @@ void BatchPlan::rcond(double* out, int t)
             any = true;
         }
         if (!any) break;
-        parallel_for(count, [&](int i) {
+        ctx->host.run(count, [&](int i) {
             if (active[i])
                 std::memcpy(h_panels + rc_off[i], runs[i]->input().data(), sizeof(zd) * runs[i]->input().size());
         });
         TNS_HIP(hipGraphLaunch(kind == 1 ? apply_exec : adjoint_exec, ctx->stream));
         TNS_HIP(hipStreamSynchronize(ctx->stream));
-        parallel_for(count, [&](int i) {
+        ctx->host.run(count, [&](int i) {
             if (!active[i]) return;
             std::vector<zd>& o = runs[i]->output();
             std::memcpy(o.data(), h_panels + total + rc_off[i], sizeof(zd) * o.size());

@@ -164,6 +164,85 @@ void Pool::release_all()
 
 Pool::~Pool() { release_all(); }
 
+void HostPool::run(int count, const std::function<void(int)>& fn)
+{
+    if (count <= 0) return;
+    if (count <= 4) {
+        // A few steps: waking the workers costs more than the steps.
+        for (int i = 0; i < count; i++) fn(i);
+        return;
+    }
+    if (threads_.empty()) {
+        const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+        const int workers = std::min(hw, 8) - 1;  // the lockstep estimates are at most 8
+        for (int w = 0; w < workers; w++) threads_.emplace_back([this] { work(); });
+    }
+    {
+        std::lock_guard<std::mutex> lock(m_);
+        job_ = &fn;
+        count_ = count;
+        next_.store(0);
+        error_ = nullptr;
+        busy_ = (int)threads_.size();
+        ++generation_;
+    }
+    wake_.notify_all();
+    auto take = [&]() {
+        try {
+            for (int i; (i = next_.fetch_add(1)) < count;) fn(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(m_);
+            if (!error_) error_ = std::current_exception();
+            next_.store(count);
+        }
+    };
+    take();
+    std::unique_lock<std::mutex> lock(m_);
+    done_.wait(lock, [&] { return busy_ == 0; });
+    job_ = nullptr;
+    if (error_) std::rethrow_exception(error_);
+}
+
+void HostPool::work()
+{
+    unsigned long long seen = 0;
+    for (;;) {
+        const std::function<void(int)>* job;
+        int count;
+        {
+            std::unique_lock<std::mutex> lock(m_);
+            wake_.wait(lock, [&] { return stopping_ || generation_ != seen; });
+            if (stopping_) return;
+            seen = generation_;
+            job = job_;
+            count = count_;
+        }
+        try {
+            for (int i; (i = next_.fetch_add(1)) < count;) (*job)(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(m_);
+            if (!error_) error_ = std::current_exception();
+            next_.store(count);
+        }
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+}
+
+void HostPool::stop()
+{
+    {
+        std::lock_guard<std::mutex> lock(m_);
+        stopping_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : threads_) t.join();
+    threads_.clear();
+    stopping_ = false;
+}
+
 void read_back(Context* ctx, void* h, const void* d, size_t bytes)
 {
     if (bytes > kMailBytes) fail(TNS_EINVAL, "read_back: more than the mailbox holds");
@@ -834,6 +913,7 @@ TNS_API int tns_context_destroy(tns_context* ctx)
     // handles stay valid and refuse further calls with TNS_EDISPOSED.
     for (tns_batch* b : ctx->batches) b->plan.reset();
     ctx->batches.clear();
+    c.host.stop();
     c.pool.release_all();  // the solve's exchange area among the blocks
     c.flow = nullptr;
     c.flow_bytes = 0;

@@ -6,11 +6,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tensile_hip.h"
@@ -60,6 +65,33 @@ private:
     size_t held_ = 0;
     size_t limit_;
     size_t retain_ = ~size_t(0);
+};
+
+// Host threads a context keeps for the estimators' host steps (many
+// independent runs advanced in lockstep between device products): run()
+// hands out the indices [0, count) to the caller and the workers and returns
+// when all are done, rethrowing the first exception. Started on first use,
+// stopped with the context; one run() at a time (the context's lock).
+class HostPool {
+public:
+    HostPool() = default;
+    HostPool(const HostPool&) = delete;
+    HostPool& operator=(const HostPool&) = delete;
+    ~HostPool() { stop(); }
+    void run(int count, const std::function<void(int)>& fn);
+    void stop();
+
+private:
+    void work();
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable wake_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    int count_ = 0, busy_ = 0;
+    unsigned long long generation_ = 0;
+    bool stopping_ = false;
+    std::atomic<int> next_{0};
+    std::exception_ptr error_;
 };
 
 struct Context;
@@ -138,6 +170,7 @@ struct Context {
     unsigned flow_seq = 0;
     bool capturing = false;  // a graph capture is recording `stream`
     bool no_alloc = false;   // inside the no-allocation entry point
+    HostPool host;           // host threads for the estimators' lockstep steps
 };
 
 // Points ctx->stream at s for the guard's life: the helpers launch on
