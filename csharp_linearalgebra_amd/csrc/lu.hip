@@ -1240,7 +1240,8 @@ __global__ void __launch_bounds__(1024) next_panel_update_kernel(int m2, int w, 
     constexpr int TN = W / 16;  // 16-column blocks of the widest update
     __shared__ T Ls[W][W];      // Ls[k][i] = L(i, k) below the diagonal
     __shared__ T Us[W][W + 1];  // Us[c][k] = U12(k, c)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // The wavefront index as a uniform value (scalar loop control below).
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int e = tid; e < W * W; e += 64 * NWAVE) {
         const int i = e % W, k = e / W;
         Ls[k][i] = load_if(pa + i + (size_t)k * lda, i < w && k < w && i > k);
@@ -1253,10 +1254,9 @@ __global__ void __launch_bounds__(1024) next_panel_update_kernel(int m2, int w, 
         for (int k = 0; k < W - 1; k++) {
             if (k + 1 >= w) break;
             const T xk = lane_value(x, k);
-            if (!s_iszero(xk)) {
-                const T d = Ls[k][lane < W ? lane : 0];
-                if (lane > k && live) x = s_add(x, s_mul(s_neg(xk), d));
-            }
+            // A zero x_k leaves the column as it is (selected).
+            const T d = Ls[k][lane < W ? lane : 0];
+            x = s_sel(lane > k && live && !s_iszero(xk), s_add(x, s_mul(s_neg(xk), d)), x);
         }
         if (live) Us[c][lane] = x;
     }
@@ -1275,9 +1275,25 @@ __global__ void __launch_bounds__(1024) next_panel_update_kernel(int m2, int w, 
     const int kq = lane >> 4, col_in = lane & 15;
     const int arow = r0 + col_in;
     v4d_t re = v4d_t{0.0, 0.0, 0.0, 0.0}, im = v4d_t{0.0, 0.0, 0.0, 0.0};
-    for (int kk = 0; kk < w; kk += 4) {
-        const int k = kk + kq;
-        const T av = load_if(l21 + (arow < m2 ? arow : 0) + (size_t)(k < w ? k : 0) * lda, arow < m2 && k < w);
+    // This lane's entries of L21, all loaded before the products (indexes
+    // clamped into the panel; entries past it zeroed by selection): a
+    // guarded load per step waited for its data, one memory latency a step.
+    constexpr int KQ = W / 4;
+    T avs[KQ];
+    {
+        const int ar = arow < m2 ? arow : m2 - 1;
+#pragma unroll
+        for (int q = 0; q < KQ; q++) {
+            const int k = 4 * q + kq;
+            avs[q] = l21[ar + (size_t)(k < w ? k : w - 1) * lda];
+        }
+    }
+    const int kqs = (w + 3) / 4;
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+        if (q >= kqs) break;
+        const int k = 4 * q + kq;
+        const T av = s_sel(arow < m2 && k < w, avs[q], s_zero(T{}));
         const T bv = Us[16 * j + col_in][k < W ? k : 0];
         if constexpr (CPLX) {
             const double a_r = av.x, a_i = av.y, a_n = -a_i;
@@ -1293,16 +1309,22 @@ __global__ void __launch_bounds__(1024) next_panel_update_kernel(int m2, int w, 
     // 4r + l/16, column l%16 of the block.
     const int col = 16 * j + col_in;
     if (col >= nc) return;
+    // The four old entries of C first (all loads in flight), then the stores.
+    T cv[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int row = r0 + 4 * r + kq;
+        cv[r] = a22[(row < m2 ? row : m2 - 1) + (size_t)col * lda];
+    }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int row = r0 + 4 * r + kq;
         if (row >= m2) continue;
         T* p = a22 + row + (size_t)col * lda;
         if constexpr (CPLX) {
-            const T c0 = *p;
-            *p = make_zd(c0.x + (-re[r]), c0.y + (-im[r]));
+            *p = make_zd(cv[r].x + (-re[r]), cv[r].y + (-im[r]));
         } else {
-            *p = *p + (-re[r]);
+            *p = cv[r] + (-re[r]);
         }
     }
 }
