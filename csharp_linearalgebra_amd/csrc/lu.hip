@@ -23,10 +23,13 @@
 //    payload moved as (destination, source) pairs, and one kernel applies
 //    that list to every matrix column outside the kernel's columns: a gather
 //    of at most 2w rows per column instead of w dependent swaps.
-//  * Triangular solves with few right-hand sides run one workgroup per
-//    column: 64-row diagonal blocks solved by one wavefront with shuffles,
-//    the rest updated by the whole workgroup. Many right-hand sides (the
-//    Pade denominator's n) use blocked substitution: each diagonal block
+//  * Triangular solves with few right-hand sides run in one launch
+//    (trsv_flow_kernel): chains of workgroups on one XCD, each owning
+//    64-row diagonal blocks, streaming the triangle's tiles ahead and
+//    exchanging only the solved x blocks through the L2. Where that launch
+//    cannot run, one workgroup per column (or one launch per diagonal
+//    block at large orders). Many right-hand sides (the Pade denominator's
+//    n) use blocked substitution: each diagonal block
 //    solved by workgroups of 32 columns with the block in LDS (the unit
 //    lower one, the LU's U12 solve, by one wavefront per column with x_k
 //    broadcast by readlane), the off-diagonal update as a GEMM.
@@ -2566,9 +2569,11 @@ void trsm_diag(Context* ctx, int w, int ncols, const T* A, int lda, T* B, int ld
 }
 
 // op(A) X = B in place, A n x n triangular.
-// Order from which few-right-hand-side solves take trsv_step_kernel (one
-// launch per diagonal block, the triangle read by the whole chip) instead of
-// one workgroup per column. Measured at 1024 (2 and 16 right-hand sides):
+// Few right-hand sides take trsv_flow_kernel (one launch) wherever it can
+// run; where it cannot (graph capture, the LU's second stream, the
+// no-allocation entry point), this is the order from which they take
+// trsv_step_kernel (one launch per diagonal block, the triangle read by the
+// whole chip) instead of one workgroup per column. Measured at 1024 (2 and 16 right-hand sides):
 // the 16 dependent launches cost more than one workgroup streams (gecon
 // 2.1 -> 2.8 ms); at 8192 the step form is 5x faster.
 constexpr int TRSV_STEP_FROM = 2048;
