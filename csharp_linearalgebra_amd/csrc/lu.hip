@@ -1693,11 +1693,10 @@ __global__ void __launch_bounds__(256) trsv_step_kernel(int n, const T* __restri
 // solves the diagonal block for its column by readlane substitution
 // (trsv_lds_kernel's steps), publishes x as sequence-tagged words (the
 // launch's own number, so a word left by an earlier launch is never taken
-// for current) and stores x to B. A reader polls one word of the block with
-// one lane (every lane polling its own words kept the L2 lines of the
-// newest block busy), then reads its words once, each checked for the
-// sequence number and waited for if not yet current. Only x crosses
-// workgroups, and
+// for current) and stores x to B. A reader's lanes poll their own words
+// until they carry the sequence number, so one round trip both sees a
+// block arrive and delivers it (one lane polling, then every lane reading,
+// took two: 4.65 -> 4.47 us a block). Only x crosses workgroups, and
 // the triangle is read once, by G workgroups at a time instead of one per
 // column: the chain's time is the diagonal solves and one L2 hop a block.
 // The G <= FLOW_MAXG workgroups of a chain fit on one XCD together (two per
@@ -1853,35 +1852,18 @@ trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int
         }
     };
     // x of block sb for this wavefront's NC columns, lane l holding entry l
-    // (zero past the block): one lane polls the block's last word, then every
-    // lane reads its words once, and a word not yet current is waited for.
-    // False if a wait gave up.
+    // (zero past the block): every lane polls its own words until they carry
+    // the launch's sequence number, so the poll that sees them current also
+    // delivers them. False if a wait gave up.
     auto fetch_x = [&](auto nc_, int sb, int kbs, T (&xv)[NCW]) __attribute__((always_inline)) -> bool {
         constexpr int NC = decltype(nc_)::value;
         const unsigned long long* xs = slots + (size_t)sb * slot_words + (size_t)cw0 * BS * WPE;
         bool ok = true;
-        if (lane == 0) {
-            unsigned f[1];
-            ok = xwait<1>(xs + ((size_t)(NC - 1) * BS + kbs - 1) * WPE + WPE - 1, seq, abort_word, f);
-        }
-        if (!__all(ok)) return false;
-        const int lc = lane < kbs ? lane : kbs - 1;
-        unsigned long long xw[NC][WPE];
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-#pragma unroll
-            for (int q = 0; q < WPE; q++) xw[c][q] = xload(xs + ((size_t)c * BS + lc) * WPE + q);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             unsigned w[WPE];
-            bool cur = true;
-#pragma unroll
-            for (int q = 0; q < WPE; q++) {
-                w[q] = (unsigned)xw[c][q];
-                cur = cur && (unsigned)(xw[c][q] >> 32) == seq;
-            }
-            if (!cur) ok = xwait<WPE>(xs + ((size_t)c * BS + lc) * WPE, seq, abort_word, w);
-            xv[c] = s_sel(lane < kbs, from_words<T>(w), s_zero(T{}));
+            if (lane < kbs) ok = ok && xwait<WPE>(xs + ((size_t)c * BS + lane) * WPE, seq, abort_word, w);
+            xv[c] = s_sel(lane < kbs && ok, from_words<T>(w), s_zero(T{}));
         }
         return __all(ok);
     };
