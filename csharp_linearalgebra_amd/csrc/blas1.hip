@@ -358,7 +358,21 @@ __global__ void __launch_bounds__(256) rowvec_kernel(int n, const double* __rest
     if (j >= n) return;
     const double* col = mags + (size_t)j * n;
     double s = 0.0;
-    for (int i = lane; i < n; i += 64) s += row[i] * col[i];
+    // Sixteen rows' loads in flight at once (past the end a lane reloads
+    // its first row and skips the add), added in the same order as one at a
+    // time.
+    for (int i = lane; i < n; i += 16 * 64) {
+        double r[16], c[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int k = i + 64 * u < n ? i + 64 * u : i;
+            r[u] = row[k];
+            c[u] = col[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            if (i + 64 * u < n) s += r[u] * c[u];
+    }
     s = wave_sum(s);
     if (lane == 0) next[j] = s;
 }
