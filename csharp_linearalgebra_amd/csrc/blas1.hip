@@ -114,46 +114,60 @@ __global__ void __launch_bounds__(256) vec_finish_kernel(int n, const double* __
 // ---------------------------------------------------------------------------
 constexpr int PN_ROWS = 64;   // rows per workgroup
 constexpr int PN_KL = 4;      // k-lanes per workgroup (one wavefront each)
+constexpr int PN_SUB = 64;    // columns of the chunk staged at a time
+constexpr int PN_PER = PN_SUB / PN_KL;  // a wavefront's columns per stage
 
 // Y = A X, split over k: part[ks][c][i] = sum over the ks-th column chunk.
-// A wavefront covers 64 consecutive rows of one column of A at a time
-// (coalesced); X's row j is uniform across the wavefront.
+// Wavefront kl takes columns j0 + kl, j0 + kl + 4, ... of the chunk, its
+// lanes 64 consecutive rows (coalesced), and adds in that column order.
+// The chunk is taken 64 columns at a time: the wavefront's 16 columns of A
+// are loaded at once while the workgroup copies X's 64 rows to LDS, from
+// where every lane reads the same entry (a broadcast).
 // FULL: t == TMAX, so no column test guards a load.
 template <typename T, int TMAX, bool FULL>
 __device__ __forceinline__ void panel_n_body(int n, int t, const T* __restrict__ a, int lda, const T* __restrict__ x,
-                                             int ldx, int kchunk, T* __restrict__ part, T (&red)[PN_KL][TMAX][PN_ROWS])
+                                             int ldx, int kchunk, T* __restrict__ part, T (&red)[PN_KL][TMAX][PN_ROWS],
+                                             T (&xs)[TMAX][PN_SUB])
 {
-    const int lane = threadIdx.x & 63, kl = threadIdx.x >> 6;
+    constexpr int XPER = (TMAX * PN_SUB + 255) / 256;  // X entries a thread stages
+    const int lane = threadIdx.x & 63;
+    const int kl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = blockIdx.x * PN_ROWS + lane;
+    const int ir = i < n ? i : n - 1;  // rows past the end load row n - 1; never stored
     const int ks = blockIdx.y;
     const int j0 = ks * kchunk, j1 = j0 + kchunk < n ? j0 + kchunk : n;
     T acc[TMAX];
 #pragma unroll
     for (int c = 0; c < TMAX; c++) acc[c] = s_zero(T{});
-    if (i < n) {
-        // Eight columns' loads in flight at once, then their products in
-        // the same order as one at a time.
-        int j = j0 + kl;
-        for (; j + 7 * PN_KL < j1; j += 8 * PN_KL) {
-            T av[8], xv[8][TMAX];
+    for (int s0 = j0; s0 < j1; s0 += PN_SUB) {
+        const int cnt = j1 - s0 < PN_SUB ? j1 - s0 : PN_SUB;
+        T xv[XPER];
 #pragma unroll
-            for (int u = 0; u < 8; u++) av[u] = a[i + (size_t)(j + u * PN_KL) * lda];
+        for (int q = 0; q < XPER; q++) {
+            const int e = threadIdx.x + 256 * q, c = e / PN_SUB, jj = e % PN_SUB;
+            xv[q] = e < TMAX * PN_SUB && jj < cnt && (FULL || c < t) ? x[s0 + jj + (size_t)c * ldx] : s_zero(T{});
+        }
+        T av[PN_PER];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+        for (int u = 0; u < PN_PER; u++) {
+            const int jj = kl + PN_KL * u;
+            av[u] = a[ir + (size_t)(s0 + (jj < cnt ? jj : 0)) * lda];
+        }
+        __syncthreads();  // the previous stage's readers are done with xs
 #pragma unroll
-                for (int c = 0; c < TMAX; c++) xv[u][c] = FULL ? x[j + u * PN_KL + (size_t)c * ldx] : s_zero(T{});
+        for (int q = 0; q < XPER; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < TMAX * PN_SUB) xs[e / PN_SUB][e % PN_SUB] = xv[q];
+        }
+        __syncthreads();
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+        for (int u = 0; u < PN_PER; u++) {
+            const int jj = kl + PN_KL * u;
+            if (jj < cnt) {
 #pragma unroll
                 for (int c = 0; c < TMAX; c++)
-                    if (FULL || c < t)
-                        acc[c] = s_add(acc[c], s_mul(av[u], FULL ? xv[u][c] : x[j + u * PN_KL + (size_t)c * ldx]));
-        }
-        for (; j < j1; j += PN_KL) {
-            const T av = a[i + (size_t)j * lda];
-#pragma unroll
-            for (int c = 0; c < TMAX; c++)
-                if (FULL || c < t) acc[c] = s_add(acc[c], s_mul(av, x[j + (size_t)c * ldx]));
+                    if (FULL || c < t) acc[c] = s_add(acc[c], s_mul(av[u], xs[c][jj]));
+            }
         }
     }
 #pragma unroll
@@ -181,10 +195,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
 {
     if (skip != nullptr && *skip) return;  // a finished Taylor stage: no work
     __shared__ T red[PN_KL][TMAX][PN_ROWS];
+    __shared__ T xs[TMAX][PN_SUB];
     if (t == TMAX)
-        panel_n_body<T, TMAX, true>(n, t, a, lda, x, ldx, kchunk, part, red);
+        panel_n_body<T, TMAX, true>(n, t, a, lda, x, ldx, kchunk, part, red, xs);
     else
-        panel_n_body<T, TMAX, false>(n, t, a, lda, x, ldx, kchunk, part, red);
+        panel_n_body<T, TMAX, false>(n, t, a, lda, x, ldx, kchunk, part, red, xs);
 }
 
 template <typename T>
@@ -210,51 +225,79 @@ __global__ void __launch_bounds__(256) panel_finish_kernel(int n, int t, int ksp
     y[i + (size_t)c * ldy] = s;
 }
 
-// Y = op(A)^T-form: y_j = sum_i conj?(a_ij) x_i, one wavefront per column j.
+// Rows a lane takes per stage of the transposed product: X's stage
+// (64 PT_R rows x TMAX columns) stays within 64 KB of LDS.
+template <typename T, int TMAX>
+constexpr int pt_rows()
+{
+    constexpr int cap = sizeof(T) == 8 ? 16 : 8;
+    constexpr int fit = 65536 / (64 * TMAX * (int)sizeof(T));
+    return fit < cap ? fit : cap;
+}
+
+// Y = op(A)^T-form: y_j = sum_i conj?(a_ij) x_i, one wavefront per column j,
+// four columns to a workgroup. Lane l adds rows l, l + 64, ... in that
+// order, then the wavefront sums its lanes. The rows are taken 64 R at a
+// time: the workgroup copies X's rows of the stage to LDS, shared by its
+// four columns, and each lane's R loads of A for the next stage are issued
+// before the current stage's products.
 // FULL: t == TMAX, so no column test guards a load.
 template <typename T, int TMAX, bool CONJ, bool FULL>
 __device__ __forceinline__ void panel_t_body(int n, int t, const T* __restrict__ a, int lda, const T* __restrict__ x,
-                                             int ldx, T* __restrict__ y, int ldy)
+                                             int ldx, T* __restrict__ y, int ldy, T (&xs)[TMAX][64 * pt_rows<T, TMAX>()])
 {
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    constexpr int R = pt_rows<T, TMAX>(), RC = 64 * R;
+    constexpr int XPER = TMAX * RC / 256;  // X entries a thread stages
+    const int j = blockIdx.x * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (j >= n) return;
-    const T* col = a + (size_t)j * lda;
+    // A workgroup's columns past the end read column n - 1 and store nothing
+    // (every wavefront takes part in the stages' barriers).
+    const T* col = a + (size_t)(j < n ? j : n - 1) * lda;
     T acc[TMAX];
 #pragma unroll
     for (int c = 0; c < TMAX; c++) acc[c] = s_zero(T{});
-    // Eight rows' loads in flight at once, then their products in the same
-    // order as one at a time.
-    int i = lane;
-    for (; i + 7 * 64 < n; i += 8 * 64) {
-        T av[8], xv[8][TMAX];
+    T av[R];
 #pragma unroll
-        for (int u = 0; u < 8; u++) av[u] = col[i + u * 64];
+    for (int u = 0; u < R; u++) av[u] = col[lane + 64 * u < n ? lane + 64 * u : 0];
+    for (int r0 = 0; r0 < n; r0 += RC) {
+        T xv[XPER];
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-#pragma unroll
-            for (int c = 0; c < TMAX; c++) xv[u][c] = FULL ? x[i + u * 64 + (size_t)c * ldx] : s_zero(T{});
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (CONJ) av[u] = s_conj(av[u]);
-#pragma unroll
-            for (int c = 0; c < TMAX; c++)
-                if (FULL || c < t)
-                    acc[c] = s_add(acc[c], s_mul(av[u], FULL ? xv[u][c] : x[i + u * 64 + (size_t)c * ldx]));
+        for (int q = 0; q < XPER; q++) {
+            const int e = threadIdx.x + 256 * q, c = e / RC, ii = r0 + e % RC;
+            xv[q] = ii < n && (FULL || c < t) ? x[ii + (size_t)c * ldx] : s_zero(T{});
         }
-    }
-    for (; i < n; i += 64) {
-        T av = col[i];
-        if (CONJ) av = s_conj(av);
+        __syncthreads();  // the previous stage's readers are done with xs
 #pragma unroll
-        for (int c = 0; c < TMAX; c++)
-            if (FULL || c < t) acc[c] = s_add(acc[c], s_mul(av, x[i + (size_t)c * ldx]));
-    }
+        for (int q = 0; q < XPER; q++) {
+            const int e = threadIdx.x + 256 * q;
+            xs[e / RC][e % RC] = xv[q];
+        }
+        __syncthreads();
+        T an[R];
 #pragma unroll
-    for (int c = 0; c < TMAX; c++) {
-        if (FULL || c < t) {
-            T s = wave_sum(acc[c]);
-            if (lane == 0) y[j + (size_t)c * ldy] = s;
+        for (int u = 0; u < R; u++) {
+            const int ii = r0 + RC + lane + 64 * u;
+            an[u] = col[ii < n ? ii : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            if (r0 + lane + 64 * u < n) {
+                const T v = CONJ ? s_conj(av[u]) : av[u];
+#pragma unroll
+                for (int c = 0; c < TMAX; c++)
+                    if (FULL || c < t) acc[c] = s_add(acc[c], s_mul(v, xs[c][lane + 64 * u]));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < R; u++) av[u] = an[u];
+    }
+    if (j < n) {
+#pragma unroll
+        for (int c = 0; c < TMAX; c++) {
+            if (FULL || c < t) {
+                T s = wave_sum(acc[c]);
+                if (lane == 0) y[j + (size_t)c * ldy] = s;
+            }
         }
     }
 }
@@ -263,10 +306,11 @@ template <typename T, int TMAX, bool CONJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) panel_t_kernel(int n, int t, const T* __restrict__ a, int lda,
                                                       const T* __restrict__ x, int ldx, T* __restrict__ y, int ldy)
 {
+    __shared__ T xs[TMAX][64 * pt_rows<T, TMAX>()];
     if (t == TMAX)
-        panel_t_body<T, TMAX, CONJ, true>(n, t, a, lda, x, ldx, y, ldy);
+        panel_t_body<T, TMAX, CONJ, true>(n, t, a, lda, x, ldx, y, ldy, xs);
     else
-        panel_t_body<T, TMAX, CONJ, false>(n, t, a, lda, x, ldx, y, ldy);
+        panel_t_body<T, TMAX, CONJ, false>(n, t, a, lda, x, ldx, y, ldy, xs);
 }
 
 // ---------------------------------------------------------------------------
