@@ -2022,10 +2022,11 @@ trsv_flow_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ B, int
 // the left-looking dot products keep its lane split and reduction, wave c
 // solves column c's diagonal block, and the right-looking update keeps its
 // per-column skip of zero entries.
-template <typename T, bool LOWER, int TR, bool UNIT, int G, typename PA>
+template <typename T, bool LOWER, int TR, bool UNIT, int G, int NT, typename PA>
 __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
 {
     constexpr int BS = tri_block<T>();
+    static_assert(NT % 64 == 0 && NT / 64 >= G, "a wavefront per right-hand side");
     static_assert(G >= 1 && G <= 4, "one wavefront per column for the diagonal blocks");
     __shared__ T part[G][BS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -2043,7 +2044,7 @@ __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
         if (TR != 0) {
             const int lo = FORWARD ? 0 : k0 + kb;
             const int hi = FORWARD ? k0 : n;
-            for (int i = wave; i < kb; i += 4) {
+            for (int i = wave; i < kb; i += NT / 64) {
                 T s0[G], s1[G];
 #pragma unroll
                 for (int c = 0; c < G; c++) s0[c] = s1[c] = zero;
@@ -2098,8 +2099,8 @@ __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
         if (TR == 0) {
             const int r0 = FORWARD ? k0 + kb : 0;
             const int r1 = FORWARD ? n : k0;
-            for (int i = r0 + tid; i < r1; i += 512) {
-                const int i2 = i + 256;
+            for (int i = r0 + tid; i < r1; i += 2 * NT) {
+                const int i2 = i + NT;
                 const bool two = i2 < r1;
                 T acc[G], acc2[G];
 #pragma unroll
@@ -2107,7 +2108,7 @@ __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
                     acc[c] = load_if(xs + c * n + i, c < nr);
                     acc2[c] = load_if(xs + c * n + i2, c < nr && two);
                 }
-                constexpr int U = 4;
+                constexpr int U = 8;
                 for (int q = 0; q < kb; q += U) {
                     T av[U], av2[U];
 #pragma unroll
@@ -2149,8 +2150,10 @@ __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
 // the way in, L forward (unit), U backward; or, for the adjoint, U^H
 // forward, L^H backward (unit) and the permutation scattered on the way
 // out. The same operations per column as getrs_impl's.
+constexpr int GS_NT = 512;  // threads of a grouped solve's workgroup
+
 template <typename T, int TRANS, int G>
-__global__ void __launch_bounds__(256) getrs_grouped_kernel(const SolveRec<T>* __restrict__ recs)
+__global__ void __launch_bounds__(GS_NT) getrs_grouped_kernel(const SolveRec<T>* __restrict__ recs)
 {
     extern __shared__ unsigned char getrs_smem[];
     T* xs = reinterpret_cast<T*>(getrs_smem);
@@ -2164,20 +2167,20 @@ __global__ void __launch_bounds__(256) getrs_grouped_kernel(const SolveRec<T>* _
     constexpr int TR = sizeof(T) == 16 ? 2 : 1;
     for (int c0 = 0; c0 < r.nrhs; c0 += G) {
         const int nr = r.nrhs - c0 < G ? r.nrhs - c0 : G;
-        for (int e = tid; e < nr * n; e += 256) {
+        for (int e = tid; e < nr * n; e += GS_NT) {
             const int c = e / n, i = e % n;
             const int src = TRANS ? i : perm[i];
             xs[c * n + i] = x[src + (size_t)(c0 + c) * r.ldx];
         }
         __syncthreads();
         if (!TRANS) {
-            trsv_lds_group<T, true, 0, true, G>(n, lu, r.ldlu, xs, nr);
-            trsv_lds_group<T, false, 0, false, G>(n, lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, true, 0, true, G, GS_NT>(n, lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, false, 0, false, G, GS_NT>(n, lu, r.ldlu, xs, nr);
         } else {
-            trsv_lds_group<T, false, TR, false, G>(n, lu, r.ldlu, xs, nr);
-            trsv_lds_group<T, true, TR, true, G>(n, lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, false, TR, false, G, GS_NT>(n, lu, r.ldlu, xs, nr);
+            trsv_lds_group<T, true, TR, true, G, GS_NT>(n, lu, r.ldlu, xs, nr);
         }
-        for (int e = tid; e < nr * n; e += 256) {
+        for (int e = tid; e < nr * n; e += GS_NT) {
             const int c = e / n, i = e % n;
             const int dst = TRANS ? perm[i] : i;
             y[dst + (size_t)(c0 + c) * r.ldy] = xs[c * n + i];
@@ -3710,7 +3713,7 @@ void zgrouped_solve(Context* ctx, const SolveRec<zd>* recs, int count, int trans
     if (count <= 0) return;
     const size_t bytes = (size_t)group * (nmax > 0 ? nmax : 1) * sizeof(zd);
 #define TNS_GS(TRANS_, G_)                                                                                         \
-    hipLaunchKernelGGL((getrs_grouped_kernel<zd, TRANS_, G_>), dim3(count), dim3(256), bytes, ctx->stream, recs)
+    hipLaunchKernelGGL((getrs_grouped_kernel<zd, TRANS_, G_>), dim3(count), dim3(GS_NT), bytes, ctx->stream, recs)
     if (!trans) {
         if (group >= 4) TNS_GS(0, 4);
         else if (group == 2) TNS_GS(0, 2);
