@@ -2048,22 +2048,24 @@ __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
                 T s0[G], s1[G];
 #pragma unroll
                 for (int c = 0; c < G; c++) s0[c] = s1[c] = zero;
+                // s0 adds entries lo + lane + 128 m, s1 those 64 further on,
+                // each in increasing order. The trip count is the same for
+                // every lane (entries past hi are read at lo and not added),
+                // so the unrolled loop issues its loads together.
+                const int iters = (hi - lo + 127) / 128;
                 int k = lo + lane;
-#pragma unroll 4
-                for (; k + 64 < hi; k += 128) {
-                    const T a0 = at(k0 + i, k), a1 = at(k0 + i, k + 64);
+#pragma unroll 8
+                for (int m = 0; m < iters; m++, k += 128) {
+                    const bool v0 = k < hi, v1 = k + 64 < hi;
+                    const int ka = v0 ? k : lo, kb1 = v1 ? k + 64 : lo;
+                    const T a0 = at(k0 + i, ka), a1 = at(k0 + i, kb1);
 #pragma unroll
                     for (int c = 0; c < G; c++) {
                         if (c >= nr) continue;
-                        s0[c] = s_add(s0[c], s_mul(a0, xs[c * n + k]));
-                        s1[c] = s_add(s1[c], s_mul(a1, xs[c * n + k + 64]));
+                        const T p0 = s_mul(a0, xs[c * n + ka]), p1 = s_mul(a1, xs[c * n + kb1]);
+                        s0[c] = s_sel(v0, s_add(s0[c], p0), s0[c]);
+                        s1[c] = s_sel(v1, s_add(s1[c], p1), s1[c]);
                     }
-                }
-                if (k < hi) {
-                    const T a0 = at(k0 + i, k);
-#pragma unroll
-                    for (int c = 0; c < G; c++)
-                        if (c < nr) s0[c] = s_add(s0[c], s_mul(a0, xs[c * n + k]));
                 }
 #pragma unroll
                 for (int c = 0; c < G; c++) {
