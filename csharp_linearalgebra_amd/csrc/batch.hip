@@ -409,7 +409,11 @@ void BatchPlan::build_rcond(int t)
     d_panels = DevBuf<zd>(ctx, std::max<size_t>(1, 2 * total));
     int maxn = 0;
     for (int i : order) maxn = std::max(maxn, n[i]);
-    const int group = grouped_solve_group(ctx, maxn);
+    // Right-hand sides the grouped solve takes at a time: no more than the
+    // estimates' t (each column gets the same operations whatever the group).
+    int group = grouped_solve_group(ctx, maxn);
+    if (group > 2 && t <= 2) group = 2;
+    if (group > 1 && t <= 1) group = 1;
     auto graph = [&](Schedule& s, int trans) {
         std::vector<SolveRec<zd>> sv;
         for (int i : order) {

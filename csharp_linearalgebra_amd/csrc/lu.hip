@@ -2044,35 +2044,54 @@ __device__ void trsv_lds_group(int n, PA __restrict__ A, int lda, T* xs, int nr)
         if (TR != 0) {
             const int lo = FORWARD ? 0 : k0 + kb;
             const int hi = FORWARD ? k0 : n;
-            for (int i = wave; i < kb; i += NT / 64) {
-                T s0[G], s1[G];
+            // Wavefront w takes the block's rows w, w + NW, ... together: one
+            // pass over k with every row's loads in flight, then their
+            // reductions side by side. Per row and column, s0 adds entries
+            // lo + lane + 128 m and s1 those 64 further on, each in
+            // increasing order (entries past hi are read at lo and not
+            // added), then s0 + s1 and the xor tree over the lanes.
+            constexpr int NW = NT / 64, RW = (BS + NW - 1) / NW;
+            T s0[RW][G], s1[RW][G];
 #pragma unroll
-                for (int c = 0; c < G; c++) s0[c] = s1[c] = zero;
-                // s0 adds entries lo + lane + 128 m, s1 those 64 further on,
-                // each in increasing order. The trip count is the same for
-                // every lane (entries past hi are read at lo and not added),
-                // so the unrolled loop issues its loads together.
-                const int iters = (hi - lo + 127) / 128;
-                int k = lo + lane;
-#pragma unroll 8
-                for (int m = 0; m < iters; m++, k += 128) {
-                    const bool v0 = k < hi, v1 = k + 64 < hi;
-                    const int ka = v0 ? k : lo, kb1 = v1 ? k + 64 : lo;
-                    const T a0 = at(k0 + i, ka), a1 = at(k0 + i, kb1);
+            for (int r = 0; r < RW; r++)
 #pragma unroll
-                    for (int c = 0; c < G; c++) {
-                        if (c >= nr) continue;
-                        const T p0 = s_mul(a0, xs[c * n + ka]), p1 = s_mul(a1, xs[c * n + kb1]);
-                        s0[c] = s_sel(v0, s_add(s0[c], p0), s0[c]);
-                        s1[c] = s_sel(v1, s_add(s1[c], p1), s1[c]);
-                    }
+                for (int c = 0; c < G; c++) s0[r][c] = s1[r][c] = zero;
+            const int iters = (hi - lo + 127) / 128;
+            int k = lo + lane;
+            // Two passes' loads in flight where the accumulators leave room.
+            constexpr int UNR = G <= 2 ? 2 : 1;
+#pragma unroll UNR
+            for (int m = 0; m < iters; m++, k += 128) {
+                const bool v0 = k < hi, v1 = k + 64 < hi;
+                const int ka = v0 ? k : lo, kb1 = v1 ? k + 64 : lo;
+                T a0[RW], a1[RW];
+#pragma unroll
+                for (int r = 0; r < RW; r++) {
+                    const int i = wave + NW * r;
+                    const int ir = i < kb ? i : 0;
+                    a0[r] = at(k0 + ir, ka);
+                    a1[r] = at(k0 + ir, kb1);
                 }
 #pragma unroll
                 for (int c = 0; c < G; c++) {
-                    T sum = s_add(s0[c], s1[c]);
+                    if (c >= nr) continue;
+                    const T x0 = xs[c * n + ka], x1 = xs[c * n + kb1];
+#pragma unroll
+                    for (int r = 0; r < RW; r++) {
+                        s0[r][c] = s_sel(v0, s_add(s0[r][c], s_mul(a0[r], x0)), s0[r][c]);
+                        s1[r][c] = s_sel(v1, s_add(s1[r][c], s_mul(a1[r], x1)), s1[r][c]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RW; r++) {
+                const int i = wave + NW * r;
+#pragma unroll
+                for (int c = 0; c < G; c++) {
+                    T sum = s_add(s0[r][c], s1[r][c]);
 #pragma unroll
                     for (int off = 32; off >= 1; off >>= 1) sum = s_add(sum, shfl_xor_t(sum, off));
-                    if (lane == 0) part[c][i] = sum;
+                    if (lane == 0 && i < kb && c < nr) part[c][i] = sum;
                 }
             }
             __syncthreads();
