@@ -225,8 +225,9 @@ __global__ void __launch_bounds__(256) panel_finish_kernel(int n, int t, int ksp
     y[i + (size_t)c * ldy] = s;
 }
 
-// Rows a lane takes per stage of the transposed product: X's stage
-// (64 PT_R rows x TMAX columns) stays within 64 KB of LDS.
+// Rows a lane takes per stage of the transposed product (R, at most 16
+// real or 8 complex): X's stage, 64 R rows x TMAX columns, stays within
+// 64 KB of LDS.
 template <typename T, int TMAX>
 constexpr int pt_rows()
 {
