@@ -41,7 +41,10 @@ def test_norm_of_empty_and_nan(ws):
     assert np.isnan(ws.one_norm(a)) and np.isnan(ws.infinity_norm(a))
 
 
-@pytest.mark.parametrize("n,t", [(1, 1), (50, 2), (1000, 2), (1000, 16), (777, 5), (300, 40)])
+# 3000 x 3 and 2049 x 8 take X through LDS in several stages, the last one
+# partial (column chunks of 273 and 129 in stages of 64; row stages of 512 or
+# 1024, the last of 2049 one row long).
+@pytest.mark.parametrize("n,t", [(1, 1), (50, 2), (1000, 2), (1000, 16), (777, 5), (300, 40), (3000, 3), (2049, 8)])
 def test_panel_products(ws, rng, n, t):
     a = rng.uniform(-1, 1, (n, n))
     x = rng.uniform(-1, 1, (n, t))
