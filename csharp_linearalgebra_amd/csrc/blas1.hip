@@ -21,12 +21,7 @@ __device__ __forceinline__ double nan_max(double a, double b)
     return a > b ? a : b;
 }
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_xor_sum(v); }
 
 __device__ __forceinline__ double wave_max(double v)
 {

@@ -161,43 +161,6 @@ __device__ __forceinline__ zd shfl_xor_t(zd v, int off)
 __device__ __forceinline__ unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double double_of(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// The value lane ^ OFF holds, as shfl_xor_t gives it. OFF 1 and 2: a DPP
-// quad permutation; 4 and 8: DPP row shifts both ways inside a row of 16
-// lanes (row_shl gives lane L the value of L + OFF, row_shr that of L - OFF;
-// a lane takes the one whose bit OFF it lacks or has); 16 and 32 cross rows
-// and go through the LDS crossbar. The DPP forms run on the VALU, so a
-// workgroup's reductions do not queue on the CU's one LDS unit.
-template <int OFF>
-__device__ __forceinline__ unsigned xor_lane_u32(unsigned v)
-{
-    static_assert(OFF == 1 || OFF == 2 || OFF == 4 || OFF == 8, "inside a row of 16 lanes");
-    if constexpr (OFF == 1) {
-        return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
-    } else if constexpr (OFF == 2) {
-        return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
-    } else {
-        const unsigned up = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + OFF, 0xF, 0xF, false);
-        const unsigned dn = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + OFF, 0xF, 0xF, false);
-        return (__lane_id() & OFF) ? dn : up;
-    }
-}
-template <int OFF>
-__device__ __forceinline__ double xor_lane(double v)
-{
-    if constexpr (OFF >= 16) {
-        return shfl_xor_t(v, OFF);
-    } else {
-        const unsigned long long b = bits_of(v);
-        const unsigned lo = xor_lane_u32<OFF>((unsigned)b), hi = xor_lane_u32<OFF>((unsigned)(b >> 32));
-        return double_of(((unsigned long long)hi << 32) | lo);
-    }
-}
-template <int OFF>
-__device__ __forceinline__ zd xor_lane(zd v)
-{
-    return make_zd(xor_lane<OFF>(v.x), xor_lane<OFF>(v.y));
-}
-
 template <int N>
 __device__ __forceinline__ double pick(const double (&v)[N], int c)
 {
